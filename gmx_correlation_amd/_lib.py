@@ -1,0 +1,99 @@
+"""ctypes binding of libgmxcorr_hip.so (include/gmxcorr.h).
+
+There is no CPU fallback anywhere in this package: if the library is missing
+or has no usable gfx950 device, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libgmxcorr_hip.so")
+
+GMXCORR_OK = 0
+STATUS_NAMES = {-1: "EINVAL", -2: "ENODEV", -3: "ENOMEM", -4: "EHIP", -5: "EUNSUP"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u8p = ctypes.POINTER(ctypes.c_ubyte)
+_i = ctypes.c_int
+_ll = ctypes.c_longlong
+
+
+class KsgStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_load", ctypes.c_double),
+        ("ms_build", ctypes.c_double),
+        ("ms_pairs", ctypes.c_double),
+        ("ms_fixup", ctypes.c_double),
+        ("npairs", _ll),
+        ("fixup_rows", _ll),
+        ("ties", _ll),
+        ("row_bytes", _ll),
+        ("nframes", _i),
+        ("k", _i),
+        ("natoms", _i),
+        ("pitch", _i),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GmxcorrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gmxcorr {STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+# Symbols declared in include/gmxcorr.h and include/kraskov_gpu.h.
+C_SYMBOLS = [
+    "gmxcorr_last_error", "gmxcorr_probe", "gmxcorr_ksg_max_frames", "gmxcorr_ksg_matrix",
+    "gmxcorr_ksg_plan_create", "gmxcorr_ksg_plan_load", "gmxcorr_ksg_plan_run_all",
+    "gmxcorr_ksg_plan_run_pairs", "gmxcorr_ksg_plan_result", "gmxcorr_ksg_plan_matrix",
+    "gmxcorr_ksg_plan_stats", "gmxcorr_ksg_plan_destroy",
+]
+CXX_SYMBOLS = ["_Z17kraskov_gpu_probev", "_Z22kraskov_corrmatrix_gpuPK9t_kraskoviPdi"]
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the in-tree library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run gmx_correlation_amd.build.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    L.gmxcorr_last_error.restype = ctypes.c_char_p
+    L.gmxcorr_probe.restype = _i
+    L.gmxcorr_ksg_max_frames.restype = _i
+    L.gmxcorr_ksg_matrix.argtypes = [_dp, _i, _i, _dp, _i, _dp]
+    L.gmxcorr_ksg_plan_create.argtypes = [_i, _i, _i, ctypes.POINTER(ctypes.c_void_p)]
+    L.gmxcorr_ksg_plan_load.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.gmxcorr_ksg_plan_run_all.argtypes = [ctypes.c_void_p]
+    L.gmxcorr_ksg_plan_run_pairs.argtypes = [ctypes.c_void_p, _ll, _ip, _ip, _ip, _ip, _u8p]
+    L.gmxcorr_ksg_plan_result.argtypes = [ctypes.c_void_p, _dp, _ll]
+    L.gmxcorr_ksg_plan_matrix.argtypes = [ctypes.c_void_p, _dp]
+    L.gmxcorr_ksg_plan_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(KsgStats)]
+    L.gmxcorr_ksg_plan_destroy.argtypes = [ctypes.c_void_p]
+    L.gmxcorr_ksg_plan_destroy.restype = None
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != GMXCORR_OK:
+        raise GmxcorrError(rc, load().gmxcorr_last_error().decode(errors="replace"))
+
+
+def ptr(a: np.ndarray, typ=_dp):
+    return a.ctypes.data_as(typ) if a is not None else None
+
+
+def probe() -> bool:
+    return bool(load().gmxcorr_probe())

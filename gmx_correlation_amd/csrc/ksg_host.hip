@@ -1,0 +1,365 @@
+// Host side of libgmxcorr_hip.so: device plan for the KSG matrix, the
+// extern "C" ABI (include/gmxcorr.h) and the reference's C++ GPU boundary
+// (include/kraskov_gpu.h, replacing lib/kraskov_gpu.cu:210-308).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gmxcorr.h"
+#include "kraskov_gpu.h"
+#include "ksg_internal.h"
+
+using gmxcorr::FixEntry;
+using gmxcorr::KsgArgs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where)
+{
+    return fail(e == hipErrorOutOfMemory ? GMXCORR_ENOMEM : GMXCORR_EHIP,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(expr)                                        \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);   \
+    } while (0)
+
+bool device_is_gfx950()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return false;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return false;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct gmxcorr_ksg_plan {
+    int natoms = 0, N = 0, K = 0;
+    long long P = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    double* dX = nullptr;
+    double* dPsi = nullptr;
+    float* dD = nullptr;
+    int* dA1 = nullptr;
+    int* dA2 = nullptr;
+    long long pair_cap = 0;
+    double* dPartial = nullptr;
+    double* dFixsum = nullptr;
+    double* dMI = nullptr;
+    FixEntry* dFix = nullptr;
+    unsigned fixcap = 0;
+    unsigned* dCounters = nullptr;
+    int* dRcNx = nullptr;
+    int* dRcNy = nullptr;
+    unsigned char* dRcFlag = nullptr;
+    long long rc_cap = 0;
+    double psiN_phiK = 0.0;
+    bool loaded = false;
+    bool all_pairs = false;
+    long long last_npairs = 0;
+    gmxcorr_ksg_stats st{};
+};
+
+namespace {
+
+int ensure_pairs(gmxcorr_ksg_plan* pl, long long npairs)
+{
+    if (npairs <= pl->pair_cap) return GMXCORR_OK;
+    dfree(pl->dA1); dfree(pl->dA2); dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI);
+    HIPCHK(hipMalloc(&pl->dA1, npairs * sizeof(int)));
+    HIPCHK(hipMalloc(&pl->dA2, npairs * sizeof(int)));
+    HIPCHK(hipMalloc(&pl->dPartial, npairs * sizeof(double)));
+    HIPCHK(hipMalloc(&pl->dFixsum, npairs * sizeof(double)));
+    HIPCHK(hipMalloc(&pl->dMI, npairs * sizeof(double)));
+    pl->pair_cap = npairs;
+    return GMXCORR_OK;
+}
+
+int ensure_fix(gmxcorr_ksg_plan* pl, unsigned cap)
+{
+    if (cap <= pl->fixcap) return GMXCORR_OK;
+    dfree(pl->dFix);
+    HIPCHK(hipMalloc(&pl->dFix, (size_t)cap * sizeof(FixEntry)));
+    pl->fixcap = cap;
+    return GMXCORR_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+// Pair kernel + FP64 recount + finalise on the plan stream.  If the recount
+// queue overflowed, the run is repeated with a queue large enough.
+int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail)
+{
+    KsgArgs a{};
+    a.X = pl->dX; a.D = pl->dD; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
+    a.partial = pl->dPartial; a.fixsum = pl->dFixsum; a.mi = pl->dMI;
+    a.counters = pl->dCounters;
+    a.npairs = npairs; a.P = pl->P; a.N = pl->N; a.K = pl->K; a.psiN_phiK = pl->psiN_phiK;
+    if (detail) { a.rc_nx = pl->dRcNx; a.rc_ny = pl->dRcNy; a.rc_flag = pl->dRcFlag; }
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        a.fixq = pl->dFix; a.fixcap = pl->fixcap;
+        HIPCHK(gmxcorr::launch_ksg_clear(a, pl->stream));
+        HIPCHK(hipEventRecord(pl->ev[2], pl->stream));
+        HIPCHK(gmxcorr::launch_ksg_pairs(a, pl->stream));
+        HIPCHK(hipEventRecord(pl->ev[3], pl->stream));
+        unsigned counters[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,
+                              pl->stream));
+        HIPCHK(hipStreamSynchronize(pl->stream));
+        if (counters[0] > pl->fixcap) {
+            int rc = ensure_fix(pl, counters[0] + counters[0] / 4 + 1024);
+            if (rc) return rc;
+            continue;
+        }
+        HIPCHK(gmxcorr::launch_ksg_fixup(a, counters[0], pl->stream));
+        HIPCHK(gmxcorr::launch_ksg_finalize(a, pl->stream));
+        HIPCHK(hipEventRecord(pl->ev[4], pl->stream));
+        HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,
+                              pl->stream));
+        HIPCHK(hipStreamSynchronize(pl->stream));
+        pl->st.ms_pairs = elapsed(pl->ev[2], pl->ev[3]);
+        pl->st.ms_fixup = elapsed(pl->ev[3], pl->ev[4]);
+        pl->st.npairs = npairs;
+        pl->st.fixup_rows = counters[0];
+        pl->st.ties = counters[1];
+        pl->last_npairs = npairs;
+        return GMXCORR_OK;
+    }
+    return fail(GMXCORR_EHIP, "FP64 recount queue overflowed twice");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gmxcorr_last_error(void) { return g_err.c_str(); }
+
+int gmxcorr_probe(void) { return device_is_gfx950() ? 1 : 0; }
+
+int gmxcorr_ksg_max_frames(void) { return gmxcorr::kMaxFrames; }
+
+int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** out)
+{
+    if (!out) return fail(GMXCORR_EINVAL, "plan_create: null output");
+    *out = nullptr;
+    if (natoms < 2) return fail(GMXCORR_EINVAL, "need at least 2 atoms");
+    if (k < 1 || nframes <= k) return fail(GMXCORR_EINVAL, "need 1 <= k < nframes");
+    if (nframes > gmxcorr::kMaxFrames)
+        return fail(GMXCORR_EUNSUP, "nframes above the register-resident kernel limit");
+    if (!device_is_gfx950()) return fail(GMXCORR_ENODEV, "no gfx950 device");
+    auto* pl = new gmxcorr_ksg_plan();
+    pl->natoms = natoms; pl->N = nframes; pl->K = k;
+    pl->P = ((long long)nframes + 63) / 64 * 64;
+    auto bail = [&](int rc) { gmxcorr_ksg_plan_destroy(pl); return rc; };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamCreate"));
+    for (auto& ev : pl->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
+    const size_t xbytes = (size_t)3 * natoms * nframes * sizeof(double);
+    const size_t dbytes = (size_t)natoms * nframes * pl->P * sizeof(float);
+    if ((e = hipMalloc(&pl->dX, xbytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc(X)"));
+    if ((e = hipMalloc(&pl->dPsi, (nframes + 1) * sizeof(double))) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc(psi)"));
+    if ((e = hipMalloc(&pl->dD, dbytes)) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc(distance rows)"));
+    if ((e = hipMalloc(&pl->dCounters, 4 * sizeof(unsigned))) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc(counters)"));
+    int rc = ensure_fix(pl, 65536);
+    if (rc) return bail(rc);
+    pl->st.row_bytes = (long long)dbytes;
+    pl->st.nframes = nframes; pl->st.k = k; pl->st.natoms = natoms; pl->st.pitch = (int)pl->P;
+    *out = pl;
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const double* psi)
+{
+    if (!pl || !xrows || !psi) return fail(GMXCORR_EINVAL, "plan_load: null argument");
+    const size_t nx = (size_t)3 * pl->natoms * pl->N;
+    for (size_t t = 0; t < nx; ++t)
+        if (!std::isfinite(xrows[t]))
+            return fail(GMXCORR_EINVAL,
+                        "non-finite normalised coordinate (constant component: zero SD)");
+    HIPCHK(hipEventRecord(pl->ev[0], pl->stream));
+    HIPCHK(hipMemcpyAsync(pl->dX, xrows, nx * sizeof(double), hipMemcpyHostToDevice, pl->stream));
+    HIPCHK(hipMemcpyAsync(pl->dPsi, psi, (pl->N + 1) * sizeof(double), hipMemcpyHostToDevice,
+                          pl->stream));
+    HIPCHK(hipEventRecord(pl->ev[1], pl->stream));
+    HIPCHK(gmxcorr::launch_build_rows(pl->dX, pl->natoms, pl->N, pl->P, pl->dD, pl->stream));
+    HIPCHK(hipEventRecord(pl->ev[5], pl->stream));
+    HIPCHK(hipStreamSynchronize(pl->stream));
+    pl->st.ms_load = elapsed(pl->ev[0], pl->ev[1]);
+    pl->st.ms_build = elapsed(pl->ev[1], pl->ev[5]);
+    pl->psiN_phiK = psi[pl->N] + (psi[pl->K] - 1 / (1.0 * pl->K));
+    pl->loaded = true;
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* pl)
+{
+    if (!pl || !pl->loaded) return fail(GMXCORR_EINVAL, "plan not loaded");
+    const long long n = pl->natoms;
+    const long long npairs = n * (n - 1) / 2;
+    if (!pl->all_pairs || pl->pair_cap < npairs) {
+        int rc = ensure_pairs(pl, npairs);
+        if (rc) return rc;
+        std::vector<int> h1(npairs), h2(npairs);
+        long long idx = 0;
+        for (int a1 = 1; a1 < n; ++a1)
+            for (int a2 = 0; a2 < a1; ++a2, ++idx) { h1[idx] = a1; h2[idx] = a2; }
+        HIPCHK(hipMemcpy(pl->dA1, h1.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(pl->dA2, h2.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
+        pl->all_pairs = true;
+    }
+    return run_loaded(pl, npairs, false);
+}
+
+int gmxcorr_ksg_plan_run_pairs(gmxcorr_ksg_plan* pl, long long npairs, const int* a1,
+                               const int* a2, int* nx, int* ny, unsigned char* flags)
+{
+    if (!pl || !pl->loaded) return fail(GMXCORR_EINVAL, "plan not loaded");
+    if (npairs <= 0 || !a1 || !a2) return fail(GMXCORR_EINVAL, "empty pair list");
+    for (long long p = 0; p < npairs; ++p)
+        if (a1[p] < 0 || a2[p] < 0 || a1[p] >= pl->natoms || a2[p] >= pl->natoms || a1[p] == a2[p])
+            return fail(GMXCORR_EINVAL, "pair index out of range or a1 == a2");
+    const bool detail = nx || ny || flags;
+    int rc = ensure_pairs(pl, npairs);
+    if (rc) return rc;
+    pl->all_pairs = false;
+    HIPCHK(hipMemcpy(pl->dA1, a1, npairs * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pl->dA2, a2, npairs * sizeof(int), hipMemcpyHostToDevice));
+    const long long rows = npairs * pl->N;
+    if (detail && rows > pl->rc_cap) {
+        dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
+        HIPCHK(hipMalloc(&pl->dRcNx, rows * sizeof(int)));
+        HIPCHK(hipMalloc(&pl->dRcNy, rows * sizeof(int)));
+        HIPCHK(hipMalloc(&pl->dRcFlag, rows));
+        pl->rc_cap = rows;
+    }
+    rc = run_loaded(pl, npairs, detail);
+    if (rc) return rc;
+    if (nx) HIPCHK(hipMemcpy(nx, pl->dRcNx, rows * sizeof(int), hipMemcpyDeviceToHost));
+    if (ny) HIPCHK(hipMemcpy(ny, pl->dRcNy, rows * sizeof(int), hipMemcpyDeviceToHost));
+    if (flags) HIPCHK(hipMemcpy(flags, pl->dRcFlag, rows, hipMemcpyDeviceToHost));
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_plan_result(gmxcorr_ksg_plan* pl, double* mi, long long npairs)
+{
+    if (!pl || !mi) return fail(GMXCORR_EINVAL, "plan_result: null argument");
+    if (npairs != pl->last_npairs) return fail(GMXCORR_EINVAL, "pair count differs from last run");
+    HIPCHK(hipMemcpy(mi, pl->dMI, npairs * sizeof(double), hipMemcpyDeviceToHost));
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_plan_matrix(gmxcorr_ksg_plan* pl, double* mat)
+{
+    if (!pl || !mat) return fail(GMXCORR_EINVAL, "plan_matrix: null argument");
+    const int n = pl->natoms;
+    const long long npairs = (long long)n * (n - 1) / 2;
+    if (!pl->all_pairs || pl->last_npairs != npairs)
+        return fail(GMXCORR_EINVAL, "last run was not run_all");
+    std::vector<double> mi(npairs);
+    HIPCHK(hipMemcpy(mi.data(), pl->dMI, npairs * sizeof(double), hipMemcpyDeviceToHost));
+    // diagonal 2000 and symmetric fill, as lib/kraskov_gpu.cu:294-303
+    for (int i = 0; i < n; ++i) mat[(size_t)i * n + i] = 2000.0;
+    long long idx = 0;
+    for (int a1 = 1; a1 < n; ++a1)
+        for (int a2 = 0; a2 < a1; ++a2, ++idx)
+            mat[(size_t)a1 * n + a2] = mat[(size_t)a2 * n + a1] = mi[idx];
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* pl, gmxcorr_ksg_stats* st)
+{
+    if (!pl || !st) return fail(GMXCORR_EINVAL, "plan_stats: null argument");
+    *st = pl->st;
+    return GMXCORR_OK;
+}
+
+void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* pl)
+{
+    if (!pl) return;
+    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dA1); dfree(pl->dA2);
+    dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix);
+    dfree(pl->dCounters); dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
+    for (auto& ev : pl->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (pl->stream) (void)hipStreamDestroy(pl->stream);
+    delete pl;
+}
+
+int gmxcorr_ksg_matrix(const double* xrows, int natoms, int nframes, const double* psi, int k,
+                       double* mat)
+{
+    if (!xrows || !psi || !mat) return fail(GMXCORR_EINVAL, "ksg_matrix: null argument");
+    gmxcorr_ksg_plan* pl = nullptr;
+    int rc = gmxcorr_ksg_plan_create(natoms, nframes, k, &pl);
+    if (rc) return rc;
+    rc = gmxcorr_ksg_plan_load(pl, xrows, psi);
+    if (!rc) rc = gmxcorr_ksg_plan_run_all(pl);
+    if (!rc) rc = gmxcorr_ksg_plan_matrix(pl, mat);
+    gmxcorr_ksg_plan_destroy(pl);
+    return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The reference's C++ boundary (lib/kraskov_gpu.h:20,32).
+// ---------------------------------------------------------------------------
+bool kraskov_gpu_probe() { return device_is_gfx950(); }
+
+void kraskov_corrmatrix_gpu(const t_kraskov* kr, int natoms, double* mat, int k)
+{
+    const int N = kr->N;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
+            std::fprintf(stderr, "GPU: %s  (%s, %.0f MB)\n", prop.name, prop.gcnArchName,
+                         prop.totalGlobalMem / 1e6);
+    }
+    // flatten the 3*natoms separately allocated rows (lib/kraskov_gpu.cu:240-246)
+    std::vector<double> rows((size_t)3 * natoms * N);
+    for (int c = 0; c < 3 * natoms; ++c)
+        std::memcpy(rows.data() + (size_t)c * N, kr->x[c], N * sizeof(double));
+    std::fprintf(stderr, "GPU Kraskov (HIP gfx950): %lld pairs, N=%d, k=%d\n",
+                 (long long)natoms * (natoms - 1) / 2, N, k);
+    const int rc = gmxcorr_ksg_matrix(rows.data(), natoms, N, kr->psi, k, mat);
+    if (rc != GMXCORR_OK) {
+        std::fprintf(stderr, "HIP error in kraskov_corrmatrix_gpu: %s\n", gmxcorr_last_error());
+        std::exit(1);
+    }
+}

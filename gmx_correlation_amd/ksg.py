@@ -1,0 +1,199 @@
+"""Host-side mirror of the reference's KSG interface, backed by the HIP library.
+
+Names, argument meaning and error behaviour follow the reference:
+  center()             src/gmx_correlation.cpp:184-223 (t_traj fluctuations)
+  kraskov_prepare()    lib/kraskov.cpp:2168-2220
+  kraskov_corrmatrix() lib/correlation_core.cpp:324-459 (GPU branch :337-348)
+  pearsify()           lib/correlation_core.cpp:461-469
+  write_matrix()       lib/correlation_core.cpp:202-225
+  read_matrix()        bin/analyze_matrix.py:71-102 (the format's consumer)
+
+One deliberate difference: the reference's dispatcher silently falls back to
+its CPU loop when no GPU is present or k > 128.  This package has no CPU
+path; it raises instead (the CPU path stays in the reference's own code).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+EULER_GAMMA_TRUNC = 0.57721566490153   # the reference's psi[1] = -0.57721566490153
+GPU_K_MAX = 128                         # lib/correlation_core.cpp:339
+
+
+def center(coords: np.ndarray) -> np.ndarray:
+    """float32 frames [nframes][natoms][3] -> mean-centred double
+    fluctuations [natoms][nframes][3] (t_traj.x).  The mean is a sequential
+    double sum over frames divided by nframes."""
+    c = np.asarray(coords, dtype=np.float32).astype(np.float64)
+    T = c.shape[0]
+    mean = np.cumsum(c, axis=0)[-1] / T            # sequential sum, as the reference
+    return np.ascontiguousarray((c - mean[None]).transpose(1, 0, 2))
+
+
+def kraskov_prepare(traj: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """t_traj.x [natoms][N][3] -> (x rows [3*natoms][N], psi [N+1]).
+
+    Per component: mean and sample SD (N-1) from sequential sums, z-score,
+    subtract the minimum.  psi[0]=0, psi[1]=-0.57721566490153,
+    psi[m+1]=psi[m]+1/m.  A zero SD gives non-finite rows, which the GPU
+    library rejects."""
+    t = np.asarray(traj, dtype=np.float64)
+    n, N, _ = t.shape
+    x = np.ascontiguousarray(t.transpose(0, 2, 1).reshape(3 * n, N))
+    me = np.cumsum(x, axis=1)[:, -1] / N
+    dev = x - me[:, None]
+    s = np.sqrt(np.cumsum(dev * dev, axis=1)[:, -1] / (N - 1))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        z = dev / s[:, None]
+    lo = np.minimum(z.min(axis=1), 1000000.0)
+    rows = np.ascontiguousarray(z - lo[:, None])
+    inc = np.empty(N)
+    inc[0] = -EULER_GAMMA_TRUNC
+    inc[1:] = 1.0 / np.arange(1, N, dtype=np.float64)
+    psi = np.empty(N + 1)
+    psi[0] = 0.0
+    psi[1:] = np.cumsum(inc)
+    return rows, psi
+
+
+class KsgPlan:
+    """Device-resident KSG run: distance rows stay in HBM between runs."""
+
+    def __init__(self, natoms: int, nframes: int, k: int):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.gmxcorr_ksg_plan_create(natoms, nframes, k, ctypes.byref(h)))
+        self._h = h
+        self.natoms, self.nframes, self.k = natoms, nframes, k
+
+    def load(self, rows: np.ndarray, psi: np.ndarray) -> "KsgPlan":
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        if rows.shape != (3 * self.natoms, self.nframes) or psi.shape != (self.nframes + 1,):
+            raise ValueError("rows must be [3*natoms][nframes] and psi [nframes+1]")
+        _lib.check(_lib.load().gmxcorr_ksg_plan_load(self._h, _lib.ptr(rows), _lib.ptr(psi)))
+        return self
+
+    def run_all(self) -> np.ndarray:
+        """MI of all pairs in the reference's order (a1 = 1.., a2 < a1)."""
+        L = _lib.load()
+        _lib.check(L.gmxcorr_ksg_plan_run_all(self._h))
+        n = self.natoms * (self.natoms - 1) // 2
+        out = np.empty(n)
+        _lib.check(L.gmxcorr_ksg_plan_result(self._h, _lib.ptr(out), n))
+        return out
+
+    def run_all_device(self) -> None:
+        """Run without copying results back (timing)."""
+        _lib.check(_lib.load().gmxcorr_ksg_plan_run_all(self._h))
+
+    def matrix(self) -> np.ndarray:
+        mat = np.zeros((self.natoms, self.natoms))
+        _lib.check(_lib.load().gmxcorr_ksg_plan_matrix(self._h, _lib.ptr(mat)))
+        return mat
+
+    def run_pairs(self, a1, a2, detail: bool = False):
+        a1 = np.ascontiguousarray(a1, dtype=np.int32)
+        a2 = np.ascontiguousarray(a2, dtype=np.int32)
+        n = len(a1)
+        L = _lib.load()
+        nx = ny = fl = None
+        if detail:
+            nx = np.empty((n, self.nframes), np.int32)
+            ny = np.empty((n, self.nframes), np.int32)
+            fl = np.empty((n, self.nframes), np.uint8)
+        _lib.check(L.gmxcorr_ksg_plan_run_pairs(
+            self._h, n, _lib.ptr(a1, _lib._ip), _lib.ptr(a2, _lib._ip),
+            _lib.ptr(nx, _lib._ip), _lib.ptr(ny, _lib._ip), _lib.ptr(fl, _lib._u8p)))
+        out = np.empty(n)
+        _lib.check(L.gmxcorr_ksg_plan_result(self._h, _lib.ptr(out), n))
+        if detail:
+            return out, dict(nx=nx, ny=ny, flags=fl)
+        return out
+
+    def stats(self) -> dict:
+        st = _lib.KsgStats()
+        _lib.check(_lib.load().gmxcorr_ksg_plan_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().gmxcorr_ksg_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def kraskov_corrmatrix(traj: np.ndarray, k: int, use_gpu: bool = True) -> np.ndarray:
+    """MI matrix [natoms][natoms] (nats, diagonal 2000) of a t_traj.
+
+    Raises ValueError for natoms < 2 or nframes <= k (the reference throws
+    std::runtime_error, lib/correlation_core.cpp:327-330) and RuntimeError
+    when the GPU path cannot run (no device, k > 128)."""
+    t = np.asarray(traj, dtype=np.float64)
+    natoms, nframes = t.shape[0], t.shape[1]
+    if natoms < 2:
+        raise ValueError("kraskov_corrmatrix: need at least 2 atoms")
+    if nframes <= k:
+        raise ValueError("kraskov_corrmatrix: nframes must exceed k (number of nearest neighbours)")
+    if not use_gpu:
+        raise RuntimeError("this package implements only the GPU path (use_gpu=True)")
+    if k > GPU_K_MAX:
+        raise RuntimeError(f"GPU path supports k <= {GPU_K_MAX}; k={k} requested")
+    rows, psi = kraskov_prepare(t)
+    mat = np.zeros((natoms, natoms))
+    _lib.check(_lib.load().gmxcorr_ksg_matrix(_lib.ptr(rows), natoms, nframes, _lib.ptr(psi), k,
+                                              _lib.ptr(mat)))
+    return mat
+
+
+def pearsify(mat: np.ndarray, dim: int = 3) -> np.ndarray:
+    """r = sqrt(1 - exp(-2/dim * MI)) for MI > 0 else 0 (diagonal 2000 -> 1)."""
+    m = np.asarray(mat, dtype=np.float64)
+    with np.errstate(over="ignore"):
+        r = np.sqrt(1.0 - np.exp(-2.0 / dim * m))
+    return np.where(m > 0.0, r, 0.0)
+
+
+def write_matrix(mat: np.ndarray, path: str) -> None:
+    """correl.dat: "n x n [", then mat[j*n+i] for i, j (transposed) as
+    "%10.6g " with a newline after every 21 values, then "]"."""
+    m = np.asarray(mat, dtype=np.float64)
+    n = m.shape[0]
+    parts = [f"{n} x {n} [\n"]
+    ncol = 0
+    for i in range(n):
+        for j in range(n):
+            parts.append("%10.6g " % m[j, i])
+            ncol += 1
+            if ncol > 20:
+                ncol = 0
+                parts.append("\n")
+    parts.append("]\n")
+    with open(path, "w") as f:
+        f.write("".join(parts))
+
+
+def read_matrix(path: str) -> np.ndarray:
+    """Inverse of write_matrix: returns mat with mat[j, i] as written."""
+    with open(path) as f:
+        header = f.readline().split()
+        n = int(header[0])
+        vals = [float(v) for v in f.read().replace("]", " ").split()]
+    if len(vals) != n * n:
+        raise ValueError(f"expected {n * n} values, found {len(vals)}")
+    return np.array(vals).reshape(n, n).T.copy()

@@ -1,0 +1,84 @@
+/*
+ * gmxcorr.h -- extern "C" ABI of libgmxcorr_hip.so (MI355X / gfx950).
+ *
+ * Plain pointers and sizes only.  Host arrays are passed in and out; device
+ * memory is owned by the library.  Every function returns GMXCORR_OK (0) or a
+ * negative status; gmxcorr_last_error() gives the message of the last failure
+ * on the calling thread.
+ *
+ * The reference's own boundary for this path is the C++ pair in
+ * kraskov_gpu.h (lib/kraskov_gpu.h:20,32), which this library also exports.
+ * The functions below expose the same computation to FFI callers (ctypes) and
+ * add the device-resident plan that the benchmark times.
+ */
+#ifndef GMXCORR_H
+#define GMXCORR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMXCORR_OK       0
+#define GMXCORR_EINVAL  -1   /* bad argument (sizes, k, non-finite input) */
+#define GMXCORR_ENODEV  -2   /* no usable gfx950 device */
+#define GMXCORR_ENOMEM  -3   /* device allocation failed */
+#define GMXCORR_EHIP    -4   /* HIP runtime error */
+#define GMXCORR_EUNSUP  -5   /* shape outside what the kernels support */
+
+const char* gmxcorr_last_error(void);
+
+/* 1 if device 0 is a gfx950 GPU, else 0.  Same test as kraskov_gpu_probe()
+ * (lib/kraskov_gpu.h:20). */
+int gmxcorr_probe(void);
+
+/* Largest frame count the KSG kernels accept. */
+int gmxcorr_ksg_max_frames(void);
+
+/* One-shot KSG-2 matrix: the flat-array form of kraskov_corrmatrix_gpu()
+ * (lib/kraskov_gpu.h:32).  xrows: [3*natoms][nframes] normalised rows as
+ * kraskov_prepare() leaves them (lib/kraskov.cpp:2168-2220); psi:
+ * [nframes+1]; mat: [natoms*natoms], written with the 2000 diagonal and the
+ * symmetric MI (nats).  1 <= k < nframes, natoms >= 2. */
+int gmxcorr_ksg_matrix(const double* xrows, int natoms, int nframes, const double* psi, int k,
+                       double* mat);
+
+/* ---- Device-resident plan ------------------------------------------------
+ * create -> load (host rows -> HBM, build the per-atom distance rows) ->
+ * run_all / run_pairs (any number of times) -> result / stats -> destroy.
+ * Pair order of run_all is the reference's: for a1 = 1..n-1, a2 = 0..a1-1
+ * (lib/kraskov_gpu.cu:248-256, lib/correlation_core.cpp:403-405). */
+typedef struct gmxcorr_ksg_plan gmxcorr_ksg_plan;
+
+typedef struct {
+    double ms_load;        /* host->device copy of rows and psi */
+    double ms_build;       /* per-atom distance-row build */
+    double ms_pairs;       /* main pair kernel (HIP events on the plan stream) */
+    double ms_fixup;       /* FP64 recount of flagged rows + finalise */
+    long long npairs;      /* pairs in the last run */
+    long long fixup_rows;  /* rows recounted in FP64 (FP32 near-ties) */
+    long long ties;        /* rows with an exact FP64 tie at the k-th distance */
+    long long row_bytes;   /* bytes of distance rows resident in HBM */
+    int nframes, k, natoms, pitch;
+} gmxcorr_ksg_stats;
+
+int  gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** out);
+int  gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* plan, const double* xrows, const double* psi);
+int  gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* plan);
+/* MI of an explicit pair list (a1[p] != a2[p]), e.g. the exact-KSG fallback
+ * of the surrogate.  When nx/ny/flags are non-NULL they receive per-frame
+ * marginal counts [npairs][nframes] (the reference's nx2/ny2,
+ * lib/kraskov.cpp:1311-1314) and flags (1 = recounted in FP64, 2 = exact tie). */
+int  gmxcorr_ksg_plan_run_pairs(gmxcorr_ksg_plan* plan, long long npairs, const int* a1,
+                                const int* a2, int* nx, int* ny, unsigned char* flags);
+/* MI (nats) of the last run, in its pair order. */
+int  gmxcorr_ksg_plan_result(gmxcorr_ksg_plan* plan, double* mi, long long npairs);
+/* Fill mat[natoms*natoms] from the last run_all (diagonal 2000). */
+int  gmxcorr_ksg_plan_matrix(gmxcorr_ksg_plan* plan, double* mat);
+int  gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* plan, gmxcorr_ksg_stats* st);
+void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,106 @@
+"""Host-side mirror and C-ABI library checks that need no GPU."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from gmx_correlation_amd import _lib, ksg, synth
+from tests.conftest import GOLDEN, ROOT, golden
+
+
+def _declared_symbols():
+    """Function names declared in include/*.h (C ABI) plus the two C++
+    boundary symbols of include/kraskov_gpu.h."""
+    names = set()
+    src = open(os.path.join(ROOT, "include", "gmxcorr.h")).read()
+    for m in re.finditer(r"^\s*(?:const\s+)?[\w\*]+\s+\**(gmxcorr_\w+)\s*\(", src, re.M):
+        names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(hiplib):
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    declared = _declared_symbols()
+    assert declared, "no declarations parsed"
+    assert declared <= exported, declared - exported
+    assert set(_lib.CXX_SYMBOLS) <= exported
+    assert set(_lib.C_SYMBOLS) == declared
+
+
+def test_cxx_boundary_names_demangle(hiplib):
+    out = subprocess.run(["c++filt"] + _lib.CXX_SYMBOLS, capture_output=True, text=True).stdout
+    assert "kraskov_gpu_probe()" in out
+    assert "kraskov_corrmatrix_gpu(t_kraskov const*, int, double*, int)" in out
+
+
+def test_library_loads_and_probe_is_boolean(hiplib):
+    assert _lib.probe() in (True, False)
+    assert hiplib.gmxcorr_ksg_max_frames() >= 2000
+
+
+def test_plan_rejects_bad_shapes_without_gpu(hiplib):
+    import ctypes
+    h = ctypes.c_void_p()
+    assert hiplib.gmxcorr_ksg_plan_create(1, 100, 5, ctypes.byref(h)) == -1
+    assert hiplib.gmxcorr_ksg_plan_create(4, 10, 10, ctypes.byref(h)) == -1
+    assert hiplib.gmxcorr_ksg_plan_create(4, 10, 0, ctypes.byref(h)) == -1
+    assert b"k" in hiplib.gmxcorr_last_error()
+
+
+@pytest.mark.parametrize("spec", [synth.SMALL_CPU, synth.README_247])
+def test_center_and_prepare_bitexact_vs_oracle(oracle_lib, spec):
+    c = synth.generate(spec)
+    t = ksg.center(c)
+    assert np.array_equal(t, oracle_lib.center(c))
+    rows, psi = ksg.kraskov_prepare(t)
+    r2, p2 = oracle_lib.prepare(t)
+    assert np.array_equal(rows, r2) and np.array_equal(psi, p2)
+
+
+def test_prepare_matches_reference_fixture():
+    rows, psi = ksg.kraskov_prepare(ksg.center(synth.generate(synth.SMALL_CPU)))
+    assert np.array_equal(rows[:6], golden("small_cpu_prepare_rows_a0a1.npy"))
+    assert np.array_equal(psi, golden("small_cpu_psi.npy"))
+
+
+def test_write_matrix_byte_exact_vs_reference(tmp_path, oracle_lib):
+    """correl.dat written by the reference's write_matrix (fixture) and by the
+    mirror from the same r(MI) values are identical files."""
+    c = synth.generate(synth.SMALL_CPU)[:, :8]
+    rows, psi = ksg.kraskov_prepare(ksg.center(c))
+    mi = oracle_lib.ksg_matrix(rows, psi, 8, 10)
+    path = tmp_path / "correl.dat"
+    ksg.write_matrix(ksg.pearsify(mi), str(path))
+    assert path.read_bytes() == open(os.path.join(GOLDEN, "small8_correl.dat"), "rb").read()
+    back = ksg.read_matrix(str(path))
+    assert np.allclose(back, ksg.pearsify(mi), rtol=1e-5)
+
+
+def test_pearsify_mirror(oracle_lib):
+    """numpy's exp may differ from libm's by an ulp; correl.dat keeps 6 digits."""
+    m = golden("small_cpu_k10_mi.npy")
+    assert np.max(np.abs(ksg.pearsify(m) - oracle_lib.pearsify(m))) <= 1e-15
+
+
+def test_kraskov_corrmatrix_input_checks():
+    t = np.zeros((1, 50, 3))
+    with pytest.raises(ValueError):
+        ksg.kraskov_corrmatrix(t, 5)
+    with pytest.raises(ValueError):
+        ksg.kraskov_corrmatrix(np.zeros((3, 5, 3)), 5)
+    with pytest.raises(RuntimeError):
+        ksg.kraskov_corrmatrix(np.random.default_rng(0).random((3, 300, 3)), 129)
+    with pytest.raises(RuntimeError):
+        ksg.kraskov_corrmatrix(np.random.default_rng(0).random((3, 300, 3)), 5, use_gpu=False)
+
+
+def test_batched_specs_deterministic():
+    a = synth.batched_specs()
+    b = synth.batched_specs()
+    assert a == b and len(a) == 32
+    assert all(50 <= s.natoms <= 600 and 1000 <= s.nframes <= 8000 and k in (10, 50, 100)
+               for s, k in a)
