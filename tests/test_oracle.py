@@ -24,6 +24,7 @@ def small(oracle_lib):
 def test_generator_is_frozen():
     meta = json.load(open(os.path.join(GOLDEN, "meta.json")))
     assert synth.coords_sha256(synth.generate(synth.SMALL_CPU)) == meta["small_cpu_sha256"]
+    assert synth.coords_sha256(synth.generate(synth.README_247)) == meta["readme247_sha256"]
 
 
 def test_prepare_matches_reference(small):
