@@ -43,6 +43,22 @@ class KsgStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class DenseStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_load", ctypes.c_double),
+        ("ms_copula", ctypes.c_double),
+        ("ms_split", ctypes.c_double),
+        ("ms_gemm", ctypes.c_double),
+        ("ms_pairs", ctypes.c_double),
+        ("gemm_flops", ctypes.c_double),
+        ("natoms", _i),
+        ("nframes", _i),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class GmxcorrError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"gmxcorr {STATUS_NAMES.get(code, code)}: {msg}")
@@ -51,10 +67,12 @@ class GmxcorrError(RuntimeError):
 
 # Symbols declared in include/gmxcorr.h and include/kraskov_gpu.h.
 C_SYMBOLS = [
-    "gmxcorr_last_error", "gmxcorr_probe", "gmxcorr_ksg_max_frames", "gmxcorr_ksg_matrix",
+    "gmxcorr_last_error", "gmxcorr_probe", "gmxcorr_selftest_wave", "gmxcorr_ksg_max_frames",
+    "gmxcorr_ksg_matrix",
     "gmxcorr_ksg_plan_create", "gmxcorr_ksg_plan_load", "gmxcorr_ksg_plan_run_all",
     "gmxcorr_ksg_plan_run_pairs", "gmxcorr_ksg_plan_result", "gmxcorr_ksg_plan_matrix",
     "gmxcorr_ksg_plan_stats", "gmxcorr_ksg_plan_destroy",
+    "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix",
 ]
 CXX_SYMBOLS = ["_Z17kraskov_gpu_probev", "_Z22kraskov_corrmatrix_gpuPK9t_kraskoviPdi"]
 
@@ -72,6 +90,7 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_last_error.restype = ctypes.c_char_p
     L.gmxcorr_probe.restype = _i
     L.gmxcorr_ksg_max_frames.restype = _i
+    L.gmxcorr_selftest_wave.argtypes = [_ip]
     L.gmxcorr_ksg_matrix.argtypes = [_dp, _i, _i, _dp, _i, _dp]
     L.gmxcorr_ksg_plan_create.argtypes = [_i, _i, _i, ctypes.POINTER(ctypes.c_void_p)]
     L.gmxcorr_ksg_plan_load.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -82,6 +101,8 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_ksg_plan_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(KsgStats)]
     L.gmxcorr_ksg_plan_destroy.argtypes = [ctypes.c_void_p]
     L.gmxcorr_ksg_plan_destroy.restype = None
+    L.gmxcorr_gauss_matrix.argtypes = [_dp, _i, _i, _dp, _dp, ctypes.POINTER(DenseStats)]
+    L.gmxcorr_gcmi_matrix.argtypes = [_dp, _i, _i, _dp, _dp, _dp, ctypes.POINTER(DenseStats)]
     _lib = L
     return L
 

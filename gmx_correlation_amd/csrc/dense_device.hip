@@ -1,0 +1,317 @@
+// Gaussian and Gaussian-copula (GCMI) mutual-information kernels for gfx950.
+//
+// Both estimators need, for every atom pair, a 6x6 block of a second-moment
+// matrix C = V^T V / N of a [N frames][3n components] matrix V:
+//   Gaussian  V = mean-centred coordinates (lib/correlation_core.cpp:140-166)
+//   GCMI      V = van der Waerden copula scores (lib/gcmi.cpp:95-146, 199-211)
+// C is one dense GEMM (3n x N x 3n).  It runs on the matrix cores as a
+// split-precision FP16 product with FP32 accumulation (after Ootomo & Yokota
+// 2022): v*s = hi + lo*2^-11 with hi = fp16(v*s) and lo = fp16((v*s - hi)*2^11),
+// C*s^2 ~= hi.hi + (hi.lo + lo.hi)*2^-11, the hi.hi and cross terms kept in
+// separate FP32 accumulators.  The per-pair log-determinants, the entropies
+// and the copula's inverse normal stay in FP64.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dense_internal.h"
+
+namespace gmxcorr {
+
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float16v __attribute__((ext_vector_type(16)));
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Determinant by LU with partial pivoting, the row operations of
+// lib/correlation_core.cpp:49-98 (identical in lib/gcmi.cpp:152-184).
+template <int n>
+__device__ double lu_det(double* m)
+{
+    double sign = 1.0;
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        double best = fabs(m[col * n + col]);
+        for (int r = col + 1; r < n; ++r) {
+            const double v = fabs(m[r * n + col]);
+            if (v > best) { best = v; piv = r; }
+        }
+        if (best == 0.0) return 0.0;
+        if (piv != col) {
+            for (int j = 0; j < n; ++j) {
+                const double t = m[col * n + j];
+                m[col * n + j] = m[piv * n + j];
+                m[piv * n + j] = t;
+            }
+            sign = -sign;
+        }
+        const double diag = m[col * n + col];
+        for (int r = col + 1; r < n; ++r) {
+            const double f = m[r * n + col] / diag;
+            for (int j = col + 1; j < n; ++j) m[r * n + j] -= f * m[col * n + j];
+        }
+    }
+    double det = sign;
+    for (int i = 0; i < n; ++i) det *= m[i * n + i];
+    return det;
+}
+
+// Inverse error function: Acklam's published rational approximation of the
+// inverse normal CDF at p = (x+1)/2 (Acklam 2003), divided by sqrt 2, then
+// one Newton step on erf.  The method lib/gcmi.cpp:26-79 uses.
+__device__ double erfinv_acklam(double x)
+{
+    if (x >= 1.0) return 1e30;
+    if (x <= -1.0) return -1e30;
+    if (x == 0.0) return 0.0;
+    const double a0 = -3.969683028665376e+01, a1 = 2.209460984245205e+02,
+                 a2 = -2.759285104469687e+02, a3 = 1.383577518672690e+02,
+                 a4 = -3.066479806614716e+01, a5 = 2.506628277459239e+00;
+    const double b0 = -5.447609879822406e+01, b1 = 1.615858368580409e+02,
+                 b2 = -1.556989798598866e+02, b3 = 6.680131188771972e+01,
+                 b4 = -1.328068155288572e+01;
+    const double c0 = -7.784894002430293e-03, c1 = -3.223964580411365e-01,
+                 c2 = -2.400758277161838e+00, c3 = -2.549732539343734e+00,
+                 c4 = 4.374664141464968e+00, c5 = 2.938163982698783e+00;
+    const double d0 = 7.784695709041462e-03, d1 = 3.224671290700398e-01,
+                 d2 = 2.445134137142996e+00, d3 = 3.754408661907416e+00;
+    const double p = (x + 1.0) * 0.5;
+    double r;
+    if (p < 0.02425) {
+        const double q = sqrt(-2.0 * log(p));
+        r = (((((c0 * q + c1) * q + c2) * q + c3) * q + c4) * q + c5) /
+            ((((d0 * q + d1) * q + d2) * q + d3) * q + 1.0);
+    } else if (p <= 0.97575) {
+        const double q = p - 0.5, s = q * q;
+        r = (((((a0 * s + a1) * s + a2) * s + a3) * s + a4) * s + a5) * q /
+            (((((b0 * s + b1) * s + b2) * s + b3) * s + b4) * s + 1.0);
+    } else {
+        const double q = sqrt(-2.0 * log(1.0 - p));
+        r = -(((((c0 * q + c1) * q + c2) * q + c3) * q + c4) * q + c5) /
+            ((((d0 * q + d1) * q + d2) * q + d3) * q + 1.0);
+    }
+    r /= sqrt(2.0);
+    r -= (erf(r) - x) * sqrt(kPi) * 0.5 * exp(r * r);
+    return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Split of V (FP64, component rows [3n][N] with row stride ldv) into scaled
+// FP16 hi/lo rows [3n][Kp]; frames N..Kp-1 are zero.
+// ---------------------------------------------------------------------------
+__global__ void dense_split_kernel(const double* __restrict__ V, int N, long long ldv, int Kp,
+                                   double scale, _Float16* __restrict__ hi,
+                                   _Float16* __restrict__ lo)
+{
+    const long long c = blockIdx.y;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < Kp; f += gridDim.x * blockDim.x) {
+        _Float16 h = (_Float16)0.0f, l = (_Float16)0.0f;
+        if (f < N) {
+            const double v = V[c * ldv + f] * scale;
+            h = (_Float16)v;
+            l = (_Float16)((v - (double)h) * 2048.0);
+        }
+        hi[c * Kp + f] = h;
+        lo[c * Kp + f] = l;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C = (hi.hi + (hi.lo + lo.hi) 2^-11) * out_scale over the upper-triangle
+// 64x64 tiles (ti <= tj), mirrored into the lower triangle.  Block = 4 waves,
+// wave w owns the 32x32 sub-tile (w/2, w%2) and issues
+// v_mfma_f32_32x32x16_f16: lane l supplies row l%32 and the 8 consecutive
+// frames 8*(l/32).. of each 16-frame step for both operands (both read the
+// component-major rows, since B[k][j] = V[k][j] = rows[j][k]); the result
+// register q of lane l is row 8*(q/4) + 4*(l/32) + q%4, column l%32.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void dense_cov_split_kernel(const _Float16* __restrict__ hi,
+                                                              const _Float16* __restrict__ lo,
+                                                              int M, int Kp, int ntile,
+                                                              float out_scale,
+                                                              float* __restrict__ C)
+{
+    // triangular decode of the tile pair
+    int t = blockIdx.x, ti = 0;
+    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
+    const int tj = ti + t;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row0 = ti * 64 + (w >> 1) * 32, col0 = tj * 64 + (w & 1) * 32;
+    const int r = lane & 31, kq = (lane >> 5) * 8;
+    const int arow = min(row0 + r, M - 1), bcol = min(col0 + r, M - 1);
+    const _Float16* ah = hi + (size_t)arow * Kp + kq;
+    const _Float16* al = lo + (size_t)arow * Kp + kq;
+    const _Float16* bh = hi + (size_t)bcol * Kp + kq;
+    const _Float16* bl = lo + (size_t)bcol * Kp + kq;
+    float16v acc_hh = {}, acc_x = {};
+    for (int k0 = 0; k0 < Kp; k0 += 16) {
+        const half8 vah = *reinterpret_cast<const half8*>(ah + k0);
+        const half8 val = *reinterpret_cast<const half8*>(al + k0);
+        const half8 vbh = *reinterpret_cast<const half8*>(bh + k0);
+        const half8 vbl = *reinterpret_cast<const half8*>(bl + k0);
+        acc_hh = __builtin_amdgcn_mfma_f32_32x32x16_f16(vah, vbh, acc_hh, 0, 0, 0);
+        acc_x = __builtin_amdgcn_mfma_f32_32x32x16_f16(vah, vbl, acc_x, 0, 0, 0);
+        acc_x = __builtin_amdgcn_mfma_f32_32x32x16_f16(val, vbh, acc_x, 0, 0, 0);
+    }
+    const int j = col0 + (lane & 31);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = row0 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+        if (i < M && j < M) {
+            const float v = (acc_hh[q] + acc_x[q] * (1.0f / 2048.0f)) * out_scale;
+            C[(size_t)i * M + j] = v;
+            C[(size_t)j * M + i] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pair MI from C (FP32 -> FP64), one thread per pair (a1 < a2), pairs
+// enumerated row by row of the upper triangle.
+// mode 0: Gaussian, MI = H(C11) + H(C22) - H(CJ), H = 0.5(n(1+log 2pi) + log det)
+//         or 0 when det <= 0 (lib/correlation_core.cpp:100-111, 280-311);
+// mode 1: GCMI, MI = 0.5(log d11 + log d22 - log dJ), 0 if any det <= 0
+//         (lib/gcmi.cpp:212-231).
+// ---------------------------------------------------------------------------
+__global__ void dense_pair_mi_kernel(const float* __restrict__ C, int natoms, int mode,
+                                     double* __restrict__ mat)
+{
+    const long long npairs = (long long)natoms * (natoms - 1) / 2;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    // decode p -> (a1, a2) with a1 < a2, row-major over the upper triangle
+    int a1 = (int)((2.0 * natoms - 1 - sqrt((2.0 * natoms - 1) * (2.0 * natoms - 1) - 8.0 * p)) / 2);
+    auto start = [&](int a) { return (long long)a * (2 * natoms - a - 1) / 2; };
+    while (a1 > 0 && start(a1) > p) --a1;
+    while (start(a1 + 1) <= p) ++a1;
+    const int a2 = a1 + 1 + (int)(p - start(a1));
+    const int M = 3 * natoms;
+    double c1[9], c2[9], cj[36];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            const double x11 = C[(size_t)(3 * a1 + r) * M + 3 * a1 + c];
+            const double x22 = C[(size_t)(3 * a2 + r) * M + 3 * a2 + c];
+            const double x12 = C[(size_t)(3 * a1 + r) * M + 3 * a2 + c];
+            c1[r * 3 + c] = x11;
+            c2[r * 3 + c] = x22;
+            cj[r * 6 + c] = x11;
+            cj[(r + 3) * 6 + c + 3] = x22;
+            cj[r * 6 + c + 3] = x12;
+            cj[(c + 3) * 6 + r] = x12;
+        }
+    const double d1 = lu_det<3>(c1), d2 = lu_det<3>(c2), dj = lu_det<6>(cj);
+    double mi;
+    if (mode == 0) {
+        auto H = [](double det, int n) {
+            return det <= 0.0 ? 0.0 : 0.5 * (n * (1.0 + log(2.0 * kPi)) + log(det));
+        };
+        mi = H(d1, 3) + H(d2, 3) - H(dj, 6);
+    } else {
+        mi = (d1 <= 0.0 || d2 <= 0.0 || dj <= 0.0) ? 0.0 : 0.5 * (log(d1) + log(d2) - log(dj));
+    }
+    mat[(size_t)a1 * natoms + a2] = mi;
+    mat[(size_t)a2 * natoms + a1] = mi;
+    if (a2 == a1 + 1) mat[(size_t)a1 * natoms + a1] = 2000.0;
+    if (a1 == natoms - 2 && a2 == natoms - 1) mat[(size_t)a2 * natoms + a2] = 2000.0;
+}
+
+// ---------------------------------------------------------------------------
+// Copula scores: block per component row; the row is staged in LDS when it
+// fits.  Rank by counting (exact FP64 comparisons): lt = #{g: v_g < v_f},
+// eq = #{g: v_g == v_f}; the score is the mean over t in [lt, lt+eq) of
+// sqrt(2) erfinv(2(t + 0.5)/N - 1), summed in ascending t as the reference.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void dense_copula_kernel(const double* __restrict__ V, int N,
+                                                           long long ldv,
+                                                           double* __restrict__ Z,
+                                                           long long ldz, int stage)
+{
+    extern __shared__ double srow[];
+    const long long c = blockIdx.x;
+    const double* row = V + c * ldv;
+    if (stage) {
+        for (int f = threadIdx.x; f < N; f += blockDim.x) srow[f] = row[f];
+        __syncthreads();
+        row = srow;
+    }
+    for (int f = threadIdx.x; f < N; f += blockDim.x) {
+        const double v = row[f];
+        int lt = 0, eq = 0;
+        for (int g = 0; g < N; ++g) {
+            const double u = row[g];
+            lt += u < v;
+            eq += u == v;
+        }
+        double z = 0.0;
+        for (int t = lt; t < lt + eq; ++t) {
+            const double pr = (t + 0.5) / N;
+            z += sqrt(2.0) * erfinv_acklam(2.0 * pr - 1.0);
+        }
+        Z[c * ldz + f] = z / eq;
+    }
+}
+
+// traj [n][N][3] -> component rows [3n][N]
+__global__ void dense_rows_kernel(const double* __restrict__ traj, int natoms, int N,
+                                  double* __restrict__ rows)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)natoms * N * 3;
+    if (t >= total) return;
+    const long long a = t / (3LL * N);
+    const long long rem = t % (3LL * N);
+    const int f = (int)(rem / 3), d = (int)(rem % 3);
+    rows[(a * 3 + d) * N + f] = traj[t];
+}
+
+hipError_t launch_dense_rows(const double* traj, int natoms, int N, double* rows, hipStream_t s)
+{
+    const long long total = (long long)natoms * N * 3;
+    hipLaunchKernelGGL(dense_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       traj, natoms, N, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_split(const double* V, int M, int N, long long ldv, int Kp, double scale,
+                              _Float16* hi, _Float16* lo, hipStream_t s)
+{
+    const int bx = (Kp + 255) / 256 < 64 ? (Kp + 255) / 256 : 64;
+    hipLaunchKernelGGL(dense_split_kernel, dim3(bx, M), dim3(256), 0, s, V, N, ldv, Kp, scale, hi,
+                       lo);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_cov(const _Float16* hi, const _Float16* lo, int M, int Kp,
+                            float out_scale, float* C, hipStream_t s)
+{
+    const int ntile = (M + 63) / 64;
+    const unsigned nblk = (unsigned)(ntile * (ntile + 1) / 2);
+    hipLaunchKernelGGL(dense_cov_split_kernel, dim3(nblk), dim3(256), 0, s, hi, lo, M, Kp, ntile,
+                       out_scale, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_pairs(const float* C, int natoms, int mode, double* mat, hipStream_t s)
+{
+    const long long npairs = (long long)natoms * (natoms - 1) / 2;
+    hipLaunchKernelGGL(dense_pair_mi_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0,
+                       s, C, natoms, mode, mat);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_copula(const double* V, int M, int N, long long ldv, double* Z,
+                               long long ldz, hipStream_t s)
+{
+    if (N > kCopulaMaxFrames) return hipErrorInvalidValue;
+    const size_t bytes = (size_t)N * sizeof(double);
+    hipError_t e = hipFuncSetAttribute((const void*)dense_copula_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dense_copula_kernel, dim3(M), dim3(256), bytes, s, V, N, ldv, Z, ldz, 1);
+    return hipGetLastError();
+}
+
+}  // namespace gmxcorr

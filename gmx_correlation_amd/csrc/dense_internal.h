@@ -1,0 +1,20 @@
+// Internal interface of the Gaussian / GCMI kernels (dense_device.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gmxcorr {
+
+// Largest frame count whose component row is staged whole in LDS by the
+// copula kernel (8 bytes per frame within 160 KB).
+constexpr int kCopulaMaxFrames = 19456;
+
+hipError_t launch_dense_rows(const double* traj, int natoms, int N, double* rows, hipStream_t s);
+hipError_t launch_dense_split(const double* V, int M, int N, long long ldv, int Kp, double scale,
+                              _Float16* hi, _Float16* lo, hipStream_t s);
+hipError_t launch_dense_cov(const _Float16* hi, const _Float16* lo, int M, int Kp,
+                            float out_scale, float* C, hipStream_t s);
+hipError_t launch_dense_pairs(const float* C, int natoms, int mode, double* mat, hipStream_t s);
+hipError_t launch_dense_copula(const double* V, int M, int N, long long ldv, double* Z,
+                               long long ldz, hipStream_t s);
+
+}  // namespace gmxcorr

@@ -12,16 +12,23 @@
 // (row pitch P floats, padding = +inf) from the FP64 coordinates.  Rounding
 // to FP32 is monotone, so an FP32 comparison can only disagree with FP64 when
 // two FP32 values are *equal*; such rows are detected exactly (a tie at the
-// (k+1)-th joint key, or more than one frame at the marginal radius) and
+// (k+1)-th joint key, or more than one frame at a marginal radius) and
 // recounted from the FP64 coordinates by ksg_fixup_kernel.  All other rows
 // are bit-exact against the FP64 definition by construction.
 //
-// Pair kernel: one wavefront per (pair, frame row).  The two rows are
-// streamed with coalesced 16-byte loads into registers (lane l holds frames
-// 256*m + 4*l + c), the (k+1)-th smallest joint key (self included, key 0)
-// is found by a three-digit radix select (11 + 10 + 10 bits of the
-// non-negative FP32 bit pattern) with a wave-private LDS histogram, then one
-// pass takes the marginal radii over the selected set and one pass counts.
+// Pair kernel, one wavefront per (pair, frame row).  The two rows stream into
+// registers with 16-byte loads (lane l holds frames 256*m + 4*l + c).  The
+// (k+1)-th smallest joint key (self included, key 0) is located in three
+// steps, all on the non-negative FP32 bit pattern:
+//   sweep A  histogram of 2048 bins, 64 per octave over [2^-20, 2^12), in a
+//            wave-private LDS array;
+//   sweep B  frames below the target bin are in S for sure: their marginal
+//            maxima are taken on the fly; the few frames inside the target
+//            bin are appended to lane-private LDS slots;
+//   select   the candidates are compacted one per lane and the exact key is
+//            found by a 17-step ballot radix over the bits below the bin.
+// Sweep C counts the marginal neighbours.  Rows whose target bin is an edge
+// bin or holds more than 64 frames go to the FP64 recount unresolved.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -34,6 +41,13 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kPairWaves = 4;
 constexpr uint32_t kInfBits = 0x7f800000u;
+constexpr uint32_t kUnknownEps = 0xffffffffu;
+// Sweep-A binning: bin = clamp((key >> 17) - kBinBase, 0, 2047); key >> 17
+// is (exponent << 6 | top 6 mantissa bits), bin 0 starts at 2^-20.
+constexpr int kBinShift = 17;
+constexpr int kBinBase = (127 - 20) << 6;
+constexpr int kBins = 2048;
+constexpr int kCandCap = 4;  // candidate slots per lane
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -42,69 +56,101 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// DPP lane exchange; lanes without a valid source (or outside row_mask) read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+__device__ __forceinline__ int readlane(int v, int lane)
+{
+    return __builtin_amdgcn_readlane(v, lane);
+}
+
+// Sum over the wave, returned uniform: quad swaps and row mirrors reduce each
+// row of 16 lanes, four readlanes combine the rows.
 __device__ __forceinline__ int wave_sum(int v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    v += dpp<0xb1>(v);   // quad_perm [1,0,3,2]
+    v += dpp<0x4e>(v);   // quad_perm [2,3,0,1]
+    v += dpp<0x141>(v);  // row_half_mirror
+    v += dpp<0x140>(v);  // row_mirror
+    return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
 }
 
-__device__ __forceinline__ float wave_max(float v)
+__device__ __forceinline__ int wave_max_i(int v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+    v = max(v, dpp<0xb1>(v));
+    v = max(v, dpp<0x4e>(v));
+    v = max(v, dpp<0x141>(v));
+    v = max(v, dpp<0x140>(v));
+    return max(max(readlane(v, 0), readlane(v, 16)), max(readlane(v, 32), readlane(v, 48)));
+}
+
+__device__ __forceinline__ float wave_max(float f)
+{
+    int v = __float_as_int(f);
+    v = __float_as_int(fmaxf(__int_as_float(v), __int_as_float(dpp<0xb1>(v))));
+    v = __float_as_int(fmaxf(__int_as_float(v), __int_as_float(dpp<0x4e>(v))));
+    v = __float_as_int(fmaxf(__int_as_float(v), __int_as_float(dpp<0x141>(v))));
+    v = __float_as_int(fmaxf(__int_as_float(v), __int_as_float(dpp<0x140>(v))));
+    const float r0 = __int_as_float(readlane(v, 0)), r1 = __int_as_float(readlane(v, 16));
+    const float r2 = __int_as_float(readlane(v, 32)), r3 = __int_as_float(readlane(v, 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
+// Inclusive prefix sum over the 64 lanes (row shifts, then row broadcasts).
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += dpp<0x111>(v);        // row_shr:1
+    v += dpp<0x112>(v);        // row_shr:2
+    v += dpp<0x114>(v);        // row_shr:4
+    v += dpp<0x118>(v);        // row_shr:8
+    v += dpp<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
     return v;
 }
 
-__device__ __forceinline__ double wave_max(double v)
+// FP64 helpers for the recount kernel (rare rows; plain shuffles suffice).
+__device__ __forceinline__ double wave_max_d(double v)
 {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
     return v;
 }
 
-__device__ __forceinline__ double wave_min(double v)
+__device__ __forceinline__ double wave_min_d(double v)
 {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
     return v;
 }
 
-__device__ __forceinline__ int wave_incl_scan(int v, int lane)
-{
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const int t = __shfl_up(v, o, kWave);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
-// Locate the bin holding the t-th smallest element (1-based) of a
-// histogram of 64*BPL bins.  Lane l owns bins [l*BPL, (l+1)*BPL).  Returns the
-// bin, the rank of the target inside it and the bin's population.
+// Bin holding the t-th smallest element (1-based) of a histogram of 64*BPL
+// bins; lane l owns bins [l*BPL, (l+1)*BPL).  Returns the bin, the rank of
+// the target inside it and the bin's population.
 template <int BPL>
-__device__ __forceinline__ void find_bin(const uint32_t* hist, int lane, int t, int& bin, int& rank,
-                                         int& pop)
+__device__ __forceinline__ void find_bin(const uint32_t* hist, int lane, int t, int& bin,
+                                         int& rank, int& pop)
 {
     int s = 0;
-    // rotated order keeps the 64 lanes on distinct LDS banks
+    // rotated order keeps the lanes on distinct LDS banks
 #pragma unroll
     for (int q = 0; q < BPL; ++q) s += (int)hist[lane * BPL + ((q + lane) & (BPL - 1))];
-    const int incl = wave_incl_scan(s, lane);
+    const int incl = wave_incl_scan(s);
     const int excl = incl - s;
     const uint64_t bal = __ballot(excl < t && t <= incl);
-    const int owner = __ffsll((unsigned long long)bal) - 1;
-    const int base = __shfl(excl, owner, kWave);
-    // the owner's BPL bins spread over the first BPL lanes
+    const int owner = bal ? __ffsll((unsigned long long)bal) - 1 : 0;
+    const int base = readlane(excl, owner);
     const int v = lane < BPL ? (int)hist[owner * BPL + lane] : 0;
-    const int vi = wave_incl_scan(v, lane);
+    const int vi = wave_incl_scan(v);
     const int r = t - base;
     const uint64_t b2 = __ballot(lane < BPL && vi - v < r && r <= vi);
-    const int sub = __ffsll((unsigned long long)b2) - 1;
+    const int sub = b2 ? __ffsll((unsigned long long)b2) - 1 : 0;
     bin = owner * BPL + sub;
-    rank = r - __shfl(vi - v, sub, kWave);
-    pop = __shfl(v, sub, kWave);
+    rank = r - readlane(vi - v, sub);
+    pop = readlane(v, sub);
 }
 
 template <int NB>
@@ -154,18 +200,23 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
 
 // ---------------------------------------------------------------------------
 // Pair kernel.  Block = 4 waves = one pair; wave w takes frames w, w+4, ...
-// S4 = 16-byte chunks per lane per row (N <= 256*S4).
+// S4 = 16-byte chunks per lane per row (N <= 256*S4).  Blocks are spread so
+// that each XCD (block id mod 8) receives a contiguous range of pairs: pairs
+// next to each other share their first atom, whose rows then stay in that
+// XCD's L2.
 // ---------------------------------------------------------------------------
 template <int S4>
 __global__ __launch_bounds__(256) void ksg_pair_kernel(KsgArgs args)
 {
-    __shared__ uint32_t hist_all[kPairWaves][2048];
+    __shared__ uint32_t lds[kPairWaves][kBins];
     __shared__ double wsum[kPairWaves];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    uint32_t* hist = hist_all[w];
-    const long long p = blockIdx.x;
-    const int N = args.N, K = args.K, T = args.K + 1;
+    uint32_t* hist = lds[w];
+    const long long per_xcd = (args.npairs + 7) / 8;
+    const long long p = (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (p >= args.npairs) return;
+    const int N = args.N, T = args.K + 1;
     const long long P = args.P;
     const float* Da = args.D + (size_t)args.a1[p] * N * P;
     const float* Db = args.D + (size_t)args.a2[p] * N * P;
@@ -188,59 +239,85 @@ __global__ __launch_bounds__(256) void ksg_pair_kernel(KsgArgs args)
             b[4 * m + 0] = vb.x; b[4 * m + 1] = vb.y; b[4 * m + 2] = vb.z; b[4 * m + 3] = vb.w;
         }
 
-        // ---- radix select of the T-th smallest joint key -----------------
-        // digit 1: bits 30..20 (2048 bins)
-        clear_hist<2048>(hist, lane);
+        // ---- sweep A: fine histogram of the joint keys --------------------
+        clear_hist<kBins>(hist, lane);
         wave_sync();
 #pragma unroll
         for (int e = 0; e < 4 * S4; ++e) {
             const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
-            if (key < kInfBits) atomicAdd(&hist[key >> 20], 1u);
+            const int bin = min(max((int)(key >> kBinShift) - kBinBase, 0), kBins - 1);
+            atomicAdd(&hist[bin], 1u);
         }
         wave_sync();
-        int bin, rank, pop;
-        find_bin<32>(hist, lane, T, bin, rank, pop);
-        uint32_t prefix = (uint32_t)bin;
+        int b1, r0, pop;
+        find_bin<kBins / kWave>(hist, lane, T, b1, r0, pop);
         wave_sync();
-        // digit 2: bits 19..10 (1024 bins)
-        clear_hist<1024>(hist, lane);
-        wave_sync();
-#pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
-            if ((key >> 20) == prefix) atomicAdd(&hist[(key >> 10) & 1023u], 1u);
-        }
-        wave_sync();
-        find_bin<16>(hist, lane, rank, bin, rank, pop);
-        prefix = (prefix << 10) | (uint32_t)bin;
-        wave_sync();
-        // digit 3: bits 9..0 (1024 bins)
-        clear_hist<1024>(hist, lane);
-        wave_sync();
-#pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
-            if ((key >> 10) == prefix) atomicAdd(&hist[key & 1023u], 1u);
-        }
-        wave_sync();
-        find_bin<16>(hist, lane, rank, bin, rank, pop);
-        const uint32_t eps_key = (prefix << 10) | (uint32_t)bin;
-        wave_sync();
-        // keys < eps: T - rank; keys <= eps: that + pop.  Unambiguous iff == T.
-        const bool sel_tie = (T - rank + pop) != T;
+        bool unknown = b1 == 0 || b1 == kBins - 1 || pop > kWave;
+        const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
 
-        // ---- marginal radii over the selected set -------------------------
+        // ---- sweep B: sure members' marginal maxima, bin candidates -------
         float ex = 0.0f, ey = 0.0f;
+        int nc = 0;
 #pragma unroll
         for (int e = 0; e < 4 * S4; ++e) {
-            const bool in = __float_as_uint(fmaxf(a[e], b[e])) <= eps_key;
-            ex = fmaxf(ex, in ? a[e] : 0.0f);
-            ey = fmaxf(ey, in ? b[e] : 0.0f);
+            const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
+            const bool sure = key < lo;
+            ex = fmaxf(ex, sure ? a[e] : 0.0f);
+            ey = fmaxf(ey, sure ? b[e] : 0.0f);
+            if (key - lo < (1u << kBinShift)) {
+                if (nc < kCandCap) {
+                    hist[(nc * kWave + lane) * 2 + 0] = __float_as_uint(a[e]);
+                    hist[(nc * kWave + lane) * 2 + 1] = __float_as_uint(b[e]);
+                }
+                ++nc;
+            }
+        }
+        unknown = unknown || wave_max_i(nc) > kCandCap;
+        uint32_t eps_key = kUnknownEps;
+        bool sel_tie = false;
+        if (!unknown) {
+            // compact the candidates to one per lane (entries 512.. of the LDS row)
+            const int off = wave_incl_scan(nc) - nc;
+#pragma unroll
+            for (int t = 0; t < kCandCap; ++t)
+                if (t < nc) {
+                    const uint32_t ca = hist[(t * kWave + lane) * 2 + 0];
+                    const uint32_t cb = hist[(t * kWave + lane) * 2 + 1];
+                    hist[512 + (off + t) * 2 + 0] = ca;
+                    hist[512 + (off + t) * 2 + 1] = cb;
+                }
+            wave_sync();
+            const bool act = lane < pop;
+            float ca = 0.0f, cb = 0.0f;
+            uint32_t low = 0xffffffffu;
+            if (act) {
+                ca = __uint_as_float(hist[512 + lane * 2 + 0]);
+                cb = __uint_as_float(hist[512 + lane * 2 + 1]);
+                low = __float_as_uint(fmaxf(ca, cb)) - lo;   // < 2^17
+            }
+            // r0-th smallest of the candidates' low 17 bits, one bit at a time
+            uint32_t sel = 0;
+            int r = r0;
+            for (int bit = kBinShift - 1; bit >= 0; --bit) {
+                const bool zero = act && (low >> (bit + 1)) == sel && ((low >> bit) & 1u) == 0u;
+                const int c0 = __popcll(__ballot(zero));
+                sel <<= 1;
+                if (r > c0) {
+                    r -= c0;
+                    sel |= 1u;
+                }
+            }
+            eps_key = lo + sel;
+            const bool in = act && low <= sel;
+            sel_tie = __popcll(__ballot(in)) != r0;
+            ex = fmaxf(ex, in ? ca : 0.0f);
+            ey = fmaxf(ey, in ? cb : 0.0f);
         }
         ex = wave_max(ex);
         ey = wave_max(ey);
+        wave_sync();
 
-        // ---- marginal counts (self included, removed below) ---------------
+        // ---- sweep C: marginal counts (self included, removed below) ------
         int cx = 0, cy = 0;
 #pragma unroll
         for (int e = 0; e < 4 * S4; ++e) {
@@ -250,7 +327,7 @@ __global__ __launch_bounds__(256) void ksg_pair_kernel(KsgArgs args)
         cx = wave_sum(cx);
         cy = wave_sum(cy);
         const int nx = (cx & 0xffff) - 1, ny = (cy & 0xffff) - 1;
-        const bool flag = sel_tie || (cx >> 16) > 1 || (cy >> 16) > 1;
+        const bool flag = unknown || sel_tie || (cx >> 16) > 1 || (cy >> 16) > 1;
 
         if (lane == 0) {
             if (flag) {
@@ -281,15 +358,18 @@ __global__ __launch_bounds__(256) void ksg_pair_kernel(KsgArgs args)
 
 // ---------------------------------------------------------------------------
 // FP64 recount of queued rows: one wave per entry.  Recomputes dA, dB in
-// FP64 from the coordinates.  The FP32 key eps_bits fixes the group G of
-// frames whose joint distance rounds to it; the k-th neighbour distance is
-// the r-th smallest FP64 value in G with r = k - #{j != i : key < eps_bits}.
-// More than r frames at that FP64 value is an exact tie: all of them enter
-// the radius set (the tie is counted).
+// FP64 from the coordinates.  If the pair kernel left the key unresolved it
+// is found first by a three-digit radix select over the same FP32 keys.  The
+// key fixes the group G of frames whose joint distance rounds to it; the
+// k-th neighbour distance is the r-th smallest FP64 value in G with
+// r = k - #{j != i : key < eps}.  More than r frames at that FP64 value is
+// an exact tie: all of them enter the radius set (the tie is counted).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void ksg_fixup_kernel(KsgArgs args, unsigned nent)
 {
+    __shared__ uint32_t lds[4][kBins];
     const int lane = threadIdx.x & (kWave - 1);
+    uint32_t* hist = lds[threadIdx.x / kWave];
     const unsigned e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (e >= nent) return;
     const FixEntry fe = args.fixq[e];
@@ -302,14 +382,48 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(KsgArgs args, unsigned n
         A = fmax(fmax(fabs(qa0 - xa[j]), fabs(qa1 - xa[N + j])), fabs(qa2 - xa[2 * N + j]));
         B = fmax(fmax(fabs(qb0 - xb[j]), fabs(qb1 - xb[N + j])), fabs(qb2 - xb[2 * N + j]));
     };
-    const uint32_t ek = fe.eps_bits;
+    auto key_of = [&](int j) {
+        double A, B;
+        dist(j, A, B);
+        return __float_as_uint((float)fmax(A, B));
+    };
+
+    uint32_t ek = fe.eps_bits;
+    if (ek == kUnknownEps) {
+        // (k+1)-th smallest key, self included: digits 30..20, 19..10, 9..0
+        clear_hist<2048>(hist, lane);
+        wave_sync();
+        for (int j = lane; j < N; j += kWave) atomicAdd(&hist[key_of(j) >> 20], 1u);
+        wave_sync();
+        int bin, rank, pop;
+        find_bin<32>(hist, lane, K + 1, bin, rank, pop);
+        uint32_t prefix = (uint32_t)bin;
+        wave_sync();
+        clear_hist<1024>(hist, lane);
+        wave_sync();
+        for (int j = lane; j < N; j += kWave) {
+            const uint32_t key = key_of(j);
+            if ((key >> 20) == prefix) atomicAdd(&hist[(key >> 10) & 1023u], 1u);
+        }
+        wave_sync();
+        find_bin<16>(hist, lane, rank, bin, rank, pop);
+        prefix = (prefix << 10) | (uint32_t)bin;
+        wave_sync();
+        clear_hist<1024>(hist, lane);
+        wave_sync();
+        for (int j = lane; j < N; j += kWave) {
+            const uint32_t key = key_of(j);
+            if ((key >> 10) == prefix) atomicAdd(&hist[key & 1023u], 1u);
+        }
+        wave_sync();
+        find_bin<16>(hist, lane, rank, bin, rank, pop);
+        ek = (prefix << 10) | (uint32_t)bin;
+    }
 
     int clt = 0;
     for (int j = lane; j < N; j += kWave) {
         if (j == i) continue;
-        double A, B;
-        dist(j, A, B);
-        clt += __float_as_uint((float)fmax(A, B)) < ek;
+        clt += key_of(j) < ek;
     }
     clt = wave_sum(clt);
     const int r = K - clt;  // >= 1
@@ -327,7 +441,7 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(KsgArgs args, unsigned n
             const double J = fmax(A, B);
             if (__float_as_uint((float)J) == ek && J > prev) vmin = fmin(vmin, J);
         }
-        vmin = wave_min(vmin);
+        vmin = wave_min_d(vmin);
         int m = 0;
         for (int j = lane; j < N; j += kWave) {
             if (j == i) continue;
@@ -355,8 +469,8 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(KsgArgs args, unsigned n
             ey = fmax(ey, B);
         }
     }
-    ex = wave_max(ex);
-    ey = wave_max(ey);
+    ex = wave_max_d(ex);
+    ey = wave_max_d(ey);
     int nx = 0, ny = 0;
     for (int j = lane; j < N; j += kWave) {
         if (j == i) continue;
@@ -393,6 +507,23 @@ __global__ void ksg_finalize_kernel(KsgArgs args)
     if (p < args.npairs) args.mi[p] = args.psiN_phiK - (args.partial[p] + args.fixsum[p]) / args.N;
 }
 
+// Wave primitive self-check: out[0..63] inclusive scan of lane+1 (mod 7
+// pattern), out[64] sum, out[65] max, out[66] float max bits.
+__global__ void ksg_selftest_kernel(int* out)
+{
+    const int lane = threadIdx.x;
+    const int v = (lane * 5 + 3) % 7 + 1;
+    out[lane] = wave_incl_scan(v);
+    const int s = wave_sum(v);
+    const int m = wave_max_i((lane * 37) % 64);
+    const float f = wave_max((float)((lane * 11) % 64) * 0.5f);
+    if (lane == 0) {
+        out[64] = s;
+        out[65] = m;
+        out[66] = __float_as_int(f);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -416,7 +547,7 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s)
 {
     const int nchunk = (a.N + 3) / 4;
     const int s4 = (nchunk + kWave - 1) / kWave;
-    const dim3 grid((unsigned)a.npairs), block(kWave * kPairWaves);
+    const dim3 grid((unsigned)(((a.npairs + 7) / 8) * 8)), block(kWave * kPairWaves);
     if (s4 <= 1)
         hipLaunchKernelGGL(ksg_pair_kernel<1>, grid, block, 0, s, a);
     else if (s4 <= 2)
@@ -445,6 +576,12 @@ hipError_t launch_ksg_finalize(const KsgArgs& a, hipStream_t s)
 {
     hipLaunchKernelGGL(ksg_finalize_kernel, dim3((unsigned)((a.npairs + 255) / 256)), dim3(256), 0,
                        s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_selftest(int* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(ksg_selftest_kernel, dim3(1), dim3(kWave), 0, s, out);
     return hipGetLastError();
 }
 

@@ -18,20 +18,32 @@ using gmxcorr::FixEntry;
 using gmxcorr::KsgArgs;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string& msg)
+namespace gmxcorr {
+
+int set_error(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
 }
 
-int hip_fail(hipError_t e, const char* where)
+int set_hip_error(hipError_t e, const char* where)
 {
-    return fail(e == hipErrorOutOfMemory ? GMXCORR_ENOMEM : GMXCORR_EHIP,
-                std::string(where) + ": " + hipGetErrorString(e));
+    return set_error(e == hipErrorOutOfMemory ? GMXCORR_ENOMEM : GMXCORR_EHIP,
+                     std::string(where) + ": " + hipGetErrorString(e));
 }
+
+bool device_ok();
+
+}  // namespace gmxcorr
+
+namespace {
+
+int fail(int code, const std::string& msg) { return gmxcorr::set_error(code, msg); }
+
+int hip_fail(hipError_t e, const char* where) { return gmxcorr::set_hip_error(e, where); }
 
 #define HIPCHK(expr)                                        \
     do {                                                    \
@@ -47,6 +59,12 @@ bool device_is_gfx950()
     if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return false;
     return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
 }
+
+}  // namespace
+
+bool gmxcorr::device_ok() { return device_is_gfx950(); }
+
+namespace {
 
 template <class T>
 void dfree(T*& p)
@@ -167,6 +185,19 @@ const char* gmxcorr_last_error(void) { return g_err.c_str(); }
 int gmxcorr_probe(void) { return device_is_gfx950() ? 1 : 0; }
 
 int gmxcorr_ksg_max_frames(void) { return gmxcorr::kMaxFrames; }
+
+int gmxcorr_selftest_wave(int* out)
+{
+    if (!out) return fail(GMXCORR_EINVAL, "selftest: null output");
+    if (!device_is_gfx950()) return fail(GMXCORR_ENODEV, "no gfx950 device");
+    int* d = nullptr;
+    HIPCHK(hipMalloc(&d, 67 * sizeof(int)));
+    hipError_t e = gmxcorr::launch_selftest(d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, d, 67 * sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "selftest");
+    return GMXCORR_OK;
+}
 
 int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** out)
 {

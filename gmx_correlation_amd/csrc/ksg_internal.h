@@ -46,5 +46,6 @@ hipError_t launch_ksg_clear(const KsgArgs& a, hipStream_t s);
 hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s);
 hipError_t launch_ksg_fixup(const KsgArgs& a, unsigned nqueued, hipStream_t s);
 hipError_t launch_ksg_finalize(const KsgArgs& a, hipStream_t s);
+hipError_t launch_selftest(int* out, hipStream_t s);
 
 }  // namespace gmxcorr

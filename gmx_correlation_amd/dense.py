@@ -1,0 +1,62 @@
+"""Host-side mirror of the reference's Gaussian and GCMI matrices, backed by
+the HIP library (no CPU path).
+
+  gauss_corrmatrix()  lib/correlation_core.cpp:280-311  (-linear)
+  gcmi_corrmatrix()   lib/gcmi.cpp:246-317              (-gcmi)
+
+Both take t_traj.x [natoms][nframes][3] (mean-centred) and return the
+[natoms][natoms] MI matrix in nats with the 2000 diagonal.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _traj(traj) -> np.ndarray:
+    t = np.ascontiguousarray(traj, dtype=np.float64)
+    if t.ndim != 3 or t.shape[2] != 3:
+        raise ValueError("traj must be [natoms][nframes][3]")
+    return t
+
+
+def gauss_corrmatrix(traj, return_cov: bool = False, stats: bool = False):
+    t = _traj(traj)
+    n, N, _ = t.shape
+    mat = np.zeros((n, n))
+    cov = np.zeros((3 * n, 3 * n)) if return_cov else None
+    st = _lib.DenseStats()
+    _lib.check(_lib.load().gmxcorr_gauss_matrix(_lib.ptr(t), n, N, _lib.ptr(mat), _lib.ptr(cov),
+                                                ctypes.byref(st)))
+    out = [mat]
+    if return_cov:
+        out.append(cov)
+    if stats:
+        out.append(st.as_dict())
+    return out[0] if len(out) == 1 else tuple(out)
+
+
+def gcmi_corrmatrix(traj, return_cov: bool = False, return_z: bool = False, stats: bool = False):
+    t = _traj(traj)
+    n, N, _ = t.shape
+    if n < 2:
+        raise ValueError("gcmi_corrmatrix: need at least 2 atoms")
+    if N < 6:
+        raise ValueError("gcmi_corrmatrix: need at least 6 frames for a non-singular 6x6 covariance")
+    mat = np.zeros((n, n))
+    cov = np.zeros((3 * n, 3 * n)) if return_cov else None
+    z = np.zeros((3 * n, N)) if return_z else None
+    st = _lib.DenseStats()
+    _lib.check(_lib.load().gmxcorr_gcmi_matrix(_lib.ptr(t), n, N, _lib.ptr(mat), _lib.ptr(cov),
+                                               _lib.ptr(z), ctypes.byref(st)))
+    out = [mat]
+    if return_cov:
+        out.append(cov)
+    if return_z:
+        out.append(z)
+    if stats:
+        out.append(st.as_dict())
+    return out[0] if len(out) == 1 else tuple(out)

@@ -31,6 +31,10 @@ const char* gmxcorr_last_error(void);
  * (lib/kraskov_gpu.h:20). */
 int gmxcorr_probe(void);
 
+/* Runs the wave-level scan/reduction primitives of the kernels once on
+ * device 0 and copies their 67 results to out (test hook). */
+int gmxcorr_selftest_wave(int* out);
+
 /* Largest frame count the KSG kernels accept. */
 int gmxcorr_ksg_max_frames(void);
 
@@ -76,6 +80,32 @@ int  gmxcorr_ksg_plan_result(gmxcorr_ksg_plan* plan, double* mi, long long npair
 int  gmxcorr_ksg_plan_matrix(gmxcorr_ksg_plan* plan, double* mat);
 int  gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* plan, gmxcorr_ksg_stats* st);
 void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* plan);
+
+/* ---- Gaussian and Gaussian-copula matrices --------------------------------
+ * traj: mean-centred fluctuations [natoms][nframes][3] (t_traj.x flattened).
+ * mat:  [natoms*natoms], diagonal 2000, symmetric MI in nats.
+ * The second-moment matrix V^T V / N (3n x 3n) is a split-precision FP16
+ * MFMA GEMM with FP32 accumulation; log-determinants are FP64.
+ * cov_out (optional, [3n*3n]) receives that matrix; z_out (optional,
+ * [3n][nframes]) the copula scores. */
+typedef struct {
+    double ms_load;      /* host->device copy + layout change */
+    double ms_copula;    /* copula transform (GCMI only) */
+    double ms_split;     /* FP16 hi/lo split */
+    double ms_gemm;      /* MFMA second-moment GEMM */
+    double ms_pairs;     /* per-pair FP64 log-determinants */
+    double gemm_flops;   /* 2 * (3n)^2 * N, counted once */
+    int natoms, nframes;
+} gmxcorr_dense_stats;
+
+/* lib/correlation_core.cpp:280-311 (gauss_corrmatrix). */
+int gmxcorr_gauss_matrix(const double* traj, int natoms, int nframes, double* mat,
+                         double* cov_out, gmxcorr_dense_stats* st);
+/* lib/gcmi.cpp:246-317 (gcmi_corrmatrix); the reference's GPU signature for
+ * this estimator is gcmi_corrmatrix_metal (lib/gcmi_metal.h), which takes the
+ * copula scores; here the transform runs on the GPU too.  nframes >= 6. */
+int gmxcorr_gcmi_matrix(const double* traj, int natoms, int nframes, double* mat,
+                        double* cov_out, double* z_out, gmxcorr_dense_stats* st);
 
 #ifdef __cplusplus
 }
