@@ -81,6 +81,69 @@ def cpu_baseline(coords: np.ndarray, k: int, natoms: int, budget_s: float) -> di
     }
 
 
+def bench_dense(args) -> int:
+    """Gaussian and GCMI at 1000 atoms x 10000 frames (Student-t noise):
+    split-FP16 MFMA GEMM time and its error against an FP64 GEMM."""
+    from gmx_correlation_amd import dense, ksg, synth
+    from oracle import oracle as O
+    spec = synth.DENSE_1000
+    log(f"bench: generating workload {spec}")
+    traj = ksg.center(synth.generate(spec))
+    n, N = spec.natoms, spec.nframes
+    out = {"workload": "gauss+gcmi 1000 atoms x 10000 frames", "natoms": n, "nframes": N}
+    for name, fn in (("gauss", dense.gauss_corrmatrix), ("gcmi", dense.gcmi_corrmatrix)):
+        runs = []
+        for s in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            res = fn(traj, return_cov=True, stats=True)
+            runs.append((time.perf_counter() - t0, res))
+        mat, cov, st = runs[-1][1]
+        gemm_ms = float(np.mean([r[1][2]["ms_gemm"] for r in runs[args.warmup:]]))
+        V = traj.transpose(1, 0, 2).reshape(N, -1) if name == "gauss" else None
+        if name == "gcmi":
+            V = O.copula(traj).T
+        t0 = time.perf_counter()
+        c64 = V.T @ V / N
+        cpu_gemm_s = time.perf_counter() - t0
+        d = np.sqrt(np.outer(np.diag(c64), np.diag(c64)))
+        rel = float(np.max(np.abs(cov - c64) / d))
+        mi64 = O.gauss_matrix_from_cov(c64, n) if name == "gauss" else None
+        if name == "gcmi":
+            mi64 = _gcmi_from_cov(c64, n)
+        iu = np.triu_indices(n, 1)
+        out[name] = {
+            "ms_total_host": float(np.mean([r[0] for r in runs[args.warmup:]]) * 1e3),
+            "ms_gemm": gemm_ms, "ms_copula": st["ms_copula"], "ms_pairs": st["ms_pairs"],
+            "gemm_tflops": st["gemm_flops"] / (gemm_ms * 1e-3) / 1e12,
+            "mfma_frac_fp16_dense_2.5PF": st["gemm_flops"] / (gemm_ms * 1e-3) / 2.5e15,
+            "cov_max_rel_err_vs_fp64": rel,
+            "mi_max_abs_err_vs_fp64_gemm": float(np.max(np.abs(mat[iu] - mi64[iu]))),
+            "numpy_fp64_gemm_s": cpu_gemm_s,
+        }
+        log(f"bench: {name}: {out[name]}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def _gcmi_from_cov(c, n):
+    """GCMI pair MI from an FP64 second-moment matrix, batched determinants."""
+    a1, a2 = np.triu_indices(n, 1)
+    idx1 = 3 * a1[:, None] + np.arange(3)[None, :]
+    idx2 = 3 * a2[:, None] + np.arange(3)[None, :]
+    idx = np.concatenate([idx1, idx2], axis=1)                 # [npairs][6]
+    cj = c[idx[:, :, None], idx[:, None, :]]                   # [npairs][6][6]
+    d1 = np.linalg.det(cj[:, :3, :3])
+    d2 = np.linalg.det(cj[:, 3:, 3:])
+    dj = np.linalg.det(cj)
+    ok = (d1 > 0) & (d2 > 0) & (dj > 0)
+    v = np.where(ok, 0.5 * (np.log(np.where(ok, d1, 1)) + np.log(np.where(ok, d2, 1))
+                            - np.log(np.where(ok, dj, 1))), 0.0)
+    mat = np.full((n, n), 2000.0)
+    mat[a1, a2] = v
+    mat[a2, a1] = v
+    return mat
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,12 +151,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU-baseline work (0 disables)")
+    ap.add_argument("--workload", default="ksg", choices=["ksg", "dense"],
+                    help="ksg: the headline metric; dense: Gaussian/GCMI GEMM path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU pair sharding is a design in DESIGN.md, not a measured path.",
               file=sys.stderr)
         return 2
+    if args.workload == "dense":
+        return bench_dense(args)
 
     from gmx_correlation_amd import _lib, ksg, synth
 
