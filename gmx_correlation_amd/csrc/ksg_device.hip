@@ -357,6 +357,190 @@ __global__ __launch_bounds__(256) void ksg_pair_kernel(KsgArgs args)
 }
 
 // ---------------------------------------------------------------------------
+// Cooperative pair kernel for long trajectories: the 4 waves of a block share
+// one frame row, wave w holding frames [w*256*S4, (w+1)*256*S4) in registers
+// (N <= 1024*S4).  Same three sweeps as ksg_pair_kernel with a block-shared
+// histogram; wave 0 resolves the boundary candidates.  Five barriers per row.
+// ---------------------------------------------------------------------------
+template <int S4>
+__global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
+{
+    __shared__ uint32_t hist[kBins];                 // histogram, then candidate slots
+    __shared__ uint32_t dense[kPairWaves][kWave][2]; // per-wave compacted candidates
+    __shared__ int sh_nc[kPairWaves], sh_over[kPairWaves], sh_cx[kPairWaves], sh_cy[kPairWaves];
+    __shared__ float sh_ex[kPairWaves], sh_ey[kPairWaves];
+    __shared__ float res_ex, res_ey;
+    __shared__ int res_flag;
+    __shared__ uint32_t res_eps;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    const long long per_xcd = (args.npairs + 7) / 8;
+    const long long p = (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (p >= args.npairs) return;
+    const int N = args.N, T = args.K + 1;
+    const long long P = args.P;
+    const float* Da = args.D + (size_t)args.a1[p] * N * P;
+    const float* Db = args.D + (size_t)args.a2[p] * N * P;
+    const int nchunk = (N + 3) / 4;
+    uint32_t* slots = hist + w * 512;
+    double acc = 0.0;
+
+    for (int i = 0; i < N; ++i) {
+        const float4* ra = reinterpret_cast<const float4*>(Da + (size_t)i * P);
+        const float4* rb = reinterpret_cast<const float4*>(Db + (size_t)i * P);
+        float a[4 * S4], b[4 * S4];
+#pragma unroll
+        for (int m = 0; m < S4; ++m) {
+            const int c = (w * S4 + m) * kWave + lane;
+            float4 va = make_float4(INFINITY, INFINITY, INFINITY, INFINITY), vb = va;
+            if (c < nchunk) {
+                va = ra[c];
+                vb = rb[c];
+            }
+            a[4 * m + 0] = va.x; a[4 * m + 1] = va.y; a[4 * m + 2] = va.z; a[4 * m + 3] = va.w;
+            b[4 * m + 0] = vb.x; b[4 * m + 1] = vb.y; b[4 * m + 2] = vb.z; b[4 * m + 3] = vb.w;
+        }
+        reinterpret_cast<uint4*>(hist)[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+        reinterpret_cast<uint4*>(hist)[threadIdx.x + 256] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 4 * S4; ++e) {
+            const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
+            const int bin = min(max((int)(key >> kBinShift) - kBinBase, 0), kBins - 1);
+            atomicAdd(&hist[bin], 1u);
+        }
+        __syncthreads();
+        int b1, r0, pop;
+        find_bin<kBins / kWave>(hist, lane, T, b1, r0, pop);
+        const bool edge = b1 == 0 || b1 == kBins - 1 || pop > kWave;
+        const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
+        __syncthreads();
+
+        float ex = 0.0f, ey = 0.0f;
+        int nc = 0;
+#pragma unroll
+        for (int e = 0; e < 4 * S4; ++e) {
+            const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
+            const bool sure = key < lo;
+            ex = fmaxf(ex, sure ? a[e] : 0.0f);
+            ey = fmaxf(ey, sure ? b[e] : 0.0f);
+            if (key - lo < (1u << kBinShift)) {
+                if (nc < kCandCap) {
+                    slots[(nc * kWave + lane) * 2 + 0] = __float_as_uint(a[e]);
+                    slots[(nc * kWave + lane) * 2 + 1] = __float_as_uint(b[e]);
+                }
+                ++nc;
+            }
+        }
+        const int ncmax = wave_max_i(nc);
+        const int ncw = wave_sum(nc);
+        if (ncmax <= kCandCap && !edge) {
+            const int off = wave_incl_scan(nc) - nc;
+#pragma unroll
+            for (int t = 0; t < kCandCap; ++t)
+                if (t < nc && off + t < kWave) {
+                    dense[w][off + t][0] = slots[(t * kWave + lane) * 2 + 0];
+                    dense[w][off + t][1] = slots[(t * kWave + lane) * 2 + 1];
+                }
+        }
+        ex = wave_max(ex);
+        ey = wave_max(ey);
+        if (lane == 0) {
+            sh_nc[w] = ncw;
+            sh_over[w] = ncmax > kCandCap;
+            sh_ex[w] = ex;
+            sh_ey[w] = ey;
+        }
+        __syncthreads();
+
+        if (w == 0) {
+            bool unknown = edge || sh_over[0] || sh_over[1] || sh_over[2] || sh_over[3];
+            float rex = fmaxf(fmaxf(sh_ex[0], sh_ex[1]), fmaxf(sh_ex[2], sh_ex[3]));
+            float rey = fmaxf(fmaxf(sh_ey[0], sh_ey[1]), fmaxf(sh_ey[2], sh_ey[3]));
+            uint32_t eps_key = kUnknownEps;
+            bool sel_tie = false;
+            if (!unknown) {
+                const int c0 = sh_nc[0], c1 = c0 + sh_nc[1], c2 = c1 + sh_nc[2];
+                const bool act = lane < pop;
+                float ca = 0.0f, cb = 0.0f;
+                uint32_t low = 0xffffffffu;
+                if (act) {
+                    const int q = lane < c0 ? 0 : lane < c1 ? 1 : lane < c2 ? 2 : 3;
+                    const int l = lane - (q == 0 ? 0 : q == 1 ? c0 : q == 2 ? c1 : c2);
+                    ca = __uint_as_float(dense[q][l][0]);
+                    cb = __uint_as_float(dense[q][l][1]);
+                    low = __float_as_uint(fmaxf(ca, cb)) - lo;
+                }
+                uint32_t sel = 0;
+                int r = r0;
+                for (int bit = kBinShift - 1; bit >= 0; --bit) {
+                    const bool zero = act && (low >> (bit + 1)) == sel && ((low >> bit) & 1u) == 0u;
+                    const int cz = __popcll(__ballot(zero));
+                    sel <<= 1;
+                    if (r > cz) {
+                        r -= cz;
+                        sel |= 1u;
+                    }
+                }
+                eps_key = lo + sel;
+                const bool in = act && low <= sel;
+                sel_tie = __popcll(__ballot(in)) != r0;
+                rex = fmaxf(rex, wave_max(in ? ca : 0.0f));
+                rey = fmaxf(rey, wave_max(in ? cb : 0.0f));
+            }
+            if (lane == 0) {
+                res_ex = rex;
+                res_ey = rey;
+                res_flag = (unknown || sel_tie) ? 1 : 0;
+                res_eps = eps_key;
+            }
+        }
+        __syncthreads();
+        ex = res_ex;
+        ey = res_ey;
+        int cx = 0, cy = 0;
+#pragma unroll
+        for (int e = 0; e < 4 * S4; ++e) {
+            cx += (a[e] <= ex) + ((a[e] == ex) << 16);
+            cy += (b[e] <= ey) + ((b[e] == ey) << 16);
+        }
+        cx = wave_sum(cx);
+        cy = wave_sum(cy);
+        if (lane == 0) {
+            sh_cx[w] = cx;
+            sh_cy[w] = cy;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int tx = sh_cx[0] + sh_cx[1] + sh_cx[2] + sh_cx[3];
+            const int ty = sh_cy[0] + sh_cy[1] + sh_cy[2] + sh_cy[3];
+            const int nx = (tx & 0xffff) - 1, ny = (ty & 0xffff) - 1;
+            const bool flag = res_flag || (tx >> 16) > 1 || (ty >> 16) > 1;
+            if (flag) {
+                const unsigned slot = atomicAdd(&args.counters[0], 1u);
+                if (slot < args.fixcap) {
+                    FixEntry fe;
+                    fe.pair = (int)p;
+                    fe.row = i;
+                    fe.eps_bits = res_eps;
+                    fe.pad = 0;
+                    args.fixq[slot] = fe;
+                }
+            } else {
+                acc += args.psi[nx] + args.psi[ny];
+            }
+            if (args.rc_nx) {
+                const size_t o = (size_t)p * N + i;
+                args.rc_nx[o] = flag ? -1 : nx;
+                args.rc_ny[o] = flag ? -1 : ny;
+                args.rc_flag[o] = 0;
+            }
+        }
+    }
+    if (threadIdx.x == 0) args.partial[p] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 recount of queued rows: one wave per entry.  Recomputes dA, dB in
 // FP64 from the coordinates.  If the pair kernel left the key unresolved it
 // is found first by a three-digit radix select over the same FP32 keys.  The
@@ -557,7 +741,9 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s)
     else if (s4 <= 8)
         hipLaunchKernelGGL(ksg_pair_kernel<8>, grid, block, 0, s, a);
     else if (s4 <= 16)
-        hipLaunchKernelGGL(ksg_pair_kernel<16>, grid, block, 0, s, a);
+        hipLaunchKernelGGL(ksg_pair_coop_kernel<4>, grid, block, 0, s, a);
+    else if (s4 <= 32)
+        hipLaunchKernelGGL(ksg_pair_coop_kernel<8>, grid, block, 0, s, a);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

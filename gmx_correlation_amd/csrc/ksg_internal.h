@@ -37,8 +37,8 @@ struct KsgArgs {
     double        psiN_phiK; // psi[N] + (psi[K] - 1/K)
 };
 
-// Largest N handled by the register-resident pair kernel.
-constexpr int kMaxFrames = 4096;
+// Largest N handled by the pair kernels (cooperative kernel, 4 x 2048 frames).
+constexpr int kMaxFrames = 8192;
 
 hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, float* D,
                              hipStream_t s);

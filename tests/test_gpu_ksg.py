@@ -160,6 +160,24 @@ def test_readme_workload_matches_reference(gpu, oracle_lib):
         assert np.array_equal(d["ny"][p], ref_d["ny"])
 
 
+@pytest.mark.parametrize("N,k,npairs", [(3000, 50, 2), (5000, 100, 1), (8000, 10, 1)])
+def test_long_trajectories_cooperative_kernel(gpu, oracle_lib, N, k, npairs):
+    """N > 2048 runs the 4-waves-per-row kernel; counts and MI against the
+    FP64 brute-force recount."""
+    c = synth.generate(synth.TrajSpec(natoms=5, nframes=N, nmodes=4, seed=N))
+    rows, psi = ksg.kraskov_prepare(ksg.center(c))
+    a1 = np.array([1, 4])[:npairs]; a2 = np.array([0, 2])[:npairs]
+    with ksg.KsgPlan(5, N, k) as plan:
+        plan.load(rows, psi)
+        mi, d = plan.run_pairs(a1, a2, detail=True)
+    for p in range(npairs):
+        v, ref = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), k, method="brute",
+                                     detail=True)
+        assert np.array_equal(d["nx"][p], ref["nx"])
+        assert np.array_equal(d["ny"][p], ref["ny"])
+        assert abs(mi[p] - v) <= 1e-12
+
+
 def test_cxx_boundary_entry_point(gpu, small_ws):
     """Call the exported kraskov_corrmatrix_gpu(const t_kraskov*, int,
     double*, int) with a t_kraskov laid out as lib/correlation_core.h:45-52."""
