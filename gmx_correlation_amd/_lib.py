@@ -19,6 +19,7 @@ STATUS_NAMES = {-1: "EINVAL", -2: "ENODEV", -3: "ENOMEM", -4: "EHIP", -5: "EUNSU
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _u8p = ctypes.POINTER(ctypes.c_ubyte)
+_fp = ctypes.POINTER(ctypes.c_float)
 _i = ctypes.c_int
 _ll = ctypes.c_longlong
 
@@ -73,6 +74,9 @@ C_SYMBOLS = [
     "gmxcorr_ksg_plan_run_pairs", "gmxcorr_ksg_plan_result", "gmxcorr_ksg_plan_matrix",
     "gmxcorr_ksg_plan_stats", "gmxcorr_ksg_plan_destroy",
     "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix",
+    "gmxcorr_ksg_plan_features", "gmxcorr_mlp_num_params", "gmxcorr_mlp_create",
+    "gmxcorr_mlp_set_params", "gmxcorr_mlp_get_params", "gmxcorr_mlp_grad",
+    "gmxcorr_mlp_train_epoch", "gmxcorr_mlp_forward", "gmxcorr_mlp_destroy",
 ]
 CXX_SYMBOLS = ["_Z17kraskov_gpu_probev", "_Z22kraskov_corrmatrix_gpuPK9t_kraskoviPdi"]
 
@@ -103,6 +107,18 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_ksg_plan_destroy.restype = None
     L.gmxcorr_gauss_matrix.argtypes = [_dp, _i, _i, _dp, _dp, ctypes.POINTER(DenseStats)]
     L.gmxcorr_gcmi_matrix.argtypes = [_dp, _i, _i, _dp, _dp, _dp, ctypes.POINTER(DenseStats)]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    _vp = ctypes.c_void_p
+    L.gmxcorr_ksg_plan_features.argtypes = [_vp, _ll, _ip, _ip, _i, _fp, _dp]
+    L.gmxcorr_mlp_num_params.restype = _i
+    L.gmxcorr_mlp_create.argtypes = [_i, ctypes.POINTER(ctypes.c_void_p)]
+    L.gmxcorr_mlp_set_params.argtypes = [_vp, _fp]
+    L.gmxcorr_mlp_get_params.argtypes = [_vp, _fp]
+    L.gmxcorr_mlp_grad.argtypes = [_vp, _fp, _fp, _i, _fp, _fp]
+    L.gmxcorr_mlp_train_epoch.argtypes = [_vp, _fp, _fp, _ll, _i, ctypes.c_float, _fp]
+    L.gmxcorr_mlp_forward.argtypes = [_vp, _fp, _ll, _i, _fp, _dp]
+    L.gmxcorr_mlp_destroy.argtypes = [_vp]
+    L.gmxcorr_mlp_destroy.restype = None
     _lib = L
     return L
 

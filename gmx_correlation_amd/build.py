@@ -18,7 +18,10 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB_NAME = "libgmxcorr_hip.so"
 LIB_PATH = os.path.join(PKG, LIB_NAME)
 
-SOURCES = ["ksg_device.hip", "ksg_host.hip", "dense_device.hip", "dense_host.hip"]
+SOURCES = [
+    "ksg_device.hip", "ksg_host.hip", "dense_device.hip", "dense_host.hip",
+    "surrogate_device.hip", "surrogate_host.hip",
+]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

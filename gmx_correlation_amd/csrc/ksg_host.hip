@@ -178,6 +178,21 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail)
 
 }  // namespace
 
+namespace gmxcorr {
+bool plan_view(const gmxcorr_ksg_plan* pl, PlanView* v)
+{
+    if (!pl || !pl->loaded) return false;
+    v->X = pl->dX;
+    v->D = pl->dD;
+    v->natoms = pl->natoms;
+    v->N = pl->N;
+    v->K = pl->K;
+    v->P = pl->P;
+    v->stream = pl->stream;
+    return true;
+}
+}  // namespace gmxcorr
+
 extern "C" {
 
 const char* gmxcorr_last_error(void) { return g_err.c_str(); }

@@ -40,6 +40,15 @@ struct KsgArgs {
 // Largest N handled by the pair kernels (cooperative kernel, 4 x 2048 frames).
 constexpr int kMaxFrames = 8192;
 
+// Read-only view of a loaded plan's device data (for the surrogate features).
+struct PlanView {
+    const double* X;   // [3n][N] normalised rows
+    const float* D;    // [n][N][P] distance rows
+    int natoms, N, K;
+    long long P;
+    hipStream_t stream;
+};
+
 hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, float* D,
                              hipStream_t s);
 hipError_t launch_ksg_clear(const KsgArgs& a, hipStream_t s);

@@ -107,6 +107,30 @@ int gmxcorr_gauss_matrix(const double* traj, int natoms, int nframes, double* ma
 int gmxcorr_gcmi_matrix(const double* traj, int natoms, int nframes, double* mat,
                         double* cov_out, double* z_out, gmxcorr_dense_stats* st);
 
+/* ---- Learned surrogate (no counterpart in the reference) -------------------
+ * Features of a pair list on a loaded KSG plan: feat [npairs][16] holds the
+ * Gaussian MI, the six eigenvalues of the pair's 6x6 correlation matrix,
+ * min/max of the atoms' marginal excess kurtosis, min/max of the atoms'
+ * median and 90th-percentile k-NN radius (nsample frames per atom), r of the
+ * Gaussian MI, and two zeros.  ms (optional): device time. */
+int gmxcorr_ksg_plan_features(gmxcorr_ksg_plan* plan, long long npairs, const int* a1,
+                              const int* a2, int nsample, float* feat, double* ms);
+
+/* MLP 16 -> 64 -> 64 -> 2 (GELU), outputs (mu, log sigma^2) of r(MI); FP32
+ * master weights trained on the GPU with Adam; BF16 matrix-core inference. */
+typedef struct gmxcorr_mlp gmxcorr_mlp;
+int  gmxcorr_mlp_num_params(void);
+int  gmxcorr_mlp_create(int max_batch, gmxcorr_mlp** out);
+int  gmxcorr_mlp_set_params(gmxcorr_mlp* m, const float* params);
+int  gmxcorr_mlp_get_params(gmxcorr_mlp* m, float* params);
+int  gmxcorr_mlp_grad(gmxcorr_mlp* m, const float* x, const float* y, int batch, float* grad,
+                      float* loss);
+int  gmxcorr_mlp_train_epoch(gmxcorr_mlp* m, const float* x, const float* y, long long n,
+                             int batch, float lr, float* mean_loss);
+int  gmxcorr_mlp_forward(gmxcorr_mlp* m, const float* x, long long n, int bf16, float* out,
+                         double* ms);
+void gmxcorr_mlp_destroy(gmxcorr_mlp* m);
+
 #ifdef __cplusplus
 }
 #endif
