@@ -60,6 +60,22 @@ class DenseStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double),
+        ("ms_build", ctypes.c_double),
+        ("ms_pairs", ctypes.c_double),
+        ("ms_fixup", ctypes.c_double),
+        ("npairs", _ll),
+        ("fixup_rows", _ll),
+        ("ties", _ll),
+        ("nwaves", _i),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class GmxcorrError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"gmxcorr {STATUS_NAMES.get(code, code)}: {msg}")
@@ -72,8 +88,11 @@ C_SYMBOLS = [
     "gmxcorr_ksg_matrix",
     "gmxcorr_ksg_plan_create", "gmxcorr_ksg_plan_load", "gmxcorr_ksg_plan_run_all",
     "gmxcorr_ksg_plan_run_pairs", "gmxcorr_ksg_plan_result", "gmxcorr_ksg_plan_matrix",
-    "gmxcorr_ksg_plan_stats", "gmxcorr_ksg_plan_destroy",
+    "gmxcorr_ksg_plan_stats", "gmxcorr_ksg_plan_destroy", "gmxcorr_ksg_plan_set_variant",
+    "gmxcorr_ksg_variant_fits",
     "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix",
+    "gmxcorr_ksg_batch_create", "gmxcorr_ksg_batch_add", "gmxcorr_ksg_batch_run",
+    "gmxcorr_ksg_batch_result", "gmxcorr_ksg_batch_stats", "gmxcorr_ksg_batch_destroy",
     "gmxcorr_ksg_plan_features", "gmxcorr_mlp_num_params", "gmxcorr_mlp_create",
     "gmxcorr_mlp_set_params", "gmxcorr_mlp_get_params", "gmxcorr_mlp_grad",
     "gmxcorr_mlp_train_epoch", "gmxcorr_mlp_forward", "gmxcorr_mlp_destroy",
@@ -105,6 +124,8 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_ksg_plan_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(KsgStats)]
     L.gmxcorr_ksg_plan_destroy.argtypes = [ctypes.c_void_p]
     L.gmxcorr_ksg_plan_destroy.restype = None
+    L.gmxcorr_ksg_plan_set_variant.argtypes = [ctypes.c_void_p, _i, _i]
+    L.gmxcorr_ksg_variant_fits.argtypes = [_i, _i, _i]
     L.gmxcorr_gauss_matrix.argtypes = [_dp, _i, _i, _dp, _dp, ctypes.POINTER(DenseStats)]
     L.gmxcorr_gcmi_matrix.argtypes = [_dp, _i, _i, _dp, _dp, _dp, ctypes.POINTER(DenseStats)]
     _fp = ctypes.POINTER(ctypes.c_float)
@@ -119,6 +140,13 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_mlp_forward.argtypes = [_vp, _fp, _ll, _i, _fp, _dp]
     L.gmxcorr_mlp_destroy.argtypes = [_vp]
     L.gmxcorr_mlp_destroy.restype = None
+    L.gmxcorr_ksg_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    L.gmxcorr_ksg_batch_add.argtypes = [_vp, _dp, _dp, _i, _i, _i, _i, _i]
+    L.gmxcorr_ksg_batch_run.argtypes = [_vp, _ll, _i]
+    L.gmxcorr_ksg_batch_result.argtypes = [_vp, _i, _dp, _ll, _dp, _ip]
+    L.gmxcorr_ksg_batch_stats.argtypes = [_vp, ctypes.POINTER(BatchStats)]
+    L.gmxcorr_ksg_batch_destroy.argtypes = [_vp]
+    L.gmxcorr_ksg_batch_destroy.restype = None
     _lib = L
     return L
 

@@ -1,0 +1,132 @@
+"""Batched KSG matrices of many independent trajectories (uneven atom and
+frame counts, different k) with one persistent pair-kernel launch per memory
+wave, plus the autotuner that picks each trajectory's kernel path.
+
+No counterpart in the reference, which computes one trajectory per run
+(lib/correlation_core.cpp:324-459); every trajectory's MI follows the same
+KSG-2 definition (lib/kraskov.cpp:1228-1331).
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from .ksg import KsgPlan
+
+# Code paths compiled into the persistent batch kernel: (kind, s4) with
+# kind 0 = one wave per frame row (N <= 256*s4), kind 1 = four waves per row
+# (N <= 1024*s4).
+BATCH_PATHS = [(0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (1, 4), (1, 6), (1, 8)]
+
+
+def fitting_paths(nframes: int) -> list[tuple[int, int]]:
+    return [(k, s) for k, s in BATCH_PATHS if nframes <= (256 if k == 0 else 1024) * s]
+
+
+def tune_variant(rows: np.ndarray, psi: np.ndarray, k: int, natoms_probe: int = 6,
+                 reps: int = 2, paths=None) -> tuple[tuple[int, int], dict]:
+    """Time every batch code path that covers the trajectory's frame count on
+    the pairs of its first `natoms_probe` atoms (standalone launches, HIP
+    events) and return the fastest (kind, s4) with all timings in ms."""
+    N = rows.shape[1]
+    m = min(natoms_probe, rows.shape[0] // 3)
+    cands = paths if paths is not None else fitting_paths(N)
+    # keep only the two smallest of each kind: larger register tiles only add work
+    by_kind: dict[int, list[int]] = {}
+    for kind, s4 in sorted(cands):
+        by_kind.setdefault(kind, []).append(s4)
+    cands = [(kd, s) for kd, ss in by_kind.items() for s in ss[:2]]
+    a1, a2 = np.tril_indices(m, -1)
+    timings = {}
+    with KsgPlan(m, N, k) as plan:
+        plan.load(rows[:3 * m], psi)
+        for kind, s4 in cands:
+            plan.set_variant(kind, s4)
+            plan.run_pairs(a1, a2)                      # warm-up
+            t = []
+            for _ in range(reps):
+                plan.run_pairs(a1, a2)
+                t.append(plan.stats()["ms_pairs"])
+            timings[f"{kind}:{s4}"] = float(np.min(t))
+    best = min(timings, key=timings.get)
+    kind, s4 = (int(v) for v in best.split(":"))
+    return (kind, s4), timings
+
+
+class Tuner:
+    """Per-(frame count, k) cache of tuned paths, optionally kept in a JSON
+    file between runs."""
+
+    def __init__(self, path: str | None = None):
+        self.path = path
+        self.cache: dict[str, dict] = {}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.cache = json.load(f)
+
+    def choose(self, rows, psi, k) -> tuple[int, int]:
+        key = f"{rows.shape[1]}:{k}"
+        if key not in self.cache:
+            (kind, s4), timings = tune_variant(rows, psi, k)
+            self.cache[key] = {"kind": kind, "s4": s4, "ms": timings}
+            if self.path:
+                os.makedirs(os.path.dirname(os.path.abspath(self.path)), exist_ok=True)
+                with open(self.path, "w") as f:
+                    json.dump(self.cache, f, indent=1)
+        c = self.cache[key]
+        return c["kind"], c["s4"]
+
+
+class KsgBatch:
+    def __init__(self):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.gmxcorr_ksg_batch_create(ctypes.byref(h)))
+        self._h = h
+        self.shapes: list[tuple[int, int, int]] = []
+
+    def add(self, rows: np.ndarray, psi: np.ndarray, k: int, kind: int = -1, s4: int = 0) -> int:
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        natoms, N = rows.shape[0] // 3, rows.shape[1]
+        rc = _lib.load().gmxcorr_ksg_batch_add(self._h, _lib.ptr(rows), _lib.ptr(psi), natoms, N,
+                                               k, kind, s4)
+        if rc < 0:
+            _lib.check(rc)
+        self.shapes.append((natoms, N, k))
+        return rc
+
+    def run(self, max_row_bytes: int = 200 * 10**9, nblocks: int = 2048) -> dict:
+        t0 = time.perf_counter()
+        _lib.check(_lib.load().gmxcorr_ksg_batch_run(self._h, int(max_row_bytes), nblocks))
+        st = _lib.BatchStats()
+        _lib.check(_lib.load().gmxcorr_ksg_batch_stats(self._h, ctypes.byref(st)))
+        out = st.as_dict()
+        out["wall_s"] = time.perf_counter() - t0
+        return out
+
+    def result(self, t: int) -> tuple[np.ndarray, float, int]:
+        natoms = self.shapes[t][0]
+        n = natoms * (natoms - 1) // 2
+        mi = np.empty(n)
+        ms = ctypes.c_double(0.0)
+        wave = ctypes.c_int(0)
+        _lib.check(_lib.load().gmxcorr_ksg_batch_result(self._h, t, _lib.ptr(mi), n,
+                                                        ctypes.byref(ms), ctypes.byref(wave)))
+        return mi, ms.value, wave.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().gmxcorr_ksg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,310 @@
+// Host side of the batched KSG run: many trajectories of uneven atom and
+// frame counts, all their atom pairs processed by one persistent kernel
+// launch per memory wave (distance rows of all trajectories of a wave
+// resident in HBM together).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "gmxcorr.h"
+#include "ksg_internal.h"
+
+namespace gmxcorr {
+int set_error(int code, const std::string& msg);
+int set_hip_error(hipError_t e, const char* where);
+bool device_ok();
+}  // namespace gmxcorr
+
+namespace {
+
+#define HIPCHK(expr)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) return gmxcorr::set_hip_error(e_, #expr);       \
+    } while (0)
+
+struct HostTraj {
+    int natoms = 0, N = 0, K = 0, kind = -1, s4 = 0;
+    long long P = 0;
+    std::vector<double> rows;   // [3n][N]
+    std::vector<double> psi;    // [N+1]
+    std::vector<double> mi;     // [npairs] after run
+    double psiN_phiK = 0.0;
+    int wave = -1;
+    double ms_done = 0.0;       // batch-relative completion time of its wave
+    long long row_bytes() const { return (long long)natoms * N * P * 4; }
+    long long npairs() const { return (long long)natoms * (natoms - 1) / 2; }
+};
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct gmxcorr_ksg_batch {
+    std::vector<HostTraj> traj;
+    gmxcorr_batch_stats st{};
+};
+
+extern "C" {
+
+int gmxcorr_ksg_batch_create(gmxcorr_ksg_batch** out)
+{
+    if (!out) return gmxcorr::set_error(GMXCORR_EINVAL, "batch_create: null output");
+    *out = new gmxcorr_ksg_batch();
+    return GMXCORR_OK;
+}
+
+void gmxcorr_ksg_batch_destroy(gmxcorr_ksg_batch* b) { delete b; }
+
+int gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const double* psi,
+                          int natoms, int nframes, int k, int kind, int s4)
+{
+    if (!b || !xrows || !psi) return gmxcorr::set_error(GMXCORR_EINVAL, "batch_add: null argument");
+    if (natoms < 2 || k < 1 || nframes <= k)
+        return gmxcorr::set_error(GMXCORR_EINVAL, "batch_add: need natoms >= 2 and 1 <= k < nframes");
+    if (nframes > gmxcorr::kMaxFrames)
+        return gmxcorr::set_error(GMXCORR_EUNSUP, "batch_add: nframes above the kernel limit");
+    if (kind < 0) {
+        // smallest compiled batch path that covers the frame count
+        static const int single[] = {2, 4, 6, 8};
+        static const int coop[] = {3, 4, 6, 8};
+        kind = -1;
+        for (int v : single)
+            if (nframes <= 256 * v) { kind = 0; s4 = v; break; }
+        if (kind < 0)
+            for (int v : coop)
+                if (nframes <= 1024 * v) { kind = 1; s4 = v; break; }
+    }
+    if (!gmxcorr::batch_variant_ok(kind, s4, nframes))
+        return gmxcorr::set_error(GMXCORR_EUNSUP, "batch_add: variant not compiled into the batch kernel");
+    const size_t nx = (size_t)3 * natoms * nframes;
+    for (size_t t = 0; t < nx; ++t)
+        if (!std::isfinite(xrows[t]))
+            return gmxcorr::set_error(GMXCORR_EINVAL, "batch_add: non-finite normalised coordinate");
+    HostTraj h;
+    h.natoms = natoms;
+    h.N = nframes;
+    h.K = k;
+    h.kind = kind;
+    h.s4 = s4;
+    h.P = ((long long)nframes + 63) / 64 * 64;
+    h.rows.assign(xrows, xrows + nx);
+    h.psi.assign(psi, psi + nframes + 1);
+    h.psiN_phiK = psi[nframes] + (psi[k] - 1 / (1.0 * k));
+    b->traj.push_back(std::move(h));
+    return (int)b->traj.size() - 1;
+}
+
+// Run every pair of every trajectory.  Trajectories are packed greedily (in
+// order of decreasing row bytes) into memory waves of at most
+// max_row_bytes of distance rows; each wave is one row build per trajectory
+// and ONE persistent pair launch of nblocks blocks over all its pairs,
+// largest frame counts first.
+int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nblocks)
+{
+    if (!b || b->traj.empty()) return gmxcorr::set_error(GMXCORR_EINVAL, "batch_run: empty batch");
+    if (!gmxcorr::device_ok()) return gmxcorr::set_error(GMXCORR_ENODEV, "no gfx950 device");
+    if (nblocks < 1) nblocks = 2048;
+    const int nt = (int)b->traj.size();
+    std::vector<int> order(nt);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(),
+              [&](int x, int y) { return b->traj[x].row_bytes() > b->traj[y].row_bytes(); });
+    std::vector<std::vector<int>> waves;
+    {
+        std::vector<long long> used;
+        for (int t : order) {
+            const long long need = b->traj[t].row_bytes();
+            if (need > max_row_bytes)
+                return gmxcorr::set_error(GMXCORR_ENOMEM, "batch_run: one trajectory exceeds the row budget");
+            bool placed = false;
+            for (size_t w = 0; w < waves.size() && !placed; ++w)
+                if (used[w] + need <= max_row_bytes) {
+                    waves[w].push_back(t);
+                    used[w] += need;
+                    placed = true;
+                }
+            if (!placed) {
+                waves.push_back({t});
+                used.push_back(need);
+            }
+        }
+    }
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t ev[5];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(ev[0], s));
+    double ms_total = 0.0, ms_pairs_total = 0.0, ms_build_total = 0.0, ms_fix_total = 0.0;
+    long long fix_rows = 0, ties = 0, items_total = 0;
+    int rc = GMXCORR_OK;
+    for (size_t wv = 0; wv < waves.size() && rc == GMXCORR_OK; ++wv) {
+        const std::vector<int>& ts = waves[wv];
+        std::vector<double*> dX(ts.size(), nullptr), dPsi(ts.size(), nullptr);
+        std::vector<float*> dD(ts.size(), nullptr);
+        gmxcorr::TrajDesc* dDesc = nullptr;
+        int *dItemTraj = nullptr, *dA1 = nullptr, *dA2 = nullptr;
+        double *dPartial = nullptr, *dFixsum = nullptr, *dMI = nullptr;
+        gmxcorr::FixEntry* dFix = nullptr;
+        unsigned* dCnt = nullptr;
+        unsigned long long* dQueue = nullptr;
+        auto cleanup = [&]() {
+            (void)hipStreamSynchronize(s);
+            for (auto& p : dX) dfree(p);
+            for (auto& p : dPsi) dfree(p);
+            for (auto& p : dD) dfree(p);
+            dfree(dDesc); dfree(dItemTraj); dfree(dA1); dfree(dA2); dfree(dPartial);
+            dfree(dFixsum); dfree(dMI); dfree(dFix); dfree(dCnt); dfree(dQueue);
+        };
+        auto body = [&]() -> int {
+            std::vector<gmxcorr::TrajDesc> desc(ts.size());
+            HIPCHK(hipEventRecord(ev[1], s));
+            for (size_t q = 0; q < ts.size(); ++q) {
+                HostTraj& h = b->traj[ts[q]];
+                HIPCHK(hipMalloc(&dX[q], h.rows.size() * sizeof(double)));
+                HIPCHK(hipMalloc(&dPsi[q], h.psi.size() * sizeof(double)));
+                HIPCHK(hipMalloc(&dD[q], (size_t)h.row_bytes()));
+                HIPCHK(hipMemcpyAsync(dX[q], h.rows.data(), h.rows.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, s));
+                HIPCHK(hipMemcpyAsync(dPsi[q], h.psi.data(), h.psi.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, s));
+                HIPCHK(gmxcorr::launch_build_rows(dX[q], h.natoms, h.N, h.P, dD[q], s));
+                desc[q] = gmxcorr::TrajDesc{dX[q], dD[q], dPsi[q], h.P, h.N, h.K, h.psiN_phiK,
+                                            h.kind, h.s4};
+            }
+            // items: trajectories by decreasing frame count, pairs a1 > a2 within each
+            std::vector<int> tq(ts.size());
+            std::iota(tq.begin(), tq.end(), 0);
+            std::sort(tq.begin(), tq.end(),
+                      [&](int x, int y) { return b->traj[ts[x]].N > b->traj[ts[y]].N; });
+            std::vector<int> it, i1, i2;
+            std::vector<long long> first(ts.size());
+            for (int q : tq) {
+                const HostTraj& h = b->traj[ts[q]];
+                first[q] = (long long)it.size();
+                for (int a1 = 1; a1 < h.natoms; ++a1)
+                    for (int a2 = 0; a2 < a1; ++a2) {
+                        it.push_back(q);
+                        i1.push_back(a1);
+                        i2.push_back(a2);
+                    }
+            }
+            const long long nitems = (long long)it.size();
+            items_total += nitems;
+            HIPCHK(hipMalloc(&dDesc, desc.size() * sizeof(gmxcorr::TrajDesc)));
+            HIPCHK(hipMalloc(&dItemTraj, nitems * sizeof(int)));
+            HIPCHK(hipMalloc(&dA1, nitems * sizeof(int)));
+            HIPCHK(hipMalloc(&dA2, nitems * sizeof(int)));
+            HIPCHK(hipMalloc(&dPartial, nitems * sizeof(double)));
+            HIPCHK(hipMalloc(&dFixsum, nitems * sizeof(double)));
+            HIPCHK(hipMalloc(&dMI, nitems * sizeof(double)));
+            HIPCHK(hipMalloc(&dCnt, 4 * sizeof(unsigned)));
+            HIPCHK(hipMalloc(&dQueue, sizeof(unsigned long long)));
+            unsigned fixcap = 1u << 18;
+            HIPCHK(hipMalloc(&dFix, (size_t)fixcap * sizeof(gmxcorr::FixEntry)));
+            HIPCHK(hipMemcpyAsync(dDesc, desc.data(), desc.size() * sizeof(gmxcorr::TrajDesc),
+                                  hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(dItemTraj, it.data(), nitems * sizeof(int), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(dA1, i1.data(), nitems * sizeof(int), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(dA2, i2.data(), nitems * sizeof(int), hipMemcpyHostToDevice, s));
+            gmxcorr::KsgBatchArgs ba{dDesc, dItemTraj, dA1, dA2, nitems, dPartial, dFix, dCnt, fixcap,
+                                     dQueue};
+            gmxcorr::FixupArgs fa{};
+            fa.traj = dDesc;
+            fa.item_traj = dItemTraj;
+            fa.a1 = dA1;
+            fa.a2 = dA2;
+            fa.fixsum = dFixsum;
+            fa.counters = dCnt;
+            unsigned counters[2] = {0, 0};
+            for (int attempt = 0; attempt < 2; ++attempt) {
+                ba.fixq = dFix;
+                ba.fixcap = fixcap;
+                fa.fixq = dFix;
+                HIPCHK(gmxcorr::launch_ksg_clear(dFixsum, nitems, dCnt, dQueue, s));
+                HIPCHK(hipEventRecord(ev[2], s));
+                HIPCHK(gmxcorr::launch_ksg_batch(ba, nblocks, s));
+                HIPCHK(hipEventRecord(ev[3], s));
+                HIPCHK(hipMemcpyAsync(counters, dCnt, sizeof(counters), hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                if (counters[0] <= fixcap) break;
+                dfree(dFix);
+                fixcap = counters[0] + counters[0] / 4 + 1024;
+                HIPCHK(hipMalloc(&dFix, (size_t)fixcap * sizeof(gmxcorr::FixEntry)));
+                if (attempt == 1) return gmxcorr::set_error(GMXCORR_EHIP, "recount queue overflow");
+            }
+            HIPCHK(gmxcorr::launch_ksg_fixup(fa, counters[0], s));
+            HIPCHK(gmxcorr::launch_ksg_finalize(fa, nitems, dPartial, dMI, s));
+            HIPCHK(hipEventRecord(ev[4], s));
+            HIPCHK(hipMemcpyAsync(counters, dCnt, sizeof(counters), hipMemcpyDeviceToHost, s));
+            std::vector<double> mi(nitems);
+            HIPCHK(hipMemcpyAsync(mi.data(), dMI, nitems * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            float tb = 0.f, tp = 0.f, tf = 0.f, tt = 0.f;
+            (void)hipEventElapsedTime(&tb, ev[1], ev[2]);
+            (void)hipEventElapsedTime(&tp, ev[2], ev[3]);
+            (void)hipEventElapsedTime(&tf, ev[3], ev[4]);
+            (void)hipEventElapsedTime(&tt, ev[0], ev[4]);
+            ms_build_total += tb;
+            ms_pairs_total += tp;
+            ms_fix_total += tf;
+            ms_total = tt;
+            fix_rows += counters[0];
+            ties += counters[1];
+            for (size_t q = 0; q < ts.size(); ++q) {
+                HostTraj& h = b->traj[ts[q]];
+                h.mi.assign(mi.begin() + first[q], mi.begin() + first[q] + h.npairs());
+                h.wave = (int)wv;
+                h.ms_done = tt;
+            }
+            return GMXCORR_OK;
+        };
+        rc = body();
+        cleanup();
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(s);
+    if (rc != GMXCORR_OK) return rc;
+    b->st.nwaves = (int)waves.size();
+    b->st.ms_total = ms_total;
+    b->st.ms_build = ms_build_total;
+    b->st.ms_pairs = ms_pairs_total;
+    b->st.ms_fixup = ms_fix_total;
+    b->st.npairs = items_total;
+    b->st.fixup_rows = fix_rows;
+    b->st.ties = ties;
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_batch_result(gmxcorr_ksg_batch* b, int t, double* mi, long long npairs,
+                             double* ms_done, int* wave)
+{
+    if (!b || t < 0 || t >= (int)b->traj.size())
+        return gmxcorr::set_error(GMXCORR_EINVAL, "batch_result: bad trajectory index");
+    const HostTraj& h = b->traj[t];
+    if (h.mi.empty() || npairs != h.npairs())
+        return gmxcorr::set_error(GMXCORR_EINVAL, "batch_result: not run or wrong pair count");
+    if (mi) std::memcpy(mi, h.mi.data(), npairs * sizeof(double));
+    if (ms_done) *ms_done = h.ms_done;
+    if (wave) *wave = h.wave;
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_batch_stats(const gmxcorr_ksg_batch* b, gmxcorr_batch_stats* st)
+{
+    if (!b || !st) return gmxcorr::set_error(GMXCORR_EINVAL, "batch_stats: null argument");
+    *st = b->st;
+    return GMXCORR_OK;
+}
+
+}  // extern "C"

@@ -153,6 +153,44 @@ __device__ __forceinline__ void find_bin(const uint32_t* hist, int lane, int t, 
     pop = readlane(v, sub);
 }
 
+// Destination of a row's result: the psi table, the recount queue and the
+// optional per-row count output.
+struct RowOut {
+    const double* psi;
+    unsigned* counters;
+    FixEntry* fixq;
+    unsigned fixcap;
+    int* rc_nx;
+    int* rc_ny;
+    unsigned char* rc_flag;
+};
+
+__device__ __forceinline__ void queue_row(const RowOut& o, long long item, int row, uint32_t eps)
+{
+    const unsigned slot = atomicAdd(&o.counters[0], 1u);
+    if (slot < o.fixcap) {
+        FixEntry fe;
+        fe.pair = (int)item;
+        fe.row = row;
+        fe.eps_bits = eps;
+        fe.pad = 0;
+        o.fixq[slot] = fe;
+    }
+}
+
+__device__ __forceinline__ RowOut row_out(const KsgArgs& a)
+{
+    RowOut o;
+    o.psi = a.psi;
+    o.counters = a.counters;
+    o.fixq = a.fixq;
+    o.fixcap = a.fixcap;
+    o.rc_nx = a.rc_nx;
+    o.rc_ny = a.rc_ny;
+    o.rc_flag = a.rc_flag;
+    return o;
+}
+
 template <int NB>
 __device__ __forceinline__ void clear_hist(uint32_t* hist, int lane)
 {
@@ -199,11 +237,316 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
 }
 
 // ---------------------------------------------------------------------------
-// Pair kernel.  Block = 4 waves = one pair; wave w takes frames w, w+4, ...
-// S4 = 16-byte chunks per lane per row (N <= 256*S4).  Blocks are spread so
-// that each XCD (block id mod 8) receives a contiguous range of pairs: pairs
-// next to each other share their first atom, whose rows then stay in that
+// One frame row of one pair, processed by one wave (N <= 256*S4).  Returns
+// psi[nx] + psi[ny] on lane 0 (0 elsewhere, and for rows queued to the FP64
+// recount).  hist: the wave's 2048-word LDS array.
+// ---------------------------------------------------------------------------
+template <int S4>
+__device__ __forceinline__ double row_single(const float* Da, const float* Db, long long P, int N,
+                                             int T, int i, long long item, uint32_t* hist,
+                                             int lane, const RowOut& o)
+{
+    const int nchunk = (N + 3) / 4;
+    const float4* ra = reinterpret_cast<const float4*>(Da + (size_t)i * P);
+    const float4* rb = reinterpret_cast<const float4*>(Db + (size_t)i * P);
+    float a[4 * S4], b[4 * S4];
+#pragma unroll
+    for (int m = 0; m < S4; ++m) {
+        const int c = m * kWave + lane;
+        float4 va = make_float4(INFINITY, INFINITY, INFINITY, INFINITY), vb = va;
+        if (c < nchunk) {
+            va = ra[c];
+            vb = rb[c];
+        }
+        a[4 * m + 0] = va.x; a[4 * m + 1] = va.y; a[4 * m + 2] = va.z; a[4 * m + 3] = va.w;
+        b[4 * m + 0] = vb.x; b[4 * m + 1] = vb.y; b[4 * m + 2] = vb.z; b[4 * m + 3] = vb.w;
+    }
+
+    // ---- sweep A: fine histogram of the joint keys --------------------
+    clear_hist<kBins>(hist, lane);
+    wave_sync();
+#pragma unroll
+    for (int e = 0; e < 4 * S4; ++e) {
+        const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
+        const int bin = min(max((int)(key >> kBinShift) - kBinBase, 0), kBins - 1);
+        atomicAdd(&hist[bin], 1u);
+    }
+    wave_sync();
+    int b1, r0, pop;
+    find_bin<kBins / kWave>(hist, lane, T, b1, r0, pop);
+    wave_sync();
+    bool unknown = b1 == 0 || b1 == kBins - 1 || pop > kWave;
+    const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
+
+    // ---- sweep B: sure members' marginal maxima, bin candidates -------
+    float ex = 0.0f, ey = 0.0f;
+    int nc = 0;
+#pragma unroll
+    for (int e = 0; e < 4 * S4; ++e) {
+        const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
+        const bool sure = key < lo;
+        ex = fmaxf(ex, sure ? a[e] : 0.0f);
+        ey = fmaxf(ey, sure ? b[e] : 0.0f);
+        if (key - lo < (1u << kBinShift)) {
+            if (nc < kCandCap) {
+                hist[(nc * kWave + lane) * 2 + 0] = __float_as_uint(a[e]);
+                hist[(nc * kWave + lane) * 2 + 1] = __float_as_uint(b[e]);
+            }
+            ++nc;
+        }
+    }
+    unknown = unknown || wave_max_i(nc) > kCandCap;
+    uint32_t eps_key = kUnknownEps;
+    bool sel_tie = false;
+    if (!unknown) {
+        // compact the candidates to one per lane (entries 512.. of the LDS row)
+        const int off = wave_incl_scan(nc) - nc;
+#pragma unroll
+        for (int t = 0; t < kCandCap; ++t)
+            if (t < nc) {
+                const uint32_t ca = hist[(t * kWave + lane) * 2 + 0];
+                const uint32_t cb = hist[(t * kWave + lane) * 2 + 1];
+                hist[512 + (off + t) * 2 + 0] = ca;
+                hist[512 + (off + t) * 2 + 1] = cb;
+            }
+        wave_sync();
+        const bool act = lane < pop;
+        float ca = 0.0f, cb = 0.0f;
+        uint32_t low = 0xffffffffu;
+        if (act) {
+            ca = __uint_as_float(hist[512 + lane * 2 + 0]);
+            cb = __uint_as_float(hist[512 + lane * 2 + 1]);
+            low = __float_as_uint(fmaxf(ca, cb)) - lo;   // < 2^17
+        }
+        // r0-th smallest of the candidates' low 17 bits, one bit at a time
+        uint32_t sel = 0;
+        int r = r0;
+        for (int bit = kBinShift - 1; bit >= 0; --bit) {
+            const bool zero = act && (low >> (bit + 1)) == sel && ((low >> bit) & 1u) == 0u;
+            const int c0 = __popcll(__ballot(zero));
+            sel <<= 1;
+            if (r > c0) {
+                r -= c0;
+                sel |= 1u;
+            }
+        }
+        eps_key = lo + sel;
+        const bool in = act && low <= sel;
+        sel_tie = __popcll(__ballot(in)) != r0;
+        ex = fmaxf(ex, in ? ca : 0.0f);
+        ey = fmaxf(ey, in ? cb : 0.0f);
+    }
+    ex = wave_max(ex);
+    ey = wave_max(ey);
+    wave_sync();
+
+    // ---- sweep C: marginal counts (self included, removed below) ------
+    int cx = 0, cy = 0;
+#pragma unroll
+    for (int e = 0; e < 4 * S4; ++e) {
+        cx += (a[e] <= ex) + ((a[e] == ex) << 16);
+        cy += (b[e] <= ey) + ((b[e] == ey) << 16);
+    }
+    cx = wave_sum(cx);
+    cy = wave_sum(cy);
+    const int nx = (cx & 0xffff) - 1, ny = (cy & 0xffff) - 1;
+    const bool flag = unknown || sel_tie || (cx >> 16) > 1 || (cy >> 16) > 1;
+    double contrib = 0.0;
+    if (lane == 0) {
+        if (flag)
+            queue_row(o, item, i, eps_key);
+        else
+            contrib = o.psi[nx] + o.psi[ny];
+        if (o.rc_nx) {
+            const size_t off = (size_t)item * N + i;
+            o.rc_nx[off] = flag ? -1 : nx;
+            o.rc_ny[off] = flag ? -1 : ny;
+            o.rc_flag[off] = 0;
+        }
+    }
+    return contrib;
+}
+
+// ---------------------------------------------------------------------------
+// One frame row processed by the 4 waves of a block together, wave w holding
+// frames [w*256*S4, (w+1)*256*S4) (N <= 1024*S4).  Same three sweeps with a
+// block-shared histogram; wave 0 resolves the boundary candidates.  Five
+// barriers per row.  Returns the row's psi sum on thread 0.
+// ---------------------------------------------------------------------------
+struct CoopLds {
+    uint32_t hist[kBins];                 // histogram, then candidate slots
+    uint32_t dense[kPairWaves][kWave][2]; // per-wave compacted candidates
+    int nc[kPairWaves], over[kPairWaves], cx[kPairWaves], cy[kPairWaves];
+    float ex[kPairWaves], ey[kPairWaves];
+    float rex, rey;
+    int rflag;
+    uint32_t reps;
+};
+
+template <int S4>
+__device__ __forceinline__ double row_coop(const float* Da, const float* Db, long long P, int N,
+                                           int T, int i, long long item, CoopLds& sh, int lane,
+                                           int w, const RowOut& o)
+{
+    const int nchunk = (N + 3) / 4;
+    uint32_t* hist = sh.hist;
+    uint32_t* slots = hist + w * 512;
+    const float4* ra = reinterpret_cast<const float4*>(Da + (size_t)i * P);
+    const float4* rb = reinterpret_cast<const float4*>(Db + (size_t)i * P);
+    float a[4 * S4], b[4 * S4];
+#pragma unroll
+    for (int m = 0; m < S4; ++m) {
+        const int c = (w * S4 + m) * kWave + lane;
+        float4 va = make_float4(INFINITY, INFINITY, INFINITY, INFINITY), vb = va;
+        if (c < nchunk) {
+            va = ra[c];
+            vb = rb[c];
+        }
+        a[4 * m + 0] = va.x; a[4 * m + 1] = va.y; a[4 * m + 2] = va.z; a[4 * m + 3] = va.w;
+        b[4 * m + 0] = vb.x; b[4 * m + 1] = vb.y; b[4 * m + 2] = vb.z; b[4 * m + 3] = vb.w;
+    }
+    reinterpret_cast<uint4*>(hist)[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+    reinterpret_cast<uint4*>(hist)[threadIdx.x + 256] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4 * S4; ++e) {
+        const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
+        const int bin = min(max((int)(key >> kBinShift) - kBinBase, 0), kBins - 1);
+        atomicAdd(&hist[bin], 1u);
+    }
+    __syncthreads();
+    int b1, r0, pop;
+    find_bin<kBins / kWave>(hist, lane, T, b1, r0, pop);
+    const bool edge = b1 == 0 || b1 == kBins - 1 || pop > kWave;
+    const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
+    __syncthreads();
+
+    float ex = 0.0f, ey = 0.0f;
+    int nc = 0;
+#pragma unroll
+    for (int e = 0; e < 4 * S4; ++e) {
+        const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
+        const bool sure = key < lo;
+        ex = fmaxf(ex, sure ? a[e] : 0.0f);
+        ey = fmaxf(ey, sure ? b[e] : 0.0f);
+        if (key - lo < (1u << kBinShift)) {
+            if (nc < kCandCap) {
+                slots[(nc * kWave + lane) * 2 + 0] = __float_as_uint(a[e]);
+                slots[(nc * kWave + lane) * 2 + 1] = __float_as_uint(b[e]);
+            }
+            ++nc;
+        }
+    }
+    const int ncmax = wave_max_i(nc);
+    const int ncw = wave_sum(nc);
+    if (ncmax <= kCandCap && !edge) {
+        const int off = wave_incl_scan(nc) - nc;
+#pragma unroll
+        for (int t = 0; t < kCandCap; ++t)
+            if (t < nc && off + t < kWave) {
+                sh.dense[w][off + t][0] = slots[(t * kWave + lane) * 2 + 0];
+                sh.dense[w][off + t][1] = slots[(t * kWave + lane) * 2 + 1];
+            }
+    }
+    ex = wave_max(ex);
+    ey = wave_max(ey);
+    if (lane == 0) {
+        sh.nc[w] = ncw;
+        sh.over[w] = ncmax > kCandCap;
+        sh.ex[w] = ex;
+        sh.ey[w] = ey;
+    }
+    __syncthreads();
+
+    if (w == 0) {
+        const bool unknown = edge || sh.over[0] || sh.over[1] || sh.over[2] || sh.over[3];
+        float rex = fmaxf(fmaxf(sh.ex[0], sh.ex[1]), fmaxf(sh.ex[2], sh.ex[3]));
+        float rey = fmaxf(fmaxf(sh.ey[0], sh.ey[1]), fmaxf(sh.ey[2], sh.ey[3]));
+        uint32_t eps_key = kUnknownEps;
+        bool sel_tie = false;
+        if (!unknown) {
+            const int c0 = sh.nc[0], c1 = c0 + sh.nc[1], c2 = c1 + sh.nc[2];
+            const bool act = lane < pop;
+            float ca = 0.0f, cb = 0.0f;
+            uint32_t low = 0xffffffffu;
+            if (act) {
+                const int q = lane < c0 ? 0 : lane < c1 ? 1 : lane < c2 ? 2 : 3;
+                const int l = lane - (q == 0 ? 0 : q == 1 ? c0 : q == 2 ? c1 : c2);
+                ca = __uint_as_float(sh.dense[q][l][0]);
+                cb = __uint_as_float(sh.dense[q][l][1]);
+                low = __float_as_uint(fmaxf(ca, cb)) - lo;
+            }
+            uint32_t sel = 0;
+            int r = r0;
+            for (int bit = kBinShift - 1; bit >= 0; --bit) {
+                const bool zero = act && (low >> (bit + 1)) == sel && ((low >> bit) & 1u) == 0u;
+                const int cz = __popcll(__ballot(zero));
+                sel <<= 1;
+                if (r > cz) {
+                    r -= cz;
+                    sel |= 1u;
+                }
+            }
+            eps_key = lo + sel;
+            const bool in = act && low <= sel;
+            sel_tie = __popcll(__ballot(in)) != r0;
+            rex = fmaxf(rex, wave_max(in ? ca : 0.0f));
+            rey = fmaxf(rey, wave_max(in ? cb : 0.0f));
+        }
+        if (lane == 0) {
+            sh.rex = rex;
+            sh.rey = rey;
+            sh.rflag = (unknown || sel_tie) ? 1 : 0;
+            sh.reps = eps_key;
+        }
+    }
+    __syncthreads();
+    ex = sh.rex;
+    ey = sh.rey;
+    int cx = 0, cy = 0;
+#pragma unroll
+    for (int e = 0; e < 4 * S4; ++e) {
+        cx += (a[e] <= ex) + ((a[e] == ex) << 16);
+        cy += (b[e] <= ey) + ((b[e] == ey) << 16);
+    }
+    cx = wave_sum(cx);
+    cy = wave_sum(cy);
+    if (lane == 0) {
+        sh.cx[w] = cx;
+        sh.cy[w] = cy;
+    }
+    __syncthreads();
+    double contrib = 0.0;
+    if (threadIdx.x == 0) {
+        const int tx = sh.cx[0] + sh.cx[1] + sh.cx[2] + sh.cx[3];
+        const int ty = sh.cy[0] + sh.cy[1] + sh.cy[2] + sh.cy[3];
+        const int nx = (tx & 0xffff) - 1, ny = (ty & 0xffff) - 1;
+        const bool flag = sh.rflag || (tx >> 16) > 1 || (ty >> 16) > 1;
+        if (flag)
+            queue_row(o, item, i, sh.reps);
+        else
+            contrib = o.psi[nx] + o.psi[ny];
+        if (o.rc_nx) {
+            const size_t off = (size_t)item * N + i;
+            o.rc_nx[off] = flag ? -1 : nx;
+            o.rc_ny[off] = flag ? -1 : ny;
+            o.rc_flag[off] = 0;
+        }
+    }
+    return contrib;
+}
+
+// Pair index of a block: each XCD (block id mod 8) receives a contiguous
+// range of pairs, so pairs sharing their first atom keep its rows in that
 // XCD's L2.
+__device__ __forceinline__ long long xcd_pair(long long npairs)
+{
+    const long long per_xcd = (npairs + 7) / 8;
+    return (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+}
+
+// ---------------------------------------------------------------------------
+// Single-trajectory pair kernels: block = one pair.
 // ---------------------------------------------------------------------------
 template <int S4>
 __global__ __launch_bounds__(256) void ksg_pair_kernel(KsgArgs args)
@@ -212,332 +555,115 @@ __global__ __launch_bounds__(256) void ksg_pair_kernel(KsgArgs args)
     __shared__ double wsum[kPairWaves];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    uint32_t* hist = lds[w];
-    const long long per_xcd = (args.npairs + 7) / 8;
-    const long long p = (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const long long p = xcd_pair(args.npairs);
     if (p >= args.npairs) return;
     const int N = args.N, T = args.K + 1;
-    const long long P = args.P;
-    const float* Da = args.D + (size_t)args.a1[p] * N * P;
-    const float* Db = args.D + (size_t)args.a2[p] * N * P;
-    const int nchunk = (N + 3) / 4;
+    const float* Da = args.D + (size_t)args.a1[p] * N * args.P;
+    const float* Db = args.D + (size_t)args.a2[p] * N * args.P;
+    const RowOut o = row_out(args);
     double acc = 0.0;
-
-    for (int i = w; i < N; i += kPairWaves) {
-        const float4* ra = reinterpret_cast<const float4*>(Da + (size_t)i * P);
-        const float4* rb = reinterpret_cast<const float4*>(Db + (size_t)i * P);
-        float a[4 * S4], b[4 * S4];
-#pragma unroll
-        for (int m = 0; m < S4; ++m) {
-            const int c = m * kWave + lane;
-            float4 va = make_float4(INFINITY, INFINITY, INFINITY, INFINITY), vb = va;
-            if (c < nchunk) {
-                va = ra[c];
-                vb = rb[c];
-            }
-            a[4 * m + 0] = va.x; a[4 * m + 1] = va.y; a[4 * m + 2] = va.z; a[4 * m + 3] = va.w;
-            b[4 * m + 0] = vb.x; b[4 * m + 1] = vb.y; b[4 * m + 2] = vb.z; b[4 * m + 3] = vb.w;
-        }
-
-        // ---- sweep A: fine histogram of the joint keys --------------------
-        clear_hist<kBins>(hist, lane);
-        wave_sync();
-#pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
-            const int bin = min(max((int)(key >> kBinShift) - kBinBase, 0), kBins - 1);
-            atomicAdd(&hist[bin], 1u);
-        }
-        wave_sync();
-        int b1, r0, pop;
-        find_bin<kBins / kWave>(hist, lane, T, b1, r0, pop);
-        wave_sync();
-        bool unknown = b1 == 0 || b1 == kBins - 1 || pop > kWave;
-        const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
-
-        // ---- sweep B: sure members' marginal maxima, bin candidates -------
-        float ex = 0.0f, ey = 0.0f;
-        int nc = 0;
-#pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
-            const bool sure = key < lo;
-            ex = fmaxf(ex, sure ? a[e] : 0.0f);
-            ey = fmaxf(ey, sure ? b[e] : 0.0f);
-            if (key - lo < (1u << kBinShift)) {
-                if (nc < kCandCap) {
-                    hist[(nc * kWave + lane) * 2 + 0] = __float_as_uint(a[e]);
-                    hist[(nc * kWave + lane) * 2 + 1] = __float_as_uint(b[e]);
-                }
-                ++nc;
-            }
-        }
-        unknown = unknown || wave_max_i(nc) > kCandCap;
-        uint32_t eps_key = kUnknownEps;
-        bool sel_tie = false;
-        if (!unknown) {
-            // compact the candidates to one per lane (entries 512.. of the LDS row)
-            const int off = wave_incl_scan(nc) - nc;
-#pragma unroll
-            for (int t = 0; t < kCandCap; ++t)
-                if (t < nc) {
-                    const uint32_t ca = hist[(t * kWave + lane) * 2 + 0];
-                    const uint32_t cb = hist[(t * kWave + lane) * 2 + 1];
-                    hist[512 + (off + t) * 2 + 0] = ca;
-                    hist[512 + (off + t) * 2 + 1] = cb;
-                }
-            wave_sync();
-            const bool act = lane < pop;
-            float ca = 0.0f, cb = 0.0f;
-            uint32_t low = 0xffffffffu;
-            if (act) {
-                ca = __uint_as_float(hist[512 + lane * 2 + 0]);
-                cb = __uint_as_float(hist[512 + lane * 2 + 1]);
-                low = __float_as_uint(fmaxf(ca, cb)) - lo;   // < 2^17
-            }
-            // r0-th smallest of the candidates' low 17 bits, one bit at a time
-            uint32_t sel = 0;
-            int r = r0;
-            for (int bit = kBinShift - 1; bit >= 0; --bit) {
-                const bool zero = act && (low >> (bit + 1)) == sel && ((low >> bit) & 1u) == 0u;
-                const int c0 = __popcll(__ballot(zero));
-                sel <<= 1;
-                if (r > c0) {
-                    r -= c0;
-                    sel |= 1u;
-                }
-            }
-            eps_key = lo + sel;
-            const bool in = act && low <= sel;
-            sel_tie = __popcll(__ballot(in)) != r0;
-            ex = fmaxf(ex, in ? ca : 0.0f);
-            ey = fmaxf(ey, in ? cb : 0.0f);
-        }
-        ex = wave_max(ex);
-        ey = wave_max(ey);
-        wave_sync();
-
-        // ---- sweep C: marginal counts (self included, removed below) ------
-        int cx = 0, cy = 0;
-#pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            cx += (a[e] <= ex) + ((a[e] == ex) << 16);
-            cy += (b[e] <= ey) + ((b[e] == ey) << 16);
-        }
-        cx = wave_sum(cx);
-        cy = wave_sum(cy);
-        const int nx = (cx & 0xffff) - 1, ny = (cy & 0xffff) - 1;
-        const bool flag = unknown || sel_tie || (cx >> 16) > 1 || (cy >> 16) > 1;
-
-        if (lane == 0) {
-            if (flag) {
-                const unsigned slot = atomicAdd(&args.counters[0], 1u);
-                if (slot < args.fixcap) {
-                    FixEntry fe;
-                    fe.pair = (int)p;
-                    fe.row = i;
-                    fe.eps_bits = eps_key;
-                    fe.pad = 0;
-                    args.fixq[slot] = fe;
-                }
-            } else {
-                acc += args.psi[nx] + args.psi[ny];
-            }
-            if (args.rc_nx) {
-                const size_t o = (size_t)p * N + i;
-                args.rc_nx[o] = flag ? -1 : nx;
-                args.rc_ny[o] = flag ? -1 : ny;
-                args.rc_flag[o] = 0;
-            }
-        }
-    }
+    for (int i = w; i < N; i += kPairWaves)
+        acc += row_single<S4>(Da, Db, args.P, N, T, i, p, lds[w], lane, o);
     if (lane == 0) wsum[w] = acc;
     __syncthreads();
     if (threadIdx.x == 0) args.partial[p] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
 }
 
-// ---------------------------------------------------------------------------
-// Cooperative pair kernel for long trajectories: the 4 waves of a block share
-// one frame row, wave w holding frames [w*256*S4, (w+1)*256*S4) in registers
-// (N <= 1024*S4).  Same three sweeps as ksg_pair_kernel with a block-shared
-// histogram; wave 0 resolves the boundary candidates.  Five barriers per row.
-// ---------------------------------------------------------------------------
 template <int S4>
 __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
 {
-    __shared__ uint32_t hist[kBins];                 // histogram, then candidate slots
-    __shared__ uint32_t dense[kPairWaves][kWave][2]; // per-wave compacted candidates
-    __shared__ int sh_nc[kPairWaves], sh_over[kPairWaves], sh_cx[kPairWaves], sh_cy[kPairWaves];
-    __shared__ float sh_ex[kPairWaves], sh_ey[kPairWaves];
-    __shared__ float res_ex, res_ey;
-    __shared__ int res_flag;
-    __shared__ uint32_t res_eps;
+    __shared__ CoopLds sh;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    const long long per_xcd = (args.npairs + 7) / 8;
-    const long long p = (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const long long p = xcd_pair(args.npairs);
     if (p >= args.npairs) return;
     const int N = args.N, T = args.K + 1;
-    const long long P = args.P;
-    const float* Da = args.D + (size_t)args.a1[p] * N * P;
-    const float* Db = args.D + (size_t)args.a2[p] * N * P;
-    const int nchunk = (N + 3) / 4;
-    uint32_t* slots = hist + w * 512;
+    const float* Da = args.D + (size_t)args.a1[p] * N * args.P;
+    const float* Db = args.D + (size_t)args.a2[p] * N * args.P;
+    const RowOut o = row_out(args);
     double acc = 0.0;
-
-    for (int i = 0; i < N; ++i) {
-        const float4* ra = reinterpret_cast<const float4*>(Da + (size_t)i * P);
-        const float4* rb = reinterpret_cast<const float4*>(Db + (size_t)i * P);
-        float a[4 * S4], b[4 * S4];
-#pragma unroll
-        for (int m = 0; m < S4; ++m) {
-            const int c = (w * S4 + m) * kWave + lane;
-            float4 va = make_float4(INFINITY, INFINITY, INFINITY, INFINITY), vb = va;
-            if (c < nchunk) {
-                va = ra[c];
-                vb = rb[c];
-            }
-            a[4 * m + 0] = va.x; a[4 * m + 1] = va.y; a[4 * m + 2] = va.z; a[4 * m + 3] = va.w;
-            b[4 * m + 0] = vb.x; b[4 * m + 1] = vb.y; b[4 * m + 2] = vb.z; b[4 * m + 3] = vb.w;
-        }
-        reinterpret_cast<uint4*>(hist)[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
-        reinterpret_cast<uint4*>(hist)[threadIdx.x + 256] = make_uint4(0u, 0u, 0u, 0u);
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
-            const int bin = min(max((int)(key >> kBinShift) - kBinBase, 0), kBins - 1);
-            atomicAdd(&hist[bin], 1u);
-        }
-        __syncthreads();
-        int b1, r0, pop;
-        find_bin<kBins / kWave>(hist, lane, T, b1, r0, pop);
-        const bool edge = b1 == 0 || b1 == kBins - 1 || pop > kWave;
-        const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
-        __syncthreads();
-
-        float ex = 0.0f, ey = 0.0f;
-        int nc = 0;
-#pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
-            const bool sure = key < lo;
-            ex = fmaxf(ex, sure ? a[e] : 0.0f);
-            ey = fmaxf(ey, sure ? b[e] : 0.0f);
-            if (key - lo < (1u << kBinShift)) {
-                if (nc < kCandCap) {
-                    slots[(nc * kWave + lane) * 2 + 0] = __float_as_uint(a[e]);
-                    slots[(nc * kWave + lane) * 2 + 1] = __float_as_uint(b[e]);
-                }
-                ++nc;
-            }
-        }
-        const int ncmax = wave_max_i(nc);
-        const int ncw = wave_sum(nc);
-        if (ncmax <= kCandCap && !edge) {
-            const int off = wave_incl_scan(nc) - nc;
-#pragma unroll
-            for (int t = 0; t < kCandCap; ++t)
-                if (t < nc && off + t < kWave) {
-                    dense[w][off + t][0] = slots[(t * kWave + lane) * 2 + 0];
-                    dense[w][off + t][1] = slots[(t * kWave + lane) * 2 + 1];
-                }
-        }
-        ex = wave_max(ex);
-        ey = wave_max(ey);
-        if (lane == 0) {
-            sh_nc[w] = ncw;
-            sh_over[w] = ncmax > kCandCap;
-            sh_ex[w] = ex;
-            sh_ey[w] = ey;
-        }
-        __syncthreads();
-
-        if (w == 0) {
-            bool unknown = edge || sh_over[0] || sh_over[1] || sh_over[2] || sh_over[3];
-            float rex = fmaxf(fmaxf(sh_ex[0], sh_ex[1]), fmaxf(sh_ex[2], sh_ex[3]));
-            float rey = fmaxf(fmaxf(sh_ey[0], sh_ey[1]), fmaxf(sh_ey[2], sh_ey[3]));
-            uint32_t eps_key = kUnknownEps;
-            bool sel_tie = false;
-            if (!unknown) {
-                const int c0 = sh_nc[0], c1 = c0 + sh_nc[1], c2 = c1 + sh_nc[2];
-                const bool act = lane < pop;
-                float ca = 0.0f, cb = 0.0f;
-                uint32_t low = 0xffffffffu;
-                if (act) {
-                    const int q = lane < c0 ? 0 : lane < c1 ? 1 : lane < c2 ? 2 : 3;
-                    const int l = lane - (q == 0 ? 0 : q == 1 ? c0 : q == 2 ? c1 : c2);
-                    ca = __uint_as_float(dense[q][l][0]);
-                    cb = __uint_as_float(dense[q][l][1]);
-                    low = __float_as_uint(fmaxf(ca, cb)) - lo;
-                }
-                uint32_t sel = 0;
-                int r = r0;
-                for (int bit = kBinShift - 1; bit >= 0; --bit) {
-                    const bool zero = act && (low >> (bit + 1)) == sel && ((low >> bit) & 1u) == 0u;
-                    const int cz = __popcll(__ballot(zero));
-                    sel <<= 1;
-                    if (r > cz) {
-                        r -= cz;
-                        sel |= 1u;
-                    }
-                }
-                eps_key = lo + sel;
-                const bool in = act && low <= sel;
-                sel_tie = __popcll(__ballot(in)) != r0;
-                rex = fmaxf(rex, wave_max(in ? ca : 0.0f));
-                rey = fmaxf(rey, wave_max(in ? cb : 0.0f));
-            }
-            if (lane == 0) {
-                res_ex = rex;
-                res_ey = rey;
-                res_flag = (unknown || sel_tie) ? 1 : 0;
-                res_eps = eps_key;
-            }
-        }
-        __syncthreads();
-        ex = res_ex;
-        ey = res_ey;
-        int cx = 0, cy = 0;
-#pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            cx += (a[e] <= ex) + ((a[e] == ex) << 16);
-            cy += (b[e] <= ey) + ((b[e] == ey) << 16);
-        }
-        cx = wave_sum(cx);
-        cy = wave_sum(cy);
-        if (lane == 0) {
-            sh_cx[w] = cx;
-            sh_cy[w] = cy;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int tx = sh_cx[0] + sh_cx[1] + sh_cx[2] + sh_cx[3];
-            const int ty = sh_cy[0] + sh_cy[1] + sh_cy[2] + sh_cy[3];
-            const int nx = (tx & 0xffff) - 1, ny = (ty & 0xffff) - 1;
-            const bool flag = res_flag || (tx >> 16) > 1 || (ty >> 16) > 1;
-            if (flag) {
-                const unsigned slot = atomicAdd(&args.counters[0], 1u);
-                if (slot < args.fixcap) {
-                    FixEntry fe;
-                    fe.pair = (int)p;
-                    fe.row = i;
-                    fe.eps_bits = res_eps;
-                    fe.pad = 0;
-                    args.fixq[slot] = fe;
-                }
-            } else {
-                acc += args.psi[nx] + args.psi[ny];
-            }
-            if (args.rc_nx) {
-                const size_t o = (size_t)p * N + i;
-                args.rc_nx[o] = flag ? -1 : nx;
-                args.rc_ny[o] = flag ? -1 : ny;
-                args.rc_flag[o] = 0;
-            }
-        }
-    }
+    for (int i = 0; i < N; ++i) acc += row_coop<S4>(Da, Db, args.P, N, T, i, p, sh, lane, w, o);
     if (threadIdx.x == 0) args.partial[p] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent batched kernel: many trajectories of uneven atom and frame
+// counts in one launch.  Blocks pull (trajectory, pair) items from a global
+// counter until the queue is empty; each item runs the code path (kind, s4)
+// that the tuner chose for its trajectory.
+// ---------------------------------------------------------------------------
+union BatchLds {
+    uint32_t single[kPairWaves][kBins];
+    CoopLds coop;
+};
+
+template <int S4>
+__device__ __forceinline__ double batch_single(const TrajDesc& d, const float* Da, const float* Db,
+                                               long long item, BatchLds& u, double* wsum,
+                                               int lane, int w, const RowOut& o)
+{
+    double acc = 0.0;
+    for (int i = w; i < d.N; i += kPairWaves)
+        acc += row_single<S4>(Da, Db, d.P, d.N, d.K + 1, i, item, u.single[w], lane, o);
+    if (lane == 0) wsum[w] = acc;
+    __syncthreads();
+    return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+
+template <int S4>
+__device__ __forceinline__ double batch_coop(const TrajDesc& d, const float* Da, const float* Db,
+                                             long long item, BatchLds& u, int lane, int w,
+                                             const RowOut& o)
+{
+    double acc = 0.0;
+    for (int i = 0; i < d.N; ++i)
+        acc += row_coop<S4>(Da, Db, d.P, d.N, d.K + 1, i, item, u.coop, lane, w, o);
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
+{
+    __shared__ BatchLds u;
+    __shared__ double wsum[kPairWaves];
+    __shared__ long long next;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    RowOut o;
+    o.counters = b.counters;
+    o.fixq = b.fixq;
+    o.fixcap = b.fixcap;
+    o.rc_nx = nullptr;
+    o.rc_ny = nullptr;
+    o.rc_flag = nullptr;
+    for (;;) {
+        if (threadIdx.x == 0) next = (long long)atomicAdd(b.queue, 1ULL);
+        __syncthreads();
+        const long long item = next;
+        __syncthreads();
+        if (item >= b.nitems) break;
+        const TrajDesc d = b.traj[b.item_traj[item]];
+        o.psi = d.psi;
+        const float* Da = d.D + (size_t)b.a1[item] * d.N * d.P;
+        const float* Db = d.D + (size_t)b.a2[item] * d.N * d.P;
+        double acc = 0.0;
+        if (d.kind == 0) {
+            switch (d.s4) {
+                case 2: acc = batch_single<2>(d, Da, Db, item, u, wsum, lane, w, o); break;
+                case 4: acc = batch_single<4>(d, Da, Db, item, u, wsum, lane, w, o); break;
+                case 6: acc = batch_single<6>(d, Da, Db, item, u, wsum, lane, w, o); break;
+                default: acc = batch_single<8>(d, Da, Db, item, u, wsum, lane, w, o); break;
+            }
+        } else {
+            switch (d.s4) {
+                case 3: acc = batch_coop<3>(d, Da, Db, item, u, lane, w, o); break;
+                case 4: acc = batch_coop<4>(d, Da, Db, item, u, lane, w, o); break;
+                case 6: acc = batch_coop<6>(d, Da, Db, item, u, lane, w, o); break;
+                default: acc = batch_coop<8>(d, Da, Db, item, u, lane, w, o); break;
+            }
+        }
+        if (threadIdx.x == 0) b.partial[item] = acc;
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -549,7 +675,7 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
 // r = k - #{j != i : key < eps}.  More than r frames at that FP64 value is
 // an exact tie: all of them enter the radius set (the tie is counted).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void ksg_fixup_kernel(KsgArgs args, unsigned nent)
+__global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned nent)
 {
     __shared__ uint32_t lds[4][kBins];
     const int lane = threadIdx.x & (kWave - 1);
@@ -557,9 +683,12 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(KsgArgs args, unsigned n
     const unsigned e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (e >= nent) return;
     const FixEntry fe = args.fixq[e];
-    const int p = fe.pair, i = fe.row, N = args.N, K = args.K;
-    const double* xa = args.X + (size_t)(3 * args.a1[p]) * N;
-    const double* xb = args.X + (size_t)(3 * args.a2[p]) * N;
+    const long long p = fe.pair;
+    const int i = fe.row;
+    const TrajDesc d = args.traj[args.item_traj ? args.item_traj[p] : 0];
+    const int N = d.N, K = d.K;
+    const double* xa = d.X + (size_t)(3 * args.a1[p]) * N;
+    const double* xb = d.X + (size_t)(3 * args.a2[p]) * N;
     const double qa0 = xa[i], qa1 = xa[N + i], qa2 = xa[2 * N + i];
     const double qb0 = xb[i], qb1 = xb[N + i], qb2 = xb[2 * N + i];
     auto dist = [&](int j, double& A, double& B) {
@@ -666,7 +795,7 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(KsgArgs args, unsigned n
     nx = wave_sum(nx);
     ny = wave_sum(ny);
     if (lane == 0) {
-        atomicAdd(&args.fixsum[p], args.psi[nx] + args.psi[ny]);
+        atomicAdd(&args.fixsum[p], d.psi[nx] + d.psi[ny]);
         if (tie) atomicAdd(&args.counters[1], 1u);
         if (args.rc_nx) {
             const size_t o = (size_t)p * N + i;
@@ -677,18 +806,23 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(KsgArgs args, unsigned n
     }
 }
 
-__global__ void ksg_clear_kernel(double* fixsum, long long npairs, unsigned* counters)
+__global__ void ksg_clear_kernel(double* fixsum, long long npairs, unsigned* counters,
+                                 unsigned long long* queue)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < npairs) fixsum[t] = 0.0;
     if (t < 2) counters[t] = 0u;
+    if (t == 0 && queue) *queue = 0ull;
 }
 
 // MI = psi[N] + phi[K] - sum/N as lib/kraskov.cpp:1318.
-__global__ void ksg_finalize_kernel(KsgArgs args)
+__global__ void ksg_finalize_kernel(FixupArgs args, long long nitems, const double* partial,
+                                    double* mi)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < args.npairs) args.mi[p] = args.psiN_phiK - (args.partial[p] + args.fixsum[p]) / args.N;
+    if (p >= nitems) return;
+    const TrajDesc d = args.traj[args.item_traj ? args.item_traj[p] : 0];
+    mi[p] = d.psiN_phiK - (partial[p] + args.fixsum[p]) / d.N;
 }
 
 // Wave primitive self-check: out[0..63] inclusive scan of lane+1 (mod 7
@@ -719,49 +853,87 @@ hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, fl
     return hipGetLastError();
 }
 
-hipError_t launch_ksg_clear(const KsgArgs& a, hipStream_t s)
+hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
+                            unsigned long long* queue, hipStream_t s)
 {
-    const long long n = a.npairs > 2 ? a.npairs : 2;
-    hipLaunchKernelGGL(ksg_clear_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       a.fixsum, a.npairs, a.counters);
+    const long long m = n > 2 ? n : 2;
+    hipLaunchKernelGGL(ksg_clear_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
+                       fixsum, n, counters, queue);
     return hipGetLastError();
 }
 
-hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s)
+// Kernel variants: kind 0 = one wave per row (N <= 256*S4), kind 1 = four
+// waves per row (N <= 1024*S4).  default_variant picks the smallest that fits.
+bool variant_fits(int kind, int s4, int N)
 {
-    const int nchunk = (a.N + 3) / 4;
-    const int s4 = (nchunk + kWave - 1) / kWave;
+    const long long cap = (kind == 0 ? 256LL : 1024LL) * s4;
+    const bool inst = kind == 0 ? (s4 == 1 || s4 == 2 || s4 == 3 || s4 == 4 || s4 == 5 ||
+                                   s4 == 6 || s4 == 8)
+                                : (s4 >= 2 && s4 <= 8);
+    return inst && N <= cap;
+}
+
+void default_variant(int N, int& kind, int& s4)
+{
+    static const int single[] = {1, 2, 3, 4, 5, 6, 8};
+    for (int v : single)
+        if (N <= 256 * v) { kind = 0; s4 = v; return; }
+    kind = 1;
+    s4 = (N + 1023) / 1024;
+    if (s4 < 2) s4 = 2;
+}
+
+hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4)
+{
+    if (kind < 0) default_variant(a.N, kind, s4);
+    if (!variant_fits(kind, s4, a.N)) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(((a.npairs + 7) / 8) * 8)), block(kWave * kPairWaves);
-    if (s4 <= 1)
-        hipLaunchKernelGGL(ksg_pair_kernel<1>, grid, block, 0, s, a);
-    else if (s4 <= 2)
-        hipLaunchKernelGGL(ksg_pair_kernel<2>, grid, block, 0, s, a);
-    else if (s4 <= 4)
-        hipLaunchKernelGGL(ksg_pair_kernel<4>, grid, block, 0, s, a);
-    else if (s4 <= 8)
-        hipLaunchKernelGGL(ksg_pair_kernel<8>, grid, block, 0, s, a);
-    else if (s4 <= 16)
-        hipLaunchKernelGGL(ksg_pair_coop_kernel<4>, grid, block, 0, s, a);
-    else if (s4 <= 32)
-        hipLaunchKernelGGL(ksg_pair_coop_kernel<8>, grid, block, 0, s, a);
-    else
-        return hipErrorInvalidValue;
+#define GMX_SINGLE(V) case V: hipLaunchKernelGGL(ksg_pair_kernel<V>, grid, block, 0, s, a); break;
+#define GMX_COOP(V) case V: hipLaunchKernelGGL(ksg_pair_coop_kernel<V>, grid, block, 0, s, a); break;
+    if (kind == 0) {
+        switch (s4) {
+            GMX_SINGLE(1) GMX_SINGLE(2) GMX_SINGLE(3) GMX_SINGLE(4) GMX_SINGLE(5) GMX_SINGLE(6)
+            GMX_SINGLE(8)
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (s4) {
+            GMX_COOP(2) GMX_COOP(3) GMX_COOP(4) GMX_COOP(5) GMX_COOP(6) GMX_COOP(7) GMX_COOP(8)
+            default: return hipErrorInvalidValue;
+        }
+    }
+#undef GMX_SINGLE
+#undef GMX_COOP
     return hipGetLastError();
 }
 
-hipError_t launch_ksg_fixup(const KsgArgs& a, unsigned nqueued, hipStream_t s)
+hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s)
 {
     if (nqueued == 0) return hipSuccess;
     const unsigned per_block = 4;
     hipLaunchKernelGGL(ksg_fixup_kernel, dim3((nqueued + per_block - 1) / per_block),
-                       dim3(kWave * per_block), 0, s, a, nqueued);
+                       dim3(kWave * per_block), 0, s, f, nqueued);
     return hipGetLastError();
 }
 
-hipError_t launch_ksg_finalize(const KsgArgs& a, hipStream_t s)
+hipError_t launch_ksg_finalize(const FixupArgs& f, long long nitems, const double* partial,
+                               double* mi, hipStream_t s)
 {
-    hipLaunchKernelGGL(ksg_finalize_kernel, dim3((unsigned)((a.npairs + 255) / 256)), dim3(256), 0,
-                       s, a);
+    hipLaunchKernelGGL(ksg_finalize_kernel, dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0,
+                       s, f, nitems, partial, mi);
+    return hipGetLastError();
+}
+
+bool batch_variant_ok(int kind, int s4, int N)
+{
+    if (kind == 0) return (s4 == 2 || s4 == 4 || s4 == 6 || s4 == 8) && N <= 256 * s4;
+    if (kind == 1) return (s4 == 3 || s4 == 4 || s4 == 6 || s4 == 8) && N <= 1024 * s4;
+    return false;
+}
+
+hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(ksg_batch_kernel, dim3(nblocks), dim3(kWave * kPairWaves), 0, s, b);
     return hipGetLastError();
 }
 

@@ -100,6 +100,8 @@ struct gmxcorr_ksg_plan {
     bool loaded = false;
     bool all_pairs = false;
     long long last_npairs = 0;
+    int var_kind = -1, var_s4 = 0;
+    gmxcorr::TrajDesc* dTraj = nullptr;   // one descriptor for the recount/finalise kernels
     gmxcorr_ksg_stats st{};
 };
 
@@ -144,11 +146,20 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail)
     a.counters = pl->dCounters;
     a.npairs = npairs; a.P = pl->P; a.N = pl->N; a.K = pl->K; a.psiN_phiK = pl->psiN_phiK;
     if (detail) { a.rc_nx = pl->dRcNx; a.rc_ny = pl->dRcNy; a.rc_flag = pl->dRcFlag; }
+    gmxcorr::FixupArgs f{};
+    f.traj = pl->dTraj;
+    f.item_traj = nullptr;
+    f.a1 = pl->dA1;
+    f.a2 = pl->dA2;
+    f.fixsum = pl->dFixsum;
+    f.counters = pl->dCounters;
+    if (detail) { f.rc_nx = pl->dRcNx; f.rc_ny = pl->dRcNy; f.rc_flag = pl->dRcFlag; }
     for (int attempt = 0; attempt < 2; ++attempt) {
         a.fixq = pl->dFix; a.fixcap = pl->fixcap;
-        HIPCHK(gmxcorr::launch_ksg_clear(a, pl->stream));
+        f.fixq = pl->dFix;
+        HIPCHK(gmxcorr::launch_ksg_clear(pl->dFixsum, npairs, pl->dCounters, nullptr, pl->stream));
         HIPCHK(hipEventRecord(pl->ev[2], pl->stream));
-        HIPCHK(gmxcorr::launch_ksg_pairs(a, pl->stream));
+        HIPCHK(gmxcorr::launch_ksg_pairs(a, pl->stream, pl->var_kind, pl->var_s4));
         HIPCHK(hipEventRecord(pl->ev[3], pl->stream));
         unsigned counters[2] = {0, 0};
         HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,
@@ -159,8 +170,8 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail)
             if (rc) return rc;
             continue;
         }
-        HIPCHK(gmxcorr::launch_ksg_fixup(a, counters[0], pl->stream));
-        HIPCHK(gmxcorr::launch_ksg_finalize(a, pl->stream));
+        HIPCHK(gmxcorr::launch_ksg_fixup(f, counters[0], pl->stream));
+        HIPCHK(gmxcorr::launch_ksg_finalize(f, npairs, pl->dPartial, pl->dMI, pl->stream));
         HIPCHK(hipEventRecord(pl->ev[4], pl->stream));
         HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,
                               pl->stream));
@@ -241,6 +252,8 @@ int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** o
         return bail(hip_fail(e, "hipMalloc(distance rows)"));
     if ((e = hipMalloc(&pl->dCounters, 4 * sizeof(unsigned))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(counters)"));
+    if ((e = hipMalloc(&pl->dTraj, sizeof(gmxcorr::TrajDesc))) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc(descriptor)"));
     int rc = ensure_fix(pl, 65536);
     if (rc) return bail(rc);
     pl->st.row_bytes = (long long)dbytes;
@@ -268,6 +281,17 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
     pl->st.ms_load = elapsed(pl->ev[0], pl->ev[1]);
     pl->st.ms_build = elapsed(pl->ev[1], pl->ev[5]);
     pl->psiN_phiK = psi[pl->N] + (psi[pl->K] - 1 / (1.0 * pl->K));
+    gmxcorr::TrajDesc td{};
+    td.X = pl->dX;
+    td.D = pl->dD;
+    td.psi = pl->dPsi;
+    td.P = pl->P;
+    td.N = pl->N;
+    td.K = pl->K;
+    td.psiN_phiK = pl->psiN_phiK;
+    td.kind = -1;
+    td.s4 = 0;
+    HIPCHK(hipMemcpy(pl->dTraj, &td, sizeof(td), hipMemcpyHostToDevice));
     pl->loaded = true;
     return GMXCORR_OK;
 }
@@ -347,6 +371,21 @@ int gmxcorr_ksg_plan_matrix(gmxcorr_ksg_plan* pl, double* mat)
     return GMXCORR_OK;
 }
 
+int gmxcorr_ksg_plan_set_variant(gmxcorr_ksg_plan* pl, int kind, int s4)
+{
+    if (!pl) return fail(GMXCORR_EINVAL, "set_variant: null plan");
+    if (kind >= 0 && !gmxcorr::variant_fits(kind, s4, pl->N))
+        return fail(GMXCORR_EUNSUP, "kernel variant does not cover this frame count");
+    pl->var_kind = kind;
+    pl->var_s4 = s4;
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_variant_fits(int kind, int s4, int nframes)
+{
+    return gmxcorr::variant_fits(kind, s4, nframes) ? 1 : 0;
+}
+
 int gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* pl, gmxcorr_ksg_stats* st)
 {
     if (!pl || !st) return fail(GMXCORR_EINVAL, "plan_stats: null argument");
@@ -361,6 +400,7 @@ void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* pl)
     dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dA1); dfree(pl->dA2);
     dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix);
     dfree(pl->dCounters); dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
+    dfree(pl->dTraj);
     for (auto& ev : pl->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (pl->stream) (void)hipStreamDestroy(pl->stream);

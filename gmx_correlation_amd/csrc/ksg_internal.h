@@ -37,6 +37,49 @@ struct KsgArgs {
     double        psiN_phiK; // psi[N] + (psi[K] - 1/K)
 };
 
+// One trajectory's device data and shape.  kind/s4 select the pair-kernel
+// code path for it in the batched kernel (see variant_fits).
+struct TrajDesc {
+    const double* X;     // [3n][N] normalised rows (FP64)
+    const float* D;      // [n][N][P] distance rows (FP32)
+    const double* psi;   // [N+1]
+    long long P;
+    int N;
+    int K;
+    double psiN_phiK;
+    int kind;
+    int s4;
+};
+
+// Arguments of the FP64 recount and finalise kernels.  item_traj == nullptr
+// means every item belongs to traj[0].
+struct FixupArgs {
+    const TrajDesc* traj;
+    const int* item_traj;
+    const int* a1;
+    const int* a2;
+    const FixEntry* fixq;
+    double* fixsum;
+    unsigned* counters;
+    int* rc_nx;
+    int* rc_ny;
+    unsigned char* rc_flag;
+};
+
+// Arguments of the persistent batched pair kernel.
+struct KsgBatchArgs {
+    const TrajDesc* traj;
+    const int* item_traj;
+    const int* a1;
+    const int* a2;
+    long long nitems;
+    double* partial;
+    FixEntry* fixq;
+    unsigned* counters;
+    unsigned fixcap;
+    unsigned long long* queue;
+};
+
 // Largest N handled by the pair kernels (cooperative kernel, 4 x 2048 frames).
 constexpr int kMaxFrames = 8192;
 
@@ -51,10 +94,19 @@ struct PlanView {
 
 hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, float* D,
                              hipStream_t s);
-hipError_t launch_ksg_clear(const KsgArgs& a, hipStream_t s);
-hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s);
-hipError_t launch_ksg_fixup(const KsgArgs& a, unsigned nqueued, hipStream_t s);
-hipError_t launch_ksg_finalize(const KsgArgs& a, hipStream_t s);
+hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
+                            unsigned long long* queue, hipStream_t s);
+// kind 0: one wave per frame row, N <= 256*s4 (s4 in 1,2,3,4,5,6,8);
+// kind 1: four waves per row, N <= 1024*s4 (s4 in 2..8); kind < 0: default.
+bool variant_fits(int kind, int s4, int N);
+void default_variant(int N, int& kind, int& s4);
+hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind = -1, int s4 = 0);
+hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s);
+hipError_t launch_ksg_finalize(const FixupArgs& f, long long nitems, const double* partial,
+                               double* mi, hipStream_t s);
+// Code paths compiled into the batched kernel.
+bool batch_variant_ok(int kind, int s4, int N);
+hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s);
 hipError_t launch_selftest(int* out, hipStream_t s);
 
 }  // namespace gmxcorr

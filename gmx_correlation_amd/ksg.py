@@ -115,6 +115,11 @@ class KsgPlan:
             return out, dict(nx=nx, ny=ny, flags=fl)
         return out
 
+    def set_variant(self, kind: int, s4: int = 0) -> None:
+        """Pair-kernel variant: kind 0 one wave per row (256*s4 frames),
+        kind 1 four waves per row (1024*s4 frames), kind -1 default."""
+        _lib.check(_lib.load().gmxcorr_ksg_plan_set_variant(self._h, kind, s4))
+
     def stats(self) -> dict:
         st = _lib.KsgStats()
         _lib.check(_lib.load().gmxcorr_ksg_plan_stats(self._h, ctypes.byref(st)))

@@ -79,6 +79,13 @@ int  gmxcorr_ksg_plan_result(gmxcorr_ksg_plan* plan, double* mi, long long npair
 /* Fill mat[natoms*natoms] from the last run_all (diagonal 2000). */
 int  gmxcorr_ksg_plan_matrix(gmxcorr_ksg_plan* plan, double* mat);
 int  gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* plan, gmxcorr_ksg_stats* st);
+/* Pair-kernel variant (the autotuner's knob): kind 0 = one wavefront per
+ * frame row holding 256*s4 frames (s4 in 1,2,3,4,5,6,8); kind 1 = four
+ * wavefronts per row holding 1024*s4 frames (s4 in 2..8); kind -1 = the
+ * default for the plan's frame count.  variant_fits: 1 if (kind, s4) is
+ * compiled and covers nframes. */
+int  gmxcorr_ksg_plan_set_variant(gmxcorr_ksg_plan* plan, int kind, int s4);
+int  gmxcorr_ksg_variant_fits(int kind, int s4, int nframes);
 void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* plan);
 
 /* ---- Gaussian and Gaussian-copula matrices --------------------------------
@@ -106,6 +113,35 @@ int gmxcorr_gauss_matrix(const double* traj, int natoms, int nframes, double* ma
  * copula scores; here the transform runs on the GPU too.  nframes >= 6. */
 int gmxcorr_gcmi_matrix(const double* traj, int natoms, int nframes, double* mat,
                         double* cov_out, double* z_out, gmxcorr_dense_stats* st);
+
+/* ---- Batched uneven trajectories (no counterpart in the reference) --------
+ * Many independent trajectories, each with its own atom count, frame count
+ * and k.  run() packs them into memory waves of at most max_row_bytes of
+ * distance rows; each wave is one persistent pair-kernel launch over all its
+ * pairs, each trajectory on the code path (kind, s4) given at add() (from
+ * the autotuner; -1 = smallest path that fits). */
+typedef struct gmxcorr_ksg_batch gmxcorr_ksg_batch;
+typedef struct {
+    double ms_total;      /* first upload to last result, all waves */
+    double ms_build;      /* uploads + row builds */
+    double ms_pairs;      /* persistent pair launches */
+    double ms_fixup;      /* FP64 recount + finalise */
+    long long npairs;
+    long long fixup_rows;
+    long long ties;
+    int nwaves;
+} gmxcorr_batch_stats;
+int  gmxcorr_ksg_batch_create(gmxcorr_ksg_batch** out);
+/* Returns the trajectory's index (>= 0) or a negative status. */
+int  gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const double* psi,
+                           int natoms, int nframes, int k, int kind, int s4);
+int  gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nblocks);
+/* MI of trajectory t's pairs (a1 = 1.., a2 < a1); ms_done: batch-relative
+ * completion time of its wave; wave: its wave index (both optional). */
+int  gmxcorr_ksg_batch_result(gmxcorr_ksg_batch* b, int t, double* mi, long long npairs,
+                              double* ms_done, int* wave);
+int  gmxcorr_ksg_batch_stats(const gmxcorr_ksg_batch* b, gmxcorr_batch_stats* st);
+void gmxcorr_ksg_batch_destroy(gmxcorr_ksg_batch* b);
 
 /* ---- Learned surrogate (no counterpart in the reference) -------------------
  * Features of a pair list on a loaded KSG plan: feat [npairs][16] holds the
