@@ -124,6 +124,136 @@ def bench_dense(args) -> int:
     print(json.dumps(out), flush=True)
     return 0
 
+def bench_hybrid(args) -> int:
+    """Surrogate + KSG fallback: train on the exact r(MI) of 500 atoms x 4000
+    frames (k=100), run on 2000 atoms x 5000 frames (k=100).  The plain GPU
+    KSG of every inference pair is also run: it times the plain path and
+    gives the exact r for the error of every pair the surrogate keeps."""
+    from gmx_correlation_amd import ksg, surrogate, synth
+    out = {"workload": "hybrid surrogate+KSG: train 500x4000, infer 2000x5000, k=100"}
+    k = 100
+    spec = synth.SURR_TRAIN
+    log(f"bench: training set {spec}")
+    rows, psi = ksg.kraskov_prepare(ksg.center(synth.generate(spec)))
+    n = spec.natoms
+    a1, a2 = np.tril_indices(n, -1)
+    with ksg.KsgPlan(n, spec.nframes, k) as plan:
+        plan.load(rows, psi)
+        t0 = time.perf_counter()
+        mi = plan.run_all()
+        out["train_ksg_s"] = time.perf_counter() - t0
+        out["train_ksg_pairs_per_s"] = len(mi) / out["train_ksg_s"]
+        feat, ms_feat = surrogate.pair_features(plan, a1, a2)
+    r = surrogate.mi_to_r(mi)
+    t0 = time.perf_counter()
+    model = surrogate.Surrogate(8192, seed=0).fit(feat, r, epochs=args.epochs, batch=4096,
+                                                   lr=2e-3)
+    out["train_fit_s"] = time.perf_counter() - t0
+    out["train_pairs"] = int(len(r))
+    mu_tr, _, _ = model.predict(feat)
+    out["train_rmse_r"] = float(np.sqrt(np.mean((mu_tr - r) ** 2)))
+    log(f"bench: trained on {len(r)} pairs in {out['train_fit_s']:.1f} s, rmse "
+        f"{out['train_rmse_r']:.4f}")
+    del rows, feat
+    spec = synth.SURR_INFER
+    log(f"bench: inference set {spec}")
+    rows, psi = ksg.kraskov_prepare(ksg.center(synth.generate(spec)))
+    n = spec.natoms
+    a1, a2 = np.tril_indices(n, -1)
+    with ksg.KsgPlan(n, spec.nframes, k) as plan:
+        plan.load(rows, psi)
+        log("bench: plain GPU KSG over all inference pairs")
+        t0 = time.perf_counter()
+        mi = plan.run_all()
+        t_plain = time.perf_counter() - t0
+        st = plan.stats()
+        r_exact = surrogate.mi_to_r(mi)
+        out["infer_pairs"] = int(len(mi))
+        out["plain_s"] = t_plain
+        out["plain_pairs_per_s"] = len(mi) / t_plain
+        out["plain_kernel_ms"] = st["ms_pairs"]
+        log(f"bench: plain {len(mi)} pairs in {t_plain:.1f} s")
+        res = surrogate.hybrid(plan, a1, a2, model, sigma_max=2e-4, audit=0.01, seed=0)
+        err = np.abs(res["r"] - r_exact)
+        out["hybrid_sigma_max"] = 2e-4
+        out["hybrid_s"] = res["wall_s"]
+        out["hybrid_pairs_per_s"] = len(mi) / res["wall_s"]
+        out["hybrid_speedup_vs_plain"] = t_plain / res["wall_s"]
+        out["hybrid_fallback_fraction"] = res["fallback_fraction"]
+        out["hybrid_max_abs_dr_all_pairs"] = float(err.max())
+        out["hybrid_audit_max_abs_dr"] = res["audit_max_abs_dr"]
+        out["ms_features"] = res["ms_features"]
+        out["ms_inference_bf16"] = res["ms_inference"]
+        mu, sigma = res["mu"], res["sigma"]
+        surr_err = np.abs(mu - r_exact)
+        out["surrogate_rmse_r"] = float(np.sqrt(np.mean(surr_err ** 2)))
+        out["gaussian_r_rmse"] = None
+        # trade-off from the exact r of every pair: kept pairs' worst error,
+        # fallback fraction and the time model t = features + inference + f * t_plain
+        t_fixed = (res["ms_features"] + res["ms_inference"]) / 1e3
+        curve = []
+        for smax in (2e-4, 1e-3, 3e-3, 1e-2, 3e-2, 1e-1):
+            keep = sigma <= smax
+            f = 1.0 - keep.mean() + 0.01
+            curve.append({"sigma_max": smax, "fallback_fraction": float(min(1.0, f)),
+                          "max_abs_dr": float(surr_err[keep].max()) if keep.any() else 0.0,
+                          "modelled_speedup": t_plain / (t_fixed + min(1.0, f) * t_plain)})
+        out["tradeoff"] = curve
+    log(f"bench: hybrid {out}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_batched(args) -> int:
+    """32 seeded trajectories (atoms 50-600, frames 1000-8000, k in 10/50/100)
+    in persistent launches, paths from the autotuner; agreement with the
+    reference's CPU float64 KSG on sampled pairs."""
+    from gmx_correlation_amd import batch, ksg, synth
+    from oracle import oracle as O
+    specs = synth.batched_specs()
+    tuner = batch.Tuner()
+    b = batch.KsgBatch()
+    coords_keep = {}
+    t_tune = 0.0
+    choices = []
+    for t, (spec, k) in enumerate(specs):
+        coords = synth.generate(spec)
+        rows, psi = ksg.kraskov_prepare(ksg.center(coords))
+        t0 = time.perf_counter()
+        kind, s4 = tuner.choose(rows, psi, k)
+        t_tune += time.perf_counter() - t0
+        choices.append({"natoms": spec.natoms, "nframes": spec.nframes, "k": k, "kind": kind,
+                        "s4": s4})
+        b.add(rows, psi, k, kind, s4)
+        if t % 4 == 0:
+            coords_keep[t] = coords
+        log(f"bench: traj {t} {spec.natoms}x{spec.nframes} k={k} -> path ({kind},{s4})")
+    st = b.run(max_row_bytes=int(args.row_budget_gb * 1e9), nblocks=2048)
+    npairs = st["npairs"]
+    lat = []
+    agree = []
+    for t, (spec, k) in enumerate(specs):
+        mi, ms_done, wave = b.result(t)
+        lat.append({"traj": t, "wave": wave, "ms_done": ms_done})
+        if t in coords_keep:
+            ref = O.ref_ksg_pairs(coords_keep[t], k, [1, spec.natoms - 1], [0, spec.natoms - 2],
+                                  nthreads=2) if O.ref_available() else None
+            if ref is not None:
+                from gmx_correlation_amd.ksg import pearsify
+                got = np.array([mi[0], mi[-1]])
+                agree.append(float(np.max(np.abs(pearsify(got) - pearsify(ref)))))
+    out = {"workload": "batched 32 uneven trajectories (configs: atoms 50-600, frames "
+                       "1000-8000, k 10/50/100)",
+           "pairs": npairs, "pairs_per_s": npairs / (st["ms_total"] / 1e3),
+           "ms_total": st["ms_total"], "ms_pairs": st["ms_pairs"], "ms_build": st["ms_build"],
+           "ms_fixup": st["ms_fixup"], "waves": st["nwaves"], "fixup_rows": st["fixup_rows"],
+           "ties": st["ties"], "tune_s": t_tune, "paths": choices,
+           "latency_ms": lat, "max_abs_dr_vs_reference_cpu_sampled": max(agree) if agree else None,
+           "sampled_pairs_checked": 2 * len(agree)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 
 def _gcmi_from_cov(c, n):
     """GCMI pair MI from an FP64 second-moment matrix, batched determinants."""
@@ -151,8 +281,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU-baseline work (0 disables)")
-    ap.add_argument("--workload", default="ksg", choices=["ksg", "dense"],
-                    help="ksg: the headline metric; dense: Gaussian/GCMI GEMM path")
+    ap.add_argument("--workload", default="ksg", choices=["ksg", "dense", "hybrid", "batched"],
+                    help="ksg: the headline metric; dense: Gaussian/GCMI GEMM path; "
+                         "hybrid: surrogate + KSG fallback; batched: uneven trajectories")
+    ap.add_argument("--epochs", type=int, default=120, help="surrogate training epochs")
+    ap.add_argument("--row-budget-gb", type=float, default=200.0,
+                    help="distance-row bytes per memory wave (batched)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -161,6 +295,10 @@ def main() -> int:
         return 2
     if args.workload == "dense":
         return bench_dense(args)
+    if args.workload == "hybrid":
+        return bench_hybrid(args)
+    if args.workload == "batched":
+        return bench_batched(args)
 
     from gmx_correlation_amd import _lib, ksg, synth
 

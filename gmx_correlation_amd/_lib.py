@@ -91,6 +91,7 @@ C_SYMBOLS = [
     "gmxcorr_ksg_plan_stats", "gmxcorr_ksg_plan_destroy", "gmxcorr_ksg_plan_set_variant",
     "gmxcorr_ksg_variant_fits",
     "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix",
+    "gmxcorr_te_matrix", "gmxcorr_te_pairs",
     "gmxcorr_ksg_batch_create", "gmxcorr_ksg_batch_add", "gmxcorr_ksg_batch_run",
     "gmxcorr_ksg_batch_result", "gmxcorr_ksg_batch_stats", "gmxcorr_ksg_batch_destroy",
     "gmxcorr_ksg_plan_features", "gmxcorr_mlp_num_params", "gmxcorr_mlp_create",
@@ -140,6 +141,8 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_mlp_forward.argtypes = [_vp, _fp, _ll, _i, _fp, _dp]
     L.gmxcorr_mlp_destroy.argtypes = [_vp]
     L.gmxcorr_mlp_destroy.restype = None
+    L.gmxcorr_te_matrix.argtypes = [_dp, _i, _i, _i, _i, _dp, _dp]
+    L.gmxcorr_te_pairs.argtypes = [_dp, _i, _i, _i, _i, _ll, _ip, _ip, _dp, _dp]
     L.gmxcorr_ksg_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
     L.gmxcorr_ksg_batch_add.argtypes = [_vp, _dp, _dp, _i, _i, _i, _i, _i]
     L.gmxcorr_ksg_batch_run.argtypes = [_vp, _ll, _i]

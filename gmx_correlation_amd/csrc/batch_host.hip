@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -140,6 +142,32 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             }
         }
     }
+    // one arena for the distance rows (and one for rows + psi), reused by
+    // every wave: allocating and freeing ~10^11 bytes per wave costs seconds
+    auto align = [](long long v) { return (v + 255) / 256 * 256; };
+    long long dbytes = 0, xbytes = 0;
+    for (const auto& ts : waves) {
+        long long d = 0, x = 0;
+        for (int t : ts) {
+            d += align(b->traj[t].row_bytes());
+            x += align((long long)b->traj[t].rows.size() * 8) + align((long long)b->traj[t].psi.size() * 8);
+        }
+        dbytes = std::max(dbytes, d);
+        xbytes = std::max(xbytes, x);
+    }
+    char* dArena = nullptr;
+    char* xArena = nullptr;
+    const auto ta0 = std::chrono::steady_clock::now();
+    HIPCHK(hipMalloc(&dArena, (size_t)dbytes));
+    if (hipMalloc(&xArena, (size_t)xbytes) != hipSuccess) {
+        (void)hipFree(dArena);
+        return gmxcorr::set_error(GMXCORR_ENOMEM, "batch_run: cannot allocate the coordinate arena");
+    }
+    const double ms_alloc =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta0).count();
+    std::fprintf(stderr, "gmxcorr batch: %zu waves, row arena %.1f GB allocated in %.1f ms\n",
+                 waves.size(), dbytes / 1e9, ms_alloc);
+    std::fflush(stderr);
     hipStream_t s;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     hipEvent_t ev[5];
@@ -160,20 +188,24 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
         unsigned long long* dQueue = nullptr;
         auto cleanup = [&]() {
             (void)hipStreamSynchronize(s);
-            for (auto& p : dX) dfree(p);
-            for (auto& p : dPsi) dfree(p);
-            for (auto& p : dD) dfree(p);
             dfree(dDesc); dfree(dItemTraj); dfree(dA1); dfree(dA2); dfree(dPartial);
             dfree(dFixsum); dfree(dMI); dfree(dFix); dfree(dCnt); dfree(dQueue);
         };
         auto body = [&]() -> int {
+            std::fprintf(stderr, "gmxcorr batch: wave %zu/%zu: building rows of %zu trajectories\n",
+                         wv + 1, waves.size(), ts.size());
+            std::fflush(stderr);
             std::vector<gmxcorr::TrajDesc> desc(ts.size());
             HIPCHK(hipEventRecord(ev[1], s));
+            long long doff = 0, xoff = 0;
             for (size_t q = 0; q < ts.size(); ++q) {
                 HostTraj& h = b->traj[ts[q]];
-                HIPCHK(hipMalloc(&dX[q], h.rows.size() * sizeof(double)));
-                HIPCHK(hipMalloc(&dPsi[q], h.psi.size() * sizeof(double)));
-                HIPCHK(hipMalloc(&dD[q], (size_t)h.row_bytes()));
+                dD[q] = reinterpret_cast<float*>(dArena + doff);
+                doff += align(h.row_bytes());
+                dX[q] = reinterpret_cast<double*>(xArena + xoff);
+                xoff += align((long long)h.rows.size() * 8);
+                dPsi[q] = reinterpret_cast<double*>(xArena + xoff);
+                xoff += align((long long)h.psi.size() * 8);
                 HIPCHK(hipMemcpyAsync(dX[q], h.rows.data(), h.rows.size() * sizeof(double),
                                       hipMemcpyHostToDevice, s));
                 HIPCHK(hipMemcpyAsync(dPsi[q], h.psi.data(), h.psi.size() * sizeof(double),
@@ -210,7 +242,12 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             HIPCHK(hipMalloc(&dMI, nitems * sizeof(double)));
             HIPCHK(hipMalloc(&dCnt, 4 * sizeof(unsigned)));
             HIPCHK(hipMalloc(&dQueue, sizeof(unsigned long long)));
-            unsigned fixcap = 1u << 18;
+            // recount queue for 0.2% of the wave's rows (0.04-0.07% measured)
+            long long rows_w = 0;
+            for (size_t q = 0; q < ts.size(); ++q)
+                rows_w += b->traj[ts[q]].npairs() * (long long)b->traj[ts[q]].N;
+            unsigned fixcap = (unsigned)std::min<long long>(std::max<long long>(1 << 18, rows_w / 500),
+                                                            64LL << 20);
             HIPCHK(hipMalloc(&dFix, (size_t)fixcap * sizeof(gmxcorr::FixEntry)));
             HIPCHK(hipMemcpyAsync(dDesc, desc.data(), desc.size() * sizeof(gmxcorr::TrajDesc),
                                   hipMemcpyHostToDevice, s));
@@ -238,6 +275,8 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                 HIPCHK(hipMemcpyAsync(counters, dCnt, sizeof(counters), hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
                 if (counters[0] <= fixcap) break;
+                std::fprintf(stderr, "gmxcorr batch: recount queue overflow (%u rows), re-running the wave\n",
+                             counters[0]);
                 dfree(dFix);
                 fixcap = counters[0] + counters[0] / 4 + 1024;
                 HIPCHK(hipMalloc(&dFix, (size_t)fixcap * sizeof(gmxcorr::FixEntry)));
@@ -261,6 +300,11 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             ms_total = tt;
             fix_rows += counters[0];
             ties += counters[1];
+            std::fprintf(stderr,
+                         "gmxcorr batch: wave %zu/%zu: %zu trajectories, %lld pairs, rows %.1f ms, "
+                         "pairs %.1f ms, recount %.1f ms\n",
+                         wv + 1, waves.size(), ts.size(), nitems, tb, tp, tf);
+            std::fflush(stderr);
             for (size_t q = 0; q < ts.size(); ++q) {
                 HostTraj& h = b->traj[ts[q]];
                 h.mi.assign(mi.begin() + first[q], mi.begin() + first[q] + h.npairs());
@@ -274,6 +318,8 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(s);
+    (void)hipFree(dArena);
+    (void)hipFree(xArena);
     if (rc != GMXCORR_OK) return rc;
     b->st.nwaves = (int)waves.size();
     b->st.ms_total = ms_total;

@@ -1,0 +1,193 @@
+// Transfer entropy (Frenzel-Pompe conditional MI) kernels for gfx950.
+//
+// TE(a2 -> a1) at lag L (lib/transfer_entropy.cpp:63-221): for query frame
+// q < Nv = N - L, the 9-D point [X_a1(q+L), X_a1(q), X_a2(q)] of the raw
+// mean-centred coordinates; eps = k-th smallest max-norm distance to the
+// other Nv-1 points; counts with strict '<' in the (Xf,Xp), (Yp,Xp) and Xp
+// subspaces; TE = mean over q of psi[k] + psi[n_xp+1] - psi[n_xfxp+1]
+// - psi[n_ypxp+1] (counts clamped to Nv).
+//
+// One lane per query frame, 64 queries per wave, one wave per block.  The
+// frame loop is uniform across the wave, so frame j's nine coordinates are
+// scalar loads.  Distances are FP64, computed exactly as the reference does.
+//   pass 1  the k smallest FP32 keys fl32(dist) in a per-lane max-heap in LDS
+//           (rounding to FP32 is monotone, so the heap root is fl32(eps));
+//   pass 2  eps in FP64: the r-th smallest FP64 distance among the frames
+//           whose key equals the root, r = k - #{key < root};
+//   pass 3  the three strict counts in FP64.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "te_internal.h"
+
+namespace gmxcorr {
+
+namespace {
+
+struct Frame9 {
+    double f[9];
+};
+
+// Nine coordinates of frame j: X_a1(j+lag), X_a1(j), X_a2(j).
+__device__ __forceinline__ void load_frame(const double* __restrict__ X,
+                                           const double* __restrict__ Y, int j, int lag,
+                                           double* c)
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        c[d] = X[(size_t)(j + lag) * 3 + d];
+        c[3 + d] = X[(size_t)j * 3 + d];
+        c[6 + d] = Y[(size_t)j * 3 + d];
+    }
+}
+
+}  // namespace
+
+template <int KMAX>
+__global__ __launch_bounds__(64) void te_kernel(TeArgs args)
+{
+    __shared__ float heap[KMAX][64];
+    const int lane = threadIdx.x;
+    const int nqb = args.nqblocks;
+    const long long p = blockIdx.x / nqb;
+    const int qb = blockIdx.x % nqb;
+    const int N = args.N, lag = args.lag, k = args.k, Nv = N - lag;
+    const double* X = args.traj + (size_t)args.a1[p] * N * 3;
+    const double* Y = args.traj + (size_t)args.a2[p] * N * 3;
+    const int q = qb * 64 + lane;
+    const bool active = q < Nv;
+    double qc[9];
+    load_frame(X, Y, active ? q : 0, lag, qc);
+    auto dist9 = [&](const double* c, double& dxf, double& dxp, double& dyp) {
+        dxf = fmax(fmax(fabs(qc[0] - c[0]), fabs(qc[1] - c[1])), fabs(qc[2] - c[2]));
+        dxp = fmax(fmax(fabs(qc[3] - c[3]), fabs(qc[4] - c[4])), fabs(qc[5] - c[5]));
+        dyp = fmax(fmax(fabs(qc[6] - c[6]), fabs(qc[7] - c[7])), fabs(qc[8] - c[8]));
+    };
+
+    // ---- pass 1: k smallest keys (max-heap, root = largest kept) -------
+    for (int t = 0; t < k; ++t) heap[t][lane] = INFINITY;
+    for (int j = 0; j < Nv; ++j) {
+        double c[9];
+        load_frame(X, Y, j, lag, c);
+        double dxf, dxp, dyp;
+        dist9(c, dxf, dxp, dyp);
+        const float key = (float)fmax(fmax(dxf, dxp), dyp);
+        if (active && j != q && key < heap[0][lane]) {
+            // replace the root and sift down
+            int pos = 0;
+            for (;;) {
+                const int l = 2 * pos + 1, r = l + 1;
+                if (l >= k) break;
+                int big = l;
+                float vb = heap[l][lane];
+                if (r < k) {
+                    const float vr = heap[r][lane];
+                    if (vr > vb) { big = r; vb = vr; }
+                }
+                if (vb <= key) break;
+                heap[pos][lane] = vb;
+                pos = big;
+            }
+            heap[pos][lane] = key;
+        }
+    }
+    const float root = heap[0][lane];
+
+    // ---- pass 2: eps in FP64 ----------------------------------------
+    int clt = 0, gcnt = 0;
+    double gmin = INFINITY, gmax = -INFINITY;
+    for (int j = 0; j < Nv; ++j) {
+        double c[9];
+        load_frame(X, Y, j, lag, c);
+        double dxf, dxp, dyp;
+        dist9(c, dxf, dxp, dyp);
+        const double dd = fmax(fmax(dxf, dxp), dyp);
+        const float key = (float)dd;
+        if (j == q) continue;
+        clt += key < root;
+        if (key == root) {
+            ++gcnt;
+            gmin = fmin(gmin, dd);
+            gmax = fmax(gmax, dd);
+        }
+    }
+    const int r = k - clt;   // 1 <= r <= gcnt
+    double eps = gmin;
+    if (gcnt > 1 && r > 1 && !(r == gcnt)) {
+        // walk the group's distinct FP64 values upwards (rare)
+        double prev = -1.0;
+        int cum = 0;
+        for (;;) {
+            double vmin = INFINITY;
+            int m = 0;
+            for (int j = 0; j < Nv; ++j) {
+                if (j == q) continue;
+                double c[9];
+                load_frame(X, Y, j, lag, c);
+                double dxf, dxp, dyp;
+                dist9(c, dxf, dxp, dyp);
+                const double dd = fmax(fmax(dxf, dxp), dyp);
+                if ((float)dd == root && dd > prev) {
+                    if (dd < vmin) { vmin = dd; m = 1; }
+                    else if (dd == vmin) ++m;
+                }
+            }
+            if (m == 0 || cum + m >= r) { eps = vmin; break; }
+            cum += m;
+            prev = vmin;
+        }
+    } else if (r == gcnt) {
+        eps = gmax;
+    }
+
+    // ---- pass 3: strict counts ------------------------------------------
+    int nxfxp = 0, nypxp = 0, nxp = 0;
+    for (int j = 0; j < Nv; ++j) {
+        double c[9];
+        load_frame(X, Y, j, lag, c);
+        double dxf, dxp, dyp;
+        dist9(c, dxf, dxp, dyp);
+        if (j == q) continue;
+        nxfxp += fmax(dxf, dxp) < eps;
+        nypxp += fmax(dyp, dxp) < eps;
+        nxp += dxp < eps;
+    }
+    double te = 0.0;
+    if (active) {
+        const double* psi = args.psi;
+        te = psi[k] + psi[min(nxp, Nv) + 1] - psi[min(nxfxp, Nv) + 1] - psi[min(nypxp, Nv) + 1];
+    }
+    // wave sum, fixed order
+    for (int o = 32; o >= 1; o >>= 1) te += __shfl_xor(te, o, 64);
+    if (lane == 0) args.partial[blockIdx.x] = te;
+}
+
+__global__ void te_finalize_kernel(TeArgs args, double* mat, int natoms)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= args.npairs) return;
+    double s = 0.0;
+    for (int b = 0; b < args.nqblocks; ++b) s += args.partial[p * args.nqblocks + b];
+    mat[(size_t)args.a1[p] * natoms + args.a2[p]] = s / (args.N - args.lag);
+}
+
+hipError_t launch_te(const TeArgs& a, hipStream_t s)
+{
+    const unsigned grid = (unsigned)(a.npairs * a.nqblocks);
+    if (a.k <= 32)
+        hipLaunchKernelGGL(te_kernel<32>, dim3(grid), dim3(64), 0, s, a);
+    else if (a.k <= 128)
+        hipLaunchKernelGGL(te_kernel<128>, dim3(grid), dim3(64), 0, s, a);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_te_finalize(const TeArgs& a, double* mat, int natoms, hipStream_t s)
+{
+    hipLaunchKernelGGL(te_finalize_kernel, dim3((unsigned)((a.npairs + 255) / 256)), dim3(256), 0,
+                       s, a, mat, natoms);
+    return hipGetLastError();
+}
+
+}  // namespace gmxcorr

@@ -60,3 +60,29 @@ def gcmi_corrmatrix(traj, return_cov: bool = False, return_z: bool = False, stat
     if stats:
         out.append(st.as_dict())
     return out[0] if len(out) == 1 else tuple(out)
+
+
+def te_matrix(traj, k: int = 100, lag: int = 1, return_ms: bool = False):
+    """Transfer-entropy matrix mat[a1][a2] = TE(a2 -> a1) of the raw
+    mean-centred trajectory (lib/transfer_entropy.cpp:63-221)."""
+    t = _traj(traj)
+    n, N, _ = t.shape
+    mat = np.zeros((n, n))
+    ms = ctypes.c_double(0.0)
+    _lib.check(_lib.load().gmxcorr_te_matrix(_lib.ptr(t), n, N, k, lag, _lib.ptr(mat),
+                                             ctypes.byref(ms)))
+    return (mat, ms.value) if return_ms else mat
+
+
+def te_pairs(traj, a1, a2, k: int = 100, lag: int = 1):
+    """TE(a2[p] -> a1[p]) for a list of ordered pairs, and the kernel ms."""
+    t = _traj(traj)
+    n, N, _ = t.shape
+    a1 = np.ascontiguousarray(a1, dtype=np.int32)
+    a2 = np.ascontiguousarray(a2, dtype=np.int32)
+    out = np.empty(len(a1))
+    ms = ctypes.c_double(0.0)
+    _lib.check(_lib.load().gmxcorr_te_pairs(_lib.ptr(t), n, N, k, lag, len(a1),
+                                            _lib.ptr(a1, _lib._ip), _lib.ptr(a2, _lib._ip),
+                                            _lib.ptr(out), ctypes.byref(ms)))
+    return out, ms.value

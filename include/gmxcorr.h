@@ -114,6 +114,18 @@ int gmxcorr_gauss_matrix(const double* traj, int natoms, int nframes, double* ma
 int gmxcorr_gcmi_matrix(const double* traj, int natoms, int nframes, double* mat,
                         double* cov_out, double* z_out, gmxcorr_dense_stats* st);
 
+/* ---- Transfer entropy -------------------------------------------------------
+ * traj: mean-centred fluctuations [natoms][nframes][3] (raw, as the reference
+ * CPU path uses them).  mat[a1*n+a2] = TE(a2 -> a1) at `lag`, diagonal 0,
+ * all zero (with a warning) when nframes - lag <= k
+ * (lib/transfer_entropy.cpp:63-221; the reference's GPU signature is
+ * te_corrmatrix_metal, lib/te_metal.h).  k <= 128.  ms (optional): kernel
+ * time.  te_pairs: the same for an explicit list of ordered pairs. */
+int gmxcorr_te_matrix(const double* traj, int natoms, int nframes, int k, int lag, double* mat,
+                      double* ms);
+int gmxcorr_te_pairs(const double* traj, int natoms, int nframes, int k, int lag,
+                     long long npairs, const int* a1, const int* a2, double* te, double* ms);
+
 /* ---- Batched uneven trajectories (no counterpart in the reference) --------
  * Many independent trajectories, each with its own atom count, frame count
  * and k.  run() packs them into memory waves of at most max_row_bytes of
