@@ -254,6 +254,33 @@ def bench_batched(args) -> int:
     return 0
 
 
+def bench_te(args) -> int:
+    """Transfer entropy at 1000 atoms x 10000 frames (Student-t noise), k=100,
+    lags 1, 5, 10, timed on a sample of ordered pairs; three pairs checked
+    against the FP64 oracle."""
+    from gmx_correlation_amd import dense, ksg, synth
+    from oracle import oracle as O
+    spec = synth.DENSE_1000
+    log(f"bench: generating workload {spec}")
+    traj = ksg.center(synth.generate(spec))
+    n = spec.natoms
+    rng = np.random.default_rng(0)
+    npairs = args.te_pairs
+    a1 = rng.integers(0, n, npairs)
+    a2 = (a1 + rng.integers(1, n, npairs)) % n
+    out = {"workload": "te 1000 atoms x 10000 frames k=100", "sample_pairs": npairs, "lags": {}}
+    for lag in (1, 5, 10):
+        te, ms = dense.te_pairs(traj, a1, a2, k=100, lag=lag)
+        chk = []
+        for p in range(2):
+            chk.append(abs(te[p] - O.te_pair(traj, int(a1[p]), int(a2[p]), 100, lag)))
+        out["lags"][lag] = {"kernel_ms": ms, "pairs_per_s": npairs / (ms / 1e3),
+                            "full_matrix_s_projected": n * (n - 1) / (npairs / (ms / 1e3)),
+                            "max_abs_dte_vs_fp64_oracle": float(max(chk))}
+        log(f"bench: te lag {lag}: {out['lags'][lag]}")
+    print(json.dumps(out), flush=True)
+    return 0
+
 
 def _gcmi_from_cov(c, n):
     """GCMI pair MI from an FP64 second-moment matrix, batched determinants."""
@@ -281,10 +308,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU-baseline work (0 disables)")
-    ap.add_argument("--workload", default="ksg", choices=["ksg", "dense", "hybrid", "batched"],
+    ap.add_argument("--workload", default="ksg", choices=["ksg", "dense", "hybrid", "batched", "te"],
                     help="ksg: the headline metric; dense: Gaussian/GCMI GEMM path; "
                          "hybrid: surrogate + KSG fallback; batched: uneven trajectories")
     ap.add_argument("--epochs", type=int, default=120, help="surrogate training epochs")
+    ap.add_argument("--te-pairs", type=int, default=512, help="ordered pairs timed (te)")
     ap.add_argument("--row-budget-gb", type=float, default=200.0,
                     help="distance-row bytes per memory wave (batched)")
     args = ap.parse_args()
@@ -299,6 +327,8 @@ def main() -> int:
         return bench_hybrid(args)
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "te":
+        return bench_te(args)
 
     from gmx_correlation_amd import _lib, ksg, synth
 
