@@ -112,6 +112,34 @@ __device__ __forceinline__ int wave_incl_scan(int v)
     return v;
 }
 
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t u)
+{
+    int v = (int)u;
+    v = (int)max((uint32_t)v, (uint32_t)dpp<0xb1>(v));
+    v = (int)max((uint32_t)v, (uint32_t)dpp<0x4e>(v));
+    v = (int)max((uint32_t)v, (uint32_t)dpp<0x141>(v));
+    v = (int)max((uint32_t)v, (uint32_t)dpp<0x140>(v));
+    const uint32_t r0 = (uint32_t)readlane(v, 0), r1 = (uint32_t)readlane(v, 16);
+    const uint32_t r2 = (uint32_t)readlane(v, 32), r3 = (uint32_t)readlane(v, 48);
+    return max(max(r0, r1), max(r2, r3));
+}
+
+// Number of set bits of m below this lane.
+__device__ __forceinline__ int mbcnt(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Sweep-A bin of a key: the key clamped to [2^-20, 2^12) keeps its exponent
+// and top 6 mantissa bits.
+__device__ __forceinline__ int bin_of(uint32_t key)
+{
+    constexpr uint32_t kLoKey = (uint32_t)kBinBase << kBinShift;
+    constexpr uint32_t kHiKey = ((uint32_t)(kBinBase + kBins) << kBinShift) - 1u;
+    return (int)(min(max(key, kLoKey), kHiKey) >> kBinShift) - kBinBase;
+}
+
 // FP64 helpers for the recount kernel (rare rows; plain shuffles suffice).
 __device__ __forceinline__ double wave_max_d(double v)
 {
@@ -247,13 +275,14 @@ __device__ __forceinline__ double row_single(const float* Da, const float* Db, l
                                              int lane, const RowOut& o)
 {
     const int nchunk = (N + 3) / 4;
-    const float4* ra = reinterpret_cast<const float4*>(Da + (size_t)i * P);
-    const float4* rb = reinterpret_cast<const float4*>(Db + (size_t)i * P);
-    float a[4 * S4], b[4 * S4];
+    const uint4* ra = reinterpret_cast<const uint4*>(Da + (size_t)i * P);
+    const uint4* rb = reinterpret_cast<const uint4*>(Db + (size_t)i * P);
+    // Bit patterns: non-negative floats (and +inf) order as unsigned ints.
+    uint32_t a[4 * S4], b[4 * S4];
 #pragma unroll
     for (int m = 0; m < S4; ++m) {
         const int c = m * kWave + lane;
-        float4 va = make_float4(INFINITY, INFINITY, INFINITY, INFINITY), vb = va;
+        uint4 va = make_uint4(kInfBits, kInfBits, kInfBits, kInfBits), vb = va;
         if (c < nchunk) {
             va = ra[c];
             vb = rb[c];
@@ -266,97 +295,90 @@ __device__ __forceinline__ double row_single(const float* Da, const float* Db, l
     clear_hist<kBins>(hist, lane);
     wave_sync();
 #pragma unroll
-    for (int e = 0; e < 4 * S4; ++e) {
-        const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
-        const int bin = min(max((int)(key >> kBinShift) - kBinBase, 0), kBins - 1);
-        atomicAdd(&hist[bin], 1u);
-    }
+    for (int e = 0; e < 4 * S4; ++e)
+        atomicAdd(&hist[bin_of(max(a[e], b[e]))], 1u);
     wave_sync();
     int b1, r0, pop;
     find_bin<kBins / kWave>(hist, lane, T, b1, r0, pop);
     wave_sync();
-    bool unknown = b1 == 0 || b1 == kBins - 1 || pop > kWave;
+    const bool unknown = b1 == 0 || b1 == kBins - 1 || pop > kWave;
     const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
+    // The keys are recomputed below rather than kept in registers.
+#pragma unroll
+    for (int e = 0; e < 4 * S4; ++e) asm volatile("" : "+v"(a[e]), "+v"(b[e]));
 
-    // ---- sweep B: sure members' marginal maxima, bin candidates -------
-    float ex = 0.0f, ey = 0.0f;
-    int nc = 0;
-#pragma unroll
-    for (int e = 0; e < 4 * S4; ++e) {
-        const uint32_t key = __float_as_uint(fmaxf(a[e], b[e]));
-        const bool sure = key < lo;
-        ex = fmaxf(ex, sure ? a[e] : 0.0f);
-        ey = fmaxf(ey, sure ? b[e] : 0.0f);
-        if (key - lo < (1u << kBinShift)) {
-            if (nc < kCandCap) {
-                hist[(nc * kWave + lane) * 2 + 0] = __float_as_uint(a[e]);
-                hist[(nc * kWave + lane) * 2 + 1] = __float_as_uint(b[e]);
-            }
-            ++nc;
-        }
-    }
-    unknown = unknown || wave_max_i(nc) > kCandCap;
     uint32_t eps_key = kUnknownEps;
-    bool sel_tie = false;
+    bool flag = true;
+    int nx = -1, ny = -1;
     if (!unknown) {
-        // compact the candidates to one per lane (entries 512.. of the LDS row)
-        const int off = wave_incl_scan(nc) - nc;
+        // ---- sweep B: sure members' marginal maxima; the frames inside
+        //      the target bin are appended, compacted, to LDS (pop <= 64)
+        uint2* cand = reinterpret_cast<uint2*>(hist);
+        uint32_t ex = 0u, ey = 0u;
+        int nstored = 0;
 #pragma unroll
-        for (int t = 0; t < kCandCap; ++t)
-            if (t < nc) {
-                const uint32_t ca = hist[(t * kWave + lane) * 2 + 0];
-                const uint32_t cb = hist[(t * kWave + lane) * 2 + 1];
-                hist[512 + (off + t) * 2 + 0] = ca;
-                hist[512 + (off + t) * 2 + 1] = cb;
+        for (int e = 0; e < 4 * S4; ++e) {
+            const uint32_t key = max(a[e], b[e]);
+            const bool sure = key < lo;
+            ex = max(ex, sure ? a[e] : 0u);
+            ey = max(ey, sure ? b[e] : 0u);
+            const bool inbin = key - lo < (1u << kBinShift);
+            const uint64_t cm = __ballot(inbin);
+            if (cm) {
+                if (inbin) cand[nstored + mbcnt(cm)] = make_uint2(a[e], b[e]);
+                nstored += __popcll(cm);
             }
-        wave_sync();
-        const bool act = lane < pop;
-        float ca = 0.0f, cb = 0.0f;
-        uint32_t low = 0xffffffffu;
-        if (act) {
-            ca = __uint_as_float(hist[512 + lane * 2 + 0]);
-            cb = __uint_as_float(hist[512 + lane * 2 + 1]);
-            low = __float_as_uint(fmaxf(ca, cb)) - lo;   // < 2^17
         }
-        // r0-th smallest of the candidates' low 17 bits, one bit at a time
-        uint32_t sel = 0;
+        wave_sync();
+        // ---- select: r0-th smallest of the candidates' low bits, a
+        //      ballot radix that stops once one candidate is left
+        const bool act = lane < pop;
+        uint32_t ca = 0u, cb = 0u, low = 0xffffffffu;
+        if (act) {
+            const uint2 c = cand[lane];
+            ca = c.x;
+            cb = c.y;
+            low = max(ca, cb) - lo;   // < 2^17
+        }
+        uint64_t live = __ballot(act);
+        uint32_t pre = 0u;
         int r = r0;
-        for (int bit = kBinShift - 1; bit >= 0; --bit) {
-            const bool zero = act && (low >> (bit + 1)) == sel && ((low >> bit) & 1u) == 0u;
-            const int c0 = __popcll(__ballot(zero));
-            sel <<= 1;
+        for (int bit = kBinShift - 1; bit >= 0 && __popcll(live) > 1; --bit) {
+            const uint64_t z = __ballot((low >> bit) == (pre << 1));
+            const int c0 = __popcll(z);
             if (r > c0) {
                 r -= c0;
-                sel |= 1u;
+                pre = (pre << 1) | 1u;
+                live &= ~z;
+            } else {
+                pre <<= 1;
+                live = z;
             }
         }
+        const uint32_t sel = (uint32_t)readlane((int)low, __ffsll((unsigned long long)live) - 1);
         eps_key = lo + sel;
         const bool in = act && low <= sel;
-        sel_tie = __popcll(__ballot(in)) != r0;
-        ex = fmaxf(ex, in ? ca : 0.0f);
-        ey = fmaxf(ey, in ? cb : 0.0f);
-    }
-    ex = wave_max(ex);
-    ey = wave_max(ey);
-    wave_sync();
+        const bool sel_tie = __popcll(__ballot(in)) != r0;
+        ex = wave_max_u(max(ex, in ? ca : 0u));
+        ey = wave_max_u(max(ey, in ? cb : 0u));
 
-    // ---- sweep C: marginal counts (self included, removed below) ------
-    int cx = 0, cy = 0;
+        // ---- sweep C: marginal counts by ballot (self included) ---------
+        int cxl = 0, cxe = 0, cyl = 0, cye = 0;
 #pragma unroll
-    for (int e = 0; e < 4 * S4; ++e) {
-        cx += (a[e] <= ex) + ((a[e] == ex) << 16);
-        cy += (b[e] <= ey) + ((b[e] == ey) << 16);
+        for (int e = 0; e < 4 * S4; ++e) {
+            cxl += __popcll(__ballot(a[e] <= ex));
+            cxe += __popcll(__ballot(a[e] == ex));
+            cyl += __popcll(__ballot(b[e] <= ey));
+            cye += __popcll(__ballot(b[e] == ey));
+        }
+        nx = cxl - 1;
+        ny = cyl - 1;
+        flag = sel_tie || cxe > 1 || cye > 1;
     }
-    cx = wave_sum(cx);
-    cy = wave_sum(cy);
-    const int nx = (cx & 0xffff) - 1, ny = (cy & 0xffff) - 1;
-    const bool flag = unknown || sel_tie || (cx >> 16) > 1 || (cy >> 16) > 1;
     double contrib = 0.0;
+    if (!flag) contrib = o.psi[nx] + o.psi[ny];
     if (lane == 0) {
-        if (flag)
-            queue_row(o, item, i, eps_key);
-        else
-            contrib = o.psi[nx] + o.psi[ny];
+        if (flag) queue_row(o, item, i, eps_key);
         if (o.rc_nx) {
             const size_t off = (size_t)item * N + i;
             o.rc_nx[off] = flag ? -1 : nx;
