@@ -90,9 +90,7 @@ int gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const doubl
     if (!gmxcorr::batch_variant_ok(kind, s4, nframes))
         return gmxcorr::set_error(GMXCORR_EUNSUP, "batch_add: variant not compiled into the batch kernel");
     const size_t nx = (size_t)3 * natoms * nframes;
-    for (size_t t = 0; t < nx; ++t)
-        if (!std::isfinite(xrows[t]))
-            return gmxcorr::set_error(GMXCORR_EINVAL, "batch_add: non-finite normalised coordinate");
+    if (int rc = gmxcorr::check_rows(xrows, 3LL * natoms, nframes, "batch_add")) return rc;
     HostTraj h;
     h.natoms = natoms;
     h.N = nframes;

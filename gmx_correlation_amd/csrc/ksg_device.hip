@@ -41,6 +41,10 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kPairWaves = 4;
 constexpr uint32_t kInfBits = 0x7f800000u;
+// Row padding (frames N..P-1): 3072.0f.  The host accepts only rows whose
+// coordinate spread is below 2048, so every real distance is below 2^11 and
+// the padding is never a neighbour; all keys stay below 2^12.
+constexpr uint32_t kPadBits = 0x45400000u;
 constexpr uint32_t kUnknownEps = 0xffffffffu;
 // Sweep-A binning: bin = clamp((key >> 17) - kBinBase, 0, 2047); key >> 17
 // is (exponent << 6 | top 6 mantissa bits), bin 0 starts at 2^-20.
@@ -48,6 +52,7 @@ constexpr int kBinShift = 17;
 constexpr int kBinBase = (127 - 20) << 6;
 constexpr int kBins = 2048;
 constexpr int kCandCap = 4;  // candidate slots per lane
+constexpr int kBinRot = (kBinBase / (kBins / kWave)) % kWave;  // raw-bin rotation in lane chunks
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -140,6 +145,26 @@ __device__ __forceinline__ int bin_of(uint32_t key)
     return (int)(min(max(key, kLoKey), kHiKey) >> kBinShift) - kBinBase;
 }
 
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+// LDS byte address of a wave's histogram (8 KB aligned by declaration).
+__device__ __forceinline__ uint32_t lds_addr(uint32_t* hist)
+{
+    return (uint32_t)(uintptr_t)(lds_u32*)hist;
+}
+
+// Sweep-A increment for the joint key max(a, b) < 2^12: the key raised to
+// 2^-20, bits 27..17, i.e. the rotated bin (bin + kBinBase) mod 2048, OR-ed
+// into the 8 KB-aligned histogram base (three VALU operations).
+__device__ __forceinline__ void hist_add(uint32_t base, uint32_t a, uint32_t b)
+{
+    constexpr uint32_t kLoKey = (uint32_t)kBinBase << kBinShift;
+    const uint32_t addr =
+        ((max(max(a, b), kLoKey) >> (kBinShift - 2)) & (uint32_t)(4 * kBins - 4)) | base;
+    __hip_atomic_fetch_add((lds_u32*)(uintptr_t)addr, 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 // FP64 helpers for the recount kernel (rare rows; plain shuffles suffice).
 __device__ __forceinline__ double wave_max_d(double v)
 {
@@ -156,22 +181,31 @@ __device__ __forceinline__ double wave_min_d(double v)
 }
 
 // Bin holding the t-th smallest element (1-based) of a histogram of 64*BPL
-// bins; lane l owns bins [l*BPL, (l+1)*BPL).  Returns the bin, the rank of
-// the target inside it and the bin's population.
-template <int BPL>
+// bins; lane l owns bins [l*BPL, (l+1)*BPL), stored from word
+// ((l + ROT) % 64) * BPL of the array (the histogram is rotated by ROT*BPL
+// words).  Returns the bin, the rank of the target inside it and the bin's
+// population.
+template <int BPL, int ROT = 0>
 __device__ __forceinline__ void find_bin(const uint32_t* hist, int lane, int t, int& bin,
                                          int& rank, int& pop)
 {
+    static_assert(BPL % 4 == 0, "16-byte reads");
+    const uint4* mine = reinterpret_cast<const uint4*>(hist + ((lane + ROT) & (kWave - 1)) * BPL);
     int s = 0;
-    // rotated order keeps the lanes on distinct LDS banks
+    // 16-byte reads, issued together; the rotated order spreads the lanes
+    // over the LDS banks
 #pragma unroll
-    for (int q = 0; q < BPL; ++q) s += (int)hist[lane * BPL + ((q + lane) & (BPL - 1))];
+    for (int q = 0; q < BPL / 4; ++q) {
+        const uint4 v = mine[(q + lane) & (BPL / 4 - 1)];
+        s += (int)((v.x + v.y) + (v.z + v.w));
+    }
     const int incl = wave_incl_scan(s);
     const int excl = incl - s;
     const uint64_t bal = __ballot(excl < t && t <= incl);
     const int owner = bal ? __ffsll((unsigned long long)bal) - 1 : 0;
     const int base = readlane(excl, owner);
-    const int v = lane < BPL ? (int)hist[owner * BPL + lane] : 0;
+    const uint32_t* own = hist + ((owner + ROT) & (kWave - 1)) * BPL;
+    const int v = lane < BPL ? (int)own[lane] : 0;
     const int vi = wave_incl_scan(v);
     const int r = t - base;
     const uint64_t b2 = __ballot(lane < BPL && vi - v < r && r <= vi);
@@ -232,7 +266,7 @@ __device__ __forceinline__ void clear_hist(uint32_t* hist, int lane)
 // ---------------------------------------------------------------------------
 // Per-atom distance rows: D[a][i][j] = fl32(max_d |X[3a+d][i] - X[3a+d][j]|)
 // computed in FP64 exactly as the reference does (fabs of the difference,
-// lib/kraskov.cpp:131-132, 280-281), then rounded once.  j in [N, P) = +inf.
+// lib/kraskov.cpp:131-132, 280-281), then rounded once.  j in [N, P) = 3072.
 // One block per row (a, i).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __restrict__ X, int N,
@@ -257,7 +291,7 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
                 d = fmax(d, fabs(q2 - x2[j]));
                 v[e] = (float)d;
             } else {
-                v[e] = __int_as_float((int)kInfBits);
+                v[e] = __int_as_float((int)kPadBits);
             }
         }
         out[c] = make_float4(v[0], v[1], v[2], v[3]);
@@ -282,7 +316,7 @@ __device__ __forceinline__ double row_single(const float* Da, const float* Db, l
 #pragma unroll
     for (int m = 0; m < S4; ++m) {
         const int c = m * kWave + lane;
-        uint4 va = make_uint4(kInfBits, kInfBits, kInfBits, kInfBits), vb = va;
+        uint4 va = make_uint4(kPadBits, kPadBits, kPadBits, kPadBits), vb = va;
         if (c < nchunk) {
             va = ra[c];
             vb = rb[c];
@@ -294,12 +328,12 @@ __device__ __forceinline__ double row_single(const float* Da, const float* Db, l
     // ---- sweep A: fine histogram of the joint keys --------------------
     clear_hist<kBins>(hist, lane);
     wave_sync();
+    const uint32_t hbase = lds_addr(hist);
 #pragma unroll
-    for (int e = 0; e < 4 * S4; ++e)
-        atomicAdd(&hist[bin_of(max(a[e], b[e]))], 1u);
+    for (int e = 0; e < 4 * S4; ++e) hist_add(hbase, a[e], b[e]);
     wave_sync();
     int b1, r0, pop;
-    find_bin<kBins / kWave>(hist, lane, T, b1, r0, pop);
+    find_bin<kBins / kWave, kBinRot>(hist, lane, T, b1, r0, pop);
     wave_sync();
     const bool unknown = b1 == 0 || b1 == kBins - 1 || pop > kWave;
     const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
@@ -573,7 +607,7 @@ __device__ __forceinline__ long long xcd_pair(long long npairs)
 template <int S4>
 __global__ __launch_bounds__(256) void ksg_pair_kernel(KsgArgs args)
 {
-    __shared__ uint32_t lds[kPairWaves][kBins];
+    __shared__ __attribute__((aligned(4 * kBins))) uint32_t lds[kPairWaves][kBins];
     __shared__ double wsum[kPairWaves];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
@@ -614,7 +648,7 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
 // counter until the queue is empty; each item runs the code path (kind, s4)
 // that the tuner chose for its trajectory.
 // ---------------------------------------------------------------------------
-union BatchLds {
+union __attribute__((aligned(4 * kBins))) BatchLds {
     uint32_t single[kPairWaves][kBins];
     CoopLds coop;
 };

@@ -38,6 +38,28 @@ int set_hip_error(hipError_t e, const char* where)
 
 bool device_ok();
 
+int check_rows(const double* xrows, long long nrows, int N, const char* who)
+{
+    for (long long r = 0; r < nrows; ++r) {
+        const double* x = xrows + r * N;
+        double lo = x[0], hi = x[0];
+        for (int j = 0; j < N; ++j) {
+            if (!std::isfinite(x[j]))
+                return set_error(GMXCORR_EINVAL,
+                                 std::string(who) +
+                                     ": non-finite normalised coordinate (constant component: zero SD)");
+            lo = std::min(lo, x[j]);
+            hi = std::max(hi, x[j]);
+        }
+        if (!(hi - lo < kMaxRowSpread))
+            return set_error(GMXCORR_EINVAL,
+                             std::string(who) +
+                                 ": normalised coordinate spread >= 2048 (rows must be z-scores, as "
+                                 "kraskov_prepare makes them)");
+    }
+    return GMXCORR_OK;
+}
+
 }  // namespace gmxcorr
 
 namespace {
@@ -276,10 +298,7 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
 {
     if (!pl || !xrows || !psi) return fail(GMXCORR_EINVAL, "plan_load: null argument");
     const size_t nx = (size_t)3 * pl->natoms * pl->N;
-    for (size_t t = 0; t < nx; ++t)
-        if (!std::isfinite(xrows[t]))
-            return fail(GMXCORR_EINVAL,
-                        "non-finite normalised coordinate (constant component: zero SD)");
+    if (int rc = gmxcorr::check_rows(xrows, 3LL * pl->natoms, pl->N, "plan_load")) return rc;
     HIPCHK(hipEventRecord(pl->ev[0], pl->stream));
     HIPCHK(hipMemcpyAsync(pl->dX, xrows, nx * sizeof(double), hipMemcpyHostToDevice, pl->stream));
     HIPCHK(hipMemcpyAsync(pl->dPsi, psi, (pl->N + 1) * sizeof(double), hipMemcpyHostToDevice,

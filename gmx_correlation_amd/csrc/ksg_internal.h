@@ -6,6 +6,14 @@
 
 namespace gmxcorr {
 
+// Largest accepted max - min of one normalised coordinate row.  z-scores of
+// N frames lie within sqrt(N) of zero, far inside it; the pair kernels rely
+// on every distance being below 2^11 (padding 3072, keys below 2^12).
+constexpr double kMaxRowSpread = 2048.0;
+
+// Rejects rows with a non-finite value or a spread >= kMaxRowSpread.
+int check_rows(const double* xrows, long long nrows, int N, const char* who);
+
 // A frame row whose FP32 boundary is ambiguous, queued for the FP64 recount.
 struct FixEntry {
     int      pair;     // index into the pair list of the run

@@ -220,6 +220,18 @@ def test_rejects_constant_component(gpu):
     assert e.value.code == -1
 
 
+def test_rejects_rows_that_are_not_zscores(gpu, small_ws):
+    """The pair kernels assume every distance is below 2048 (z-scored rows)."""
+    rows, psi = small_ws
+    wide = rows.copy()
+    wide[5, 7] += 5000.0
+    with ksg.KsgPlan(64, 500, 10) as plan:
+        with pytest.raises(_lib.GmxcorrError) as e:
+            plan.load(wide, psi)
+        assert "spread" in str(e.value)
+        plan.load(rows, psi)            # the plan stays usable
+
+
 def test_rejects_bad_pair_indices(gpu, small_ws):
     rows, psi = small_ws
     with ksg.KsgPlan(64, 500, 10) as plan:
