@@ -93,6 +93,7 @@ C_SYMBOLS = [
     "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix",
     "gmxcorr_te_matrix", "gmxcorr_te_pairs",
     "gmxcorr_ksg_batch_create", "gmxcorr_ksg_batch_add", "gmxcorr_ksg_batch_run",
+    "gmxcorr_ksg_batch_traj_bytes",
     "gmxcorr_ksg_batch_result", "gmxcorr_ksg_batch_stats", "gmxcorr_ksg_batch_destroy",
     "gmxcorr_ksg_plan_features", "gmxcorr_mlp_num_params", "gmxcorr_mlp_create",
     "gmxcorr_mlp_set_params", "gmxcorr_mlp_get_params", "gmxcorr_mlp_grad",
@@ -145,6 +146,8 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_te_pairs.argtypes = [_dp, _i, _i, _i, _i, _ll, _ip, _ip, _dp, _dp]
     L.gmxcorr_ksg_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
     L.gmxcorr_ksg_batch_add.argtypes = [_vp, _dp, _dp, _i, _i, _i, _i, _i]
+    L.gmxcorr_ksg_batch_traj_bytes.argtypes = [_vp, _i]
+    L.gmxcorr_ksg_batch_traj_bytes.restype = _ll
     L.gmxcorr_ksg_batch_run.argtypes = [_vp, _ll, _i]
     L.gmxcorr_ksg_batch_result.argtypes = [_vp, _i, _dp, _ll, _dp, _ip]
     L.gmxcorr_ksg_batch_stats.argtypes = [_vp, ctypes.POINTER(BatchStats)]

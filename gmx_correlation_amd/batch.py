@@ -101,6 +101,13 @@ class KsgBatch:
         self.shapes.append((natoms, N, k))
         return rc
 
+    def traj_bytes(self, t: int) -> int:
+        """Device bytes of trajectory t's rows in a memory wave."""
+        v = _lib.load().gmxcorr_ksg_batch_traj_bytes(self._h, t)
+        if v < 0:
+            _lib.check(int(v))
+        return int(v)
+
     def run(self, max_row_bytes: int = 200 * 10**9, nblocks: int = 2048) -> dict:
         t0 = time.perf_counter()
         _lib.check(_lib.load().gmxcorr_ksg_batch_run(self._h, int(max_row_bytes), nblocks))

@@ -39,7 +39,16 @@ struct HostTraj {
     double psiN_phiK = 0.0;
     int wave = -1;
     double ms_done = 0.0;       // batch-relative completion time of its wave
-    long long row_bytes() const { return (long long)natoms * N * P * 4; }
+    bool sorted() const { return kind == 0; }
+    long long d_bytes() const { return (long long)natoms * N * P * 4; }
+    long long sd_bytes() const { return sorted() ? (long long)natoms * N * gmxcorr::sorted_pitch(N) * 4 : 0; }
+    long long sm_bytes() const { return sorted() ? (long long)natoms * N * gmxcorr::kSumPitch * 4 : 0; }
+    // device bytes of the distance rows, sorted rows and summaries (256-aligned parts)
+    long long row_bytes() const
+    {
+        auto al = [](long long v) { return (v + 255) / 256 * 256; };
+        return al(d_bytes()) + al(sd_bytes()) + al(sm_bytes());
+    }
     long long npairs() const { return (long long)natoms * (natoms - 1) / 2; }
 };
 
@@ -97,7 +106,7 @@ int gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const doubl
     h.K = k;
     h.kind = kind;
     h.s4 = s4;
-    h.P = ((long long)nframes + 63) / 64 * 64;
+    h.P = gmxcorr::row_pitch(nframes);
     h.rows.assign(xrows, xrows + nx);
     h.psi.assign(psi, psi + nframes + 1);
     h.psiN_phiK = psi[nframes] + (psi[k] - 1 / (1.0 * k));
@@ -199,6 +208,11 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             for (size_t q = 0; q < ts.size(); ++q) {
                 HostTraj& h = b->traj[ts[q]];
                 dD[q] = reinterpret_cast<float*>(dArena + doff);
+                float* dSD = h.sorted() ? reinterpret_cast<float*>(dArena + doff + align(h.d_bytes()))
+                                        : nullptr;
+                float* dSM = h.sorted() ? reinterpret_cast<float*>(dArena + doff + align(h.d_bytes()) +
+                                                                  align(h.sd_bytes()))
+                                        : nullptr;
                 doff += align(h.row_bytes());
                 dX[q] = reinterpret_cast<double*>(xArena + xoff);
                 xoff += align((long long)h.rows.size() * 8);
@@ -208,9 +222,9 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                                       hipMemcpyHostToDevice, s));
                 HIPCHK(hipMemcpyAsync(dPsi[q], h.psi.data(), h.psi.size() * sizeof(double),
                                       hipMemcpyHostToDevice, s));
-                HIPCHK(gmxcorr::launch_build_rows(dX[q], h.natoms, h.N, h.P, dD[q], s));
-                desc[q] = gmxcorr::TrajDesc{dX[q], dD[q], dPsi[q], h.P, h.N, h.K, h.psiN_phiK,
-                                            h.kind, h.s4};
+                HIPCHK(gmxcorr::launch_build_rows(dX[q], h.natoms, h.N, h.P, dD[q], dSD, dSM, s));
+                desc[q] = gmxcorr::TrajDesc{dX[q], dD[q], dSD, dSM, dPsi[q], h.P, h.N, h.K,
+                                            h.psiN_phiK, h.kind, h.s4};
             }
             // items: trajectories by decreasing frame count, pairs a1 > a2 within each
             std::vector<int> tq(ts.size());
@@ -281,7 +295,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                 if (attempt == 1) return gmxcorr::set_error(GMXCORR_EHIP, "recount queue overflow");
             }
             HIPCHK(gmxcorr::launch_ksg_fixup(fa, counters[0], s));
-            HIPCHK(gmxcorr::launch_ksg_finalize(fa, nitems, dPartial, dMI, s));
+            HIPCHK(gmxcorr::launch_ksg_finalize(fa, nitems, dPartial, 1, dMI, s));
             HIPCHK(hipEventRecord(ev[4], s));
             HIPCHK(hipMemcpyAsync(counters, dCnt, sizeof(counters), hipMemcpyDeviceToHost, s));
             std::vector<double> mi(nitems);
@@ -328,6 +342,13 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
     b->st.fixup_rows = fix_rows;
     b->st.ties = ties;
     return GMXCORR_OK;
+}
+
+long long gmxcorr_ksg_batch_traj_bytes(const gmxcorr_ksg_batch* b, int t)
+{
+    if (!b || t < 0 || t >= (int)b->traj.size())
+        return gmxcorr::set_error(GMXCORR_EINVAL, "batch_traj_bytes: bad trajectory index");
+    return b->traj[t].row_bytes();
 }
 
 int gmxcorr_ksg_batch_result(gmxcorr_ksg_batch* b, int t, double* mi, long long npairs,

@@ -145,6 +145,14 @@ __device__ __forceinline__ int bin_of(uint32_t key)
     return (int)(min(max(key, kLoKey), kHiKey) >> kBinShift) - kBinBase;
 }
 
+// Global load at a uniform base plus a 32-bit byte offset (selects the
+// scalar-base addressing form).
+template <class T>
+__device__ __forceinline__ T ld_lane(const void* base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 // LDS byte address of a wave's histogram (8 KB aligned by declaration).
@@ -267,11 +275,19 @@ __device__ __forceinline__ void clear_hist(uint32_t* hist, int lane)
 // Per-atom distance rows: D[a][i][j] = fl32(max_d |X[3a+d][i] - X[3a+d][j]|)
 // computed in FP64 exactly as the reference does (fabs of the difference,
 // lib/kraskov.cpp:131-132, 280-281), then rounded once.  j in [N, P) = 3072.
-// One block per row (a, i).
+// One block per row (a, i).  With SD != nullptr (N <= kSortedMaxFrames) the
+// row is also sorted in LDS (bitonic) and written as
+//   SD[a][i][0..64)        0 (front pad)
+//   SD[a][i][64 + t]       t-th smallest value, t < N; 3072 up to the pitch
+//   SM[a][i][m]            SD value t = 64m + 63 (3072 past N), m < 64
+// so that #{j : D[a][i][j] <= x} is found with two 64-lane loads.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __restrict__ X, int N,
-                                                             long long P, float* __restrict__ D)
+                                                             long long P, float* __restrict__ D,
+                                                             float* __restrict__ SD,
+                                                             float* __restrict__ SM)
 {
+    __shared__ uint32_t sv[kSortedMaxFrames];
     const long long row = blockIdx.x;
     const int a = (int)(row / N), i = (int)(row % N);
     const double* x0 = X + (size_t)(3 * a) * N;
@@ -295,132 +311,244 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
             }
         }
         out[c] = make_float4(v[0], v[1], v[2], v[3]);
+        if (SD) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (4 * c + e < kSortedMaxFrames) sv[4 * c + e] = __float_as_uint(v[e]);
+        }
+    }
+    if (!SD) return;
+    int M = 64;
+    while (M < N) M <<= 1;
+    for (int t = nc * 4 + threadIdx.x; t < M; t += blockDim.x) sv[t] = kPadBits;
+    __syncthreads();
+    // bitonic sort of sv[0..M), ascending as unsigned (non-negative floats)
+    for (int k = 2; k <= M; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
+                const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
+                const uint32_t x = sv[lo], y = sv[hi];
+                if ((x > y) == ((lo & k) == 0)) {
+                    sv[lo] = y;
+                    sv[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    const long long ps = sorted_pitch(N);
+    uint32_t* srow = reinterpret_cast<uint32_t*>(SD) + row * ps;
+    for (int t = threadIdx.x; t < ps; t += blockDim.x) {
+        const int q = t - 64;
+        srow[t] = q < 0 ? 0u : q < N ? sv[q] : kPadBits;
+    }
+    if (threadIdx.x < kSumPitch) {
+        const int q = 64 * threadIdx.x + 63;
+        reinterpret_cast<uint32_t*>(SM)[row * kSumPitch + threadIdx.x] = q < N ? sv[q] : kPadBits;
     }
 }
 
 // ---------------------------------------------------------------------------
-// One frame row of one pair, processed by one wave (N <= 256*S4).  Returns
-// psi[nx] + psi[ny] on lane 0 (0 elsewhere, and for rows queued to the FP64
-// recount).  hist: the wave's 2048-word LDS array.
+// Frame rows i0, i0+step, ... < i1 of one pair, one wave (N <= 256*S4).
+// Returns the sum of psi[nx] + psi[ny] over the rows not queued to the FP64
+// recount (uniform over the wave).  hist: the wave's 2048-word, 8 KB
+// aligned LDS array.
+//
+// Per row: sweep A (histogram), find_bin, sweep B (sure members' marginal
+// maxima; candidates of the target bin appended to LDS), select (ballot
+// radix).  The marginal counts nx = #{j : dA(i,j) <= epsx} come from the
+// sorted row of atom a (SD, SM): the summary gives the 64-block, the block
+// gives the position; a tie at the radius shows as an equal predecessor.
+// Those loads are software-pipelined: summary loads issued after row i's
+// select are consumed after row i+1's sweep A, block loads issued there are
+// consumed after row i+1's select, and psi loads issued then are added
+// after row i+2's sweep A.
 // ---------------------------------------------------------------------------
-template <int S4>
-__device__ __forceinline__ double row_single(const float* Da, const float* Db, long long P, int N,
-                                             int T, int i, long long item, uint32_t* hist,
-                                             int lane, const RowOut& o)
+struct SortedRows {
+    const float* SDa;   // sorted rows of atom a (row i at i * PS)
+    const float* SDb;
+    const float* SMa;   // summaries (row i at i * kSumPitch)
+    const float* SMb;
+    long long PS;
+};
+
+// Count and tie flag of one marginal from its sorted block pair (lanes
+// 0..31: block c1-1, lanes 32..63: block c1, two values per lane).
+__device__ __forceinline__ int sorted_count(uint2 v, int c1, uint32_t r, int lane, bool& tie)
 {
-    const int nchunk = (N + 3) / 4;
-    const uint4* ra = reinterpret_cast<const uint4*>(Da + (size_t)i * P);
-    const uint4* rb = reinterpret_cast<const uint4*>(Db + (size_t)i * P);
-    // Bit patterns: non-negative floats (and +inf) order as unsigned ints.
-    uint32_t a[4 * S4], b[4 * S4];
-#pragma unroll
-    for (int m = 0; m < S4; ++m) {
-        const int c = m * kWave + lane;
-        uint4 va = make_uint4(kPadBits, kPadBits, kPadBits, kPadBits), vb = va;
-        if (c < nchunk) {
-            va = ra[c];
-            vb = rb[c];
-        }
-        a[4 * m + 0] = va.x; a[4 * m + 1] = va.y; a[4 * m + 2] = va.z; a[4 * m + 3] = va.w;
-        b[4 * m + 0] = vb.x; b[4 * m + 1] = vb.y; b[4 * m + 2] = vb.z; b[4 * m + 3] = vb.w;
-    }
+    const bool hi = lane >= 32;
+    const int c2 = __popcll(__ballot(hi && v.x <= r)) + __popcll(__ballot(hi && v.y <= r));
+    const int u = 64 * c1 + c2;   // #{values <= r}, self included
+    const int q = c2 + 62;        // position of the value before the last <= r
+    const uint32_t prev = (uint32_t)readlane((int)((q & 1) ? v.y : v.x), q >> 1);
+    tie = u >= 2 && prev == r;
+    return u;
+}
 
-    // ---- sweep A: fine histogram of the joint keys --------------------
-    clear_hist<kBins>(hist, lane);
-    wave_sync();
+template <int S4>
+__device__ __forceinline__ double rows_single(const float* Da, const float* Db, const SortedRows& sr,
+                                              long long P, int N, int T, int i0, int i1, int step,
+                                              long long item, uint32_t* hist, int lane,
+                                              const RowOut& o)
+{
+    const int nblk = (int)(P / 256);   // 64-lane chunks of 4 frames in a row
     const uint32_t hbase = lds_addr(hist);
+    uint32_t lane16 = (uint32_t)lane * 16u;
+    double acc = 0.0;
+    // pipeline: row r1 waits for its summary loads, row r2 for its block
+    // loads, (px, py) are psi values waiting to be added
+    int r1 = -1, r2 = -1;
+    uint32_t ex1 = 0u, ey1 = 0u, eps1 = 0u, ex2 = 0u, ey2 = 0u, eps2 = 0u;
+    uint32_t sa = 0u, sb = 0u;
+    uint2 ba = make_uint2(0u, 0u), bb = ba;
+    int c1a = 0, c1b = 0;
+    double px = 0.0, py = 0.0;
+    for (int i = i0;; i += step) {
+        const bool have = i < i1;
+        // keeps the lane offset a 32-bit VGPR (global_load with a scalar base)
+        asm volatile("" : "+v"(lane16));
+        // Bit patterns: non-negative floats order as unsigned ints.
+        uint32_t a[4 * S4], b[4 * S4];
+        if (have) {
+            const float* ra = Da + (size_t)i * P;
+            const float* rb = Db + (size_t)i * P;
+            // P is a multiple of 256 frames, so whole 64-lane chunks of 4
+            // frames lie in the row or past it: chunk m is skipped, by a
+            // uniform test, when m >= nblk; frames [N, P) hold padding
 #pragma unroll
-    for (int e = 0; e < 4 * S4; ++e) hist_add(hbase, a[e], b[e]);
-    wave_sync();
-    int b1, r0, pop;
-    find_bin<kBins / kWave, kBinRot>(hist, lane, T, b1, r0, pop);
-    wave_sync();
-    const bool unknown = b1 == 0 || b1 == kBins - 1 || pop > kWave;
-    const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
-    // The keys are recomputed below rather than kept in registers.
+            for (int m = 0; m < S4; ++m) {
+                if (m < nblk) {
+                    const uint4 va = ld_lane<uint4>(ra, lane16 + m * 1024u);
+                    const uint4 vb = ld_lane<uint4>(rb, lane16 + m * 1024u);
+                    a[4 * m + 0] = va.x; a[4 * m + 1] = va.y; a[4 * m + 2] = va.z; a[4 * m + 3] = va.w;
+                    b[4 * m + 0] = vb.x; b[4 * m + 1] = vb.y; b[4 * m + 2] = vb.z; b[4 * m + 3] = vb.w;
+                }
+            }
+            // ---- sweep A: fine histogram of the joint keys ----------------
+            clear_hist<kBins>(hist, lane);
+            wave_sync();
 #pragma unroll
-    for (int e = 0; e < 4 * S4; ++e) asm volatile("" : "+v"(a[e]), "+v"(b[e]));
-
-    uint32_t eps_key = kUnknownEps;
-    bool flag = true;
-    int nx = -1, ny = -1;
-    if (!unknown) {
-        // ---- sweep B: sure members' marginal maxima; the frames inside
-        //      the target bin are appended, compacted, to LDS (pop <= 64)
-        uint2* cand = reinterpret_cast<uint2*>(hist);
-        uint32_t ex = 0u, ey = 0u;
-        int nstored = 0;
+            for (int e = 0; e < 4 * S4; ++e)
+                if (e / 4 < nblk) hist_add(hbase, a[e], b[e]);
+            wave_sync();
+        }
+        // ---- row r1: block position from the summaries, block loads -------
+        acc += px + py;
+        px = py = 0.0;
+        if (r1 >= 0) {
+            c1a = __popcll(__ballot(sa <= ex1));
+            c1b = __popcll(__ballot(sb <= ey1));
+            ba = ld_lane<uint2>(sr.SDa + (size_t)r1 * sr.PS + 64 * c1a, lane16 >> 1);
+            bb = ld_lane<uint2>(sr.SDb + (size_t)r1 * sr.PS + 64 * c1b, lane16 >> 1);
+            r2 = r1; ex2 = ex1; ey2 = ey1; eps2 = eps1;
+            r1 = -1;
+        }
+        uint32_t ex = 0u, ey = 0u, eps_key = kUnknownEps;
+        bool queued = false;
+        if (have) {
+            int b1, r0, pop;
+            find_bin<kBins / kWave, kBinRot>(hist, lane, T, b1, r0, pop);
+            wave_sync();
+            const bool unknown = b1 == 0 || b1 == kBins - 1 || pop > kWave;
+            const uint32_t lo = (uint32_t)(b1 + kBinBase) << kBinShift;
+            // The keys are recomputed below rather than kept in registers.
 #pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            const uint32_t key = max(a[e], b[e]);
-            const bool sure = key < lo;
-            ex = max(ex, sure ? a[e] : 0u);
-            ey = max(ey, sure ? b[e] : 0u);
-            const bool inbin = key - lo < (1u << kBinShift);
-            const uint64_t cm = __ballot(inbin);
-            if (cm) {
-                if (inbin) cand[nstored + mbcnt(cm)] = make_uint2(a[e], b[e]);
-                nstored += __popcll(cm);
+            for (int e = 0; e < 4 * S4; ++e)
+                if (e / 4 < nblk) asm volatile("" : "+v"(a[e]), "+v"(b[e]));
+            queued = unknown;
+            if (!unknown) {
+                // ---- sweep B: sure members' marginal maxima; the frames in
+                //      the target bin are appended, compacted, to LDS
+                uint2* cand = reinterpret_cast<uint2*>(hist);
+                int nstored = 0;
+#pragma unroll
+                for (int e = 0; e < 4 * S4; ++e) {
+                    if (e / 4 >= nblk) continue;
+                    const uint32_t key = max(a[e], b[e]);
+                    const bool sure = key < lo;
+                    ex = max(ex, sure ? a[e] : 0u);
+                    ey = max(ey, sure ? b[e] : 0u);
+                    const bool inbin = key - lo < (1u << kBinShift);
+                    const uint64_t cm = __ballot(inbin);
+                    if (cm) {
+                        if (inbin) cand[nstored + mbcnt(cm)] = make_uint2(a[e], b[e]);
+                        nstored += __popcll(cm);
+                    }
+                }
+                wave_sync();
+                // ---- select: r0-th smallest of the candidates' low bits, a
+                //      ballot radix that stops once one candidate is left
+                const bool act = lane < pop;
+                uint32_t ca = 0u, cb = 0u, low = 0xffffffffu;
+                if (act) {
+                    const uint2 c = cand[lane];
+                    ca = c.x;
+                    cb = c.y;
+                    low = max(ca, cb) - lo;   // < 2^17
+                }
+                uint64_t live = __ballot(act);
+                uint32_t pre = 0u;
+                int r = r0;
+                for (int bit = kBinShift - 1; bit >= 0 && __popcll(live) > 1; --bit) {
+                    const uint64_t z = __ballot((low >> bit) == (pre << 1));
+                    const int c0 = __popcll(z);
+                    if (r > c0) {
+                        r -= c0;
+                        pre = (pre << 1) | 1u;
+                        live &= ~z;
+                    } else {
+                        pre <<= 1;
+                        live = z;
+                    }
+                }
+                const uint32_t sel =
+                    (uint32_t)readlane((int)low, __ffsll((unsigned long long)live) - 1);
+                eps_key = lo + sel;
+                const bool in = act && low <= sel;
+                queued = __popcll(__ballot(in)) != r0;
+                ex = wave_max_u(max(ex, in ? ca : 0u));
+                ey = wave_max_u(max(ey, in ? cb : 0u));
             }
         }
-        wave_sync();
-        // ---- select: r0-th smallest of the candidates' low bits, a
-        //      ballot radix that stops once one candidate is left
-        const bool act = lane < pop;
-        uint32_t ca = 0u, cb = 0u, low = 0xffffffffu;
-        if (act) {
-            const uint2 c = cand[lane];
-            ca = c.x;
-            cb = c.y;
-            low = max(ca, cb) - lo;   // < 2^17
-        }
-        uint64_t live = __ballot(act);
-        uint32_t pre = 0u;
-        int r = r0;
-        for (int bit = kBinShift - 1; bit >= 0 && __popcll(live) > 1; --bit) {
-            const uint64_t z = __ballot((low >> bit) == (pre << 1));
-            const int c0 = __popcll(z);
-            if (r > c0) {
-                r -= c0;
-                pre = (pre << 1) | 1u;
-                live &= ~z;
-            } else {
-                pre <<= 1;
-                live = z;
+        // ---- row r2: counts from the blocks, psi loads or recount ---------
+        if (r2 >= 0) {
+            bool tx, ty;
+            const int nx = sorted_count(ba, c1a, ex2, lane, tx) - 1;
+            const int ny = sorted_count(bb, c1b, ey2, lane, ty) - 1;
+            const bool flag = tx || ty;
+            if (!flag) {
+                px = o.psi[nx];
+                py = o.psi[ny];
             }
+            if (lane == 0) {
+                if (flag) queue_row(o, item, r2, eps2);
+                if (o.rc_nx) {
+                    const size_t off = (size_t)item * N + r2;
+                    o.rc_nx[off] = flag ? -1 : nx;
+                    o.rc_ny[off] = flag ? -1 : ny;
+                    o.rc_flag[off] = 0;
+                }
+            }
+            r2 = -1;
         }
-        const uint32_t sel = (uint32_t)readlane((int)low, __ffsll((unsigned long long)live) - 1);
-        eps_key = lo + sel;
-        const bool in = act && low <= sel;
-        const bool sel_tie = __popcll(__ballot(in)) != r0;
-        ex = wave_max_u(max(ex, in ? ca : 0u));
-        ey = wave_max_u(max(ey, in ? cb : 0u));
-
-        // ---- sweep C: marginal counts by ballot (self included) ---------
-        int cxl = 0, cxe = 0, cyl = 0, cye = 0;
-#pragma unroll
-        for (int e = 0; e < 4 * S4; ++e) {
-            cxl += __popcll(__ballot(a[e] <= ex));
-            cxe += __popcll(__ballot(a[e] == ex));
-            cyl += __popcll(__ballot(b[e] <= ey));
-            cye += __popcll(__ballot(b[e] == ey));
-        }
-        nx = cxl - 1;
-        ny = cyl - 1;
-        flag = sel_tie || cxe > 1 || cye > 1;
-    }
-    double contrib = 0.0;
-    if (!flag) contrib = o.psi[nx] + o.psi[ny];
-    if (lane == 0) {
-        if (flag) queue_row(o, item, i, eps_key);
-        if (o.rc_nx) {
-            const size_t off = (size_t)item * N + i;
-            o.rc_nx[off] = flag ? -1 : nx;
-            o.rc_ny[off] = flag ? -1 : ny;
-            o.rc_flag[off] = 0;
+        if (!have) break;
+        // ---- row i: queue it, or issue its summary loads ----------------------
+        if (queued) {
+            if (lane == 0) {
+                queue_row(o, item, i, eps_key);
+                if (o.rc_nx) {
+                    const size_t off = (size_t)item * N + i;
+                    o.rc_nx[off] = -1;
+                    o.rc_ny[off] = -1;
+                    o.rc_flag[off] = 0;
+                }
+            }
+        } else {
+            sa = ld_lane<uint32_t>(sr.SMa + (size_t)i * kSumPitch, lane16 >> 2);
+            sb = ld_lane<uint32_t>(sr.SMb + (size_t)i * kSumPitch, lane16 >> 2);
+            r1 = i; ex1 = ex; ey1 = ey; eps1 = eps_key;
         }
     }
-    return contrib;
+    return acc + (px + py);
 }
 
 // ---------------------------------------------------------------------------
@@ -605,24 +733,37 @@ __device__ __forceinline__ long long xcd_pair(long long npairs)
 // Single-trajectory pair kernels: block = one pair.
 // ---------------------------------------------------------------------------
 template <int S4>
-__global__ __launch_bounds__(256) void ksg_pair_kernel(KsgArgs args)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void ksg_pair_kernel(
+    KsgArgs args)
 {
     __shared__ __attribute__((aligned(4 * kBins))) uint32_t lds[kPairWaves][kBins];
-    __shared__ double wsum[kPairWaves];
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = threadIdx.x / kWave;
-    const long long p = xcd_pair(args.npairs);
-    if (p >= args.npairs) return;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    // block = (4 pairs, one range of frame rows), wave w runs one pair: the
+    // quads table's entry (a 2x2 tile of atoms, so each A and each B row is
+    // read by two waves at about the same time), or pair 4q+w (consecutive
+    // pairs share their first atom).
+    const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
+    const long long item = xcd_pair(nquads * kRowSplit);
+    if (item >= nquads * kRowSplit) return;
+    const long long q = item / kRowSplit;
+    const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
+    const int part = (int)(item % kRowSplit);
+    if (p < 0 || p >= args.npairs) return;   // no block barriers below
     const int N = args.N, T = args.K + 1;
+    const int i0 = (int)((long long)N * part / kRowSplit);
+    const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
     const float* Da = args.D + (size_t)args.a1[p] * N * args.P;
     const float* Db = args.D + (size_t)args.a2[p] * N * args.P;
     const RowOut o = row_out(args);
-    double acc = 0.0;
-    for (int i = w; i < N; i += kPairWaves)
-        acc += row_single<S4>(Da, Db, args.P, N, T, i, p, lds[w], lane, o);
-    if (lane == 0) wsum[w] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) args.partial[p] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+    SortedRows sr;
+    sr.PS = sorted_pitch(N);
+    sr.SDa = args.SD + (size_t)args.a1[p] * N * sr.PS;
+    sr.SDb = args.SD + (size_t)args.a2[p] * N * sr.PS;
+    sr.SMa = args.SM + (size_t)args.a1[p] * N * kSumPitch;
+    sr.SMb = args.SM + (size_t)args.a2[p] * N * kSumPitch;
+    const double acc = rows_single<S4>(Da, Db, sr, args.P, N, T, i0, i1, 1, p, lds[w], lane, o);
+    if (lane == 0) args.partial[p * kRowSplit + part] = acc;
 }
 
 template <int S4>
@@ -639,7 +780,7 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
     const RowOut o = row_out(args);
     double acc = 0.0;
     for (int i = 0; i < N; ++i) acc += row_coop<S4>(Da, Db, args.P, N, T, i, p, sh, lane, w, o);
-    if (threadIdx.x == 0) args.partial[p] = acc;
+    if (threadIdx.x < kRowSplit) args.partial[p * kRowSplit + threadIdx.x] = threadIdx.x == 0 ? acc : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -658,9 +799,14 @@ __device__ __forceinline__ double batch_single(const TrajDesc& d, const float* D
                                                long long item, BatchLds& u, double* wsum,
                                                int lane, int w, const RowOut& o)
 {
-    double acc = 0.0;
-    for (int i = w; i < d.N; i += kPairWaves)
-        acc += row_single<S4>(Da, Db, d.P, d.N, d.K + 1, i, item, u.single[w], lane, o);
+    SortedRows sr;
+    sr.PS = sorted_pitch(d.N);
+    sr.SDa = d.SD + (Da - d.D) / d.P * sr.PS;
+    sr.SDb = d.SD + (Db - d.D) / d.P * sr.PS;
+    sr.SMa = d.SM + (Da - d.D) / d.P * kSumPitch;
+    sr.SMb = d.SM + (Db - d.D) / d.P * kSumPitch;
+    const double acc = rows_single<S4>(Da, Db, sr, d.P, d.N, d.K + 1, w, d.N, kPairWaves, item,
+                                       u.single[w], lane, o);
     if (lane == 0) wsum[w] = acc;
     __syncthreads();
     return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
@@ -873,12 +1019,14 @@ __global__ void ksg_clear_kernel(double* fixsum, long long npairs, unsigned* cou
 
 // MI = psi[N] + phi[K] - sum/N as lib/kraskov.cpp:1318.
 __global__ void ksg_finalize_kernel(FixupArgs args, long long nitems, const double* partial,
-                                    double* mi)
+                                    int nsplit, double* mi)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nitems) return;
     const TrajDesc d = args.traj[args.item_traj ? args.item_traj[p] : 0];
-    mi[p] = d.psiN_phiK - (partial[p] + args.fixsum[p]) / d.N;
+    double sum = 0.0;
+    for (int r = 0; r < nsplit; ++r) sum += partial[p * nsplit + r];
+    mi[p] = d.psiN_phiK - (sum + args.fixsum[p]) / d.N;
 }
 
 // Wave primitive self-check: out[0..63] inclusive scan of lane+1 (mod 7
@@ -902,10 +1050,13 @@ __global__ void ksg_selftest_kernel(int* out)
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, float* D,
-                             hipStream_t s)
+                             float* SD, float* SM, hipStream_t s)
 {
+    if ((SD == nullptr) != (SM == nullptr) || (SD && N > kSortedMaxFrames))
+        return hipErrorInvalidValue;
     const long long rows = (long long)natoms * N;
-    hipLaunchKernelGGL(ksg_build_rows_kernel, dim3((unsigned)rows), dim3(256), 0, s, X, N, P, D);
+    hipLaunchKernelGGL(ksg_build_rows_kernel, dim3((unsigned)rows), dim3(256), 0, s, X, N, P, D,
+                       SD, SM);
     return hipGetLastError();
 }
 
@@ -943,7 +1094,10 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4)
 {
     if (kind < 0) default_variant(a.N, kind, s4);
     if (!variant_fits(kind, s4, a.N)) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(((a.npairs + 7) / 8) * 8)), block(kWave * kPairWaves);
+    const long long nitems =
+        kind == 0 ? (a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves) * kRowSplit
+                  : a.npairs;
+    const dim3 grid((unsigned)(((nitems + 7) / 8) * 8)), block(kWave * kPairWaves);
 #define GMX_SINGLE(V) case V: hipLaunchKernelGGL(ksg_pair_kernel<V>, grid, block, 0, s, a); break;
 #define GMX_COOP(V) case V: hipLaunchKernelGGL(ksg_pair_coop_kernel<V>, grid, block, 0, s, a); break;
     if (kind == 0) {
@@ -973,10 +1127,10 @@ hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s)
 }
 
 hipError_t launch_ksg_finalize(const FixupArgs& f, long long nitems, const double* partial,
-                               double* mi, hipStream_t s)
+                               int nsplit, double* mi, hipStream_t s)
 {
     hipLaunchKernelGGL(ksg_finalize_kernel, dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0,
-                       s, f, nitems, partial, mi);
+                       s, f, nitems, partial, nsplit, mi);
     return hipGetLastError();
 }
 

@@ -106,6 +106,10 @@ struct gmxcorr_ksg_plan {
     double* dX = nullptr;
     double* dPsi = nullptr;
     float* dD = nullptr;
+    int* dQuads = nullptr;  // all-pairs run: [nquads][4] pair indices (2x2 atom tiles)
+    long long nquads = 0;
+    float* dSD = nullptr;   // sorted rows and summaries (N <= kSortedMaxFrames)
+    float* dSM = nullptr;
     int* dA1 = nullptr;
     int* dA2 = nullptr;
     long long pair_cap = 0;
@@ -136,7 +140,7 @@ int ensure_pairs(gmxcorr_ksg_plan* pl, long long npairs)
     dfree(pl->dA1); dfree(pl->dA2); dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI);
     HIPCHK(hipMalloc(&pl->dA1, npairs * sizeof(int)));
     HIPCHK(hipMalloc(&pl->dA2, npairs * sizeof(int)));
-    HIPCHK(hipMalloc(&pl->dPartial, npairs * sizeof(double)));
+    HIPCHK(hipMalloc(&pl->dPartial, npairs * gmxcorr::kRowSplit * sizeof(double)));
     HIPCHK(hipMalloc(&pl->dFixsum, npairs * sizeof(double)));
     HIPCHK(hipMalloc(&pl->dMI, npairs * sizeof(double)));
     pl->pair_cap = npairs;
@@ -159,12 +163,37 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// Blocks of four pairs for the all-pairs run (pair index of (a1, a2), a2 <
+// a1, is a1*(a1-1)/2 + a2): 2x2 atom tiles {2t, 2t+1} x {2s, 2s+1}, s < t,
+// ordered by t then s, so two waves share each A row and two each B row;
+// the pairs outside full tiles follow in runs of four that share a1.
+std::vector<int> pair_quads(int n)
+{
+    auto pid = [](int a1, int a2) { return (int)((long long)a1 * (a1 - 1) / 2 + a2); };
+    std::vector<int> q, rest;
+    for (int t = 0; 2 * t + 1 < n; ++t) {
+        for (int s = 0; s < t; ++s) {
+            q.push_back(pid(2 * t, 2 * s));
+            q.push_back(pid(2 * t, 2 * s + 1));
+            q.push_back(pid(2 * t + 1, 2 * s));
+            q.push_back(pid(2 * t + 1, 2 * s + 1));
+        }
+        rest.push_back(pid(2 * t + 1, 2 * t));
+    }
+    if (n % 2)
+        for (int a2 = 0; a2 < n - 1; ++a2) rest.push_back(pid(n - 1, a2));
+    for (size_t r = 0; r < rest.size(); r += 4)
+        for (size_t c = 0; c < 4; ++c) q.push_back(r + c < rest.size() ? rest[r + c] : -1);
+    return q;
+}
+
 // Pair kernel + FP64 recount + finalise on the plan stream.  If the recount
 // queue overflowed, the run is repeated with a queue large enough.
-int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail)
+int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
 {
     KsgArgs a{};
-    a.X = pl->dX; a.D = pl->dD; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
+    if (tiles) { a.quads = pl->dQuads; a.nquads = pl->nquads; }
+    a.X = pl->dX; a.D = pl->dD; a.SD = pl->dSD; a.SM = pl->dSM; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
     a.partial = pl->dPartial; a.fixsum = pl->dFixsum; a.mi = pl->dMI;
     a.counters = pl->dCounters;
     a.npairs = npairs; a.P = pl->P; a.N = pl->N; a.K = pl->K; a.psiN_phiK = pl->psiN_phiK;
@@ -203,7 +232,8 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail)
             continue;
         }
         HIPCHK(gmxcorr::launch_ksg_fixup(f, counters[0], pl->stream));
-        HIPCHK(gmxcorr::launch_ksg_finalize(f, npairs, pl->dPartial, pl->dMI, pl->stream));
+        HIPCHK(gmxcorr::launch_ksg_finalize(f, npairs, pl->dPartial, gmxcorr::kRowSplit,
+                                                  pl->dMI, pl->stream));
         HIPCHK(hipEventRecord(pl->ev[4], pl->stream));
         HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,
                               pl->stream));
@@ -268,7 +298,7 @@ int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** o
     if (!device_is_gfx950()) return fail(GMXCORR_ENODEV, "no gfx950 device");
     auto* pl = new gmxcorr_ksg_plan();
     pl->natoms = natoms; pl->N = nframes; pl->K = k;
-    pl->P = ((long long)nframes + 63) / 64 * 64;
+    pl->P = gmxcorr::row_pitch(nframes);
     auto bail = [&](int rc) { gmxcorr_ksg_plan_destroy(pl); return rc; };
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking)) != hipSuccess)
@@ -282,13 +312,23 @@ int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** o
         return bail(hip_fail(e, "hipMalloc(psi)"));
     if ((e = hipMalloc(&pl->dD, dbytes)) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(distance rows)"));
+    size_t sbytes = 0;
+    if (nframes <= gmxcorr::kSortedMaxFrames) {
+        const size_t rows = (size_t)natoms * nframes;
+        sbytes = rows * (gmxcorr::sorted_pitch(nframes) + gmxcorr::kSumPitch) * sizeof(float);
+        if ((e = hipMalloc(&pl->dSD, rows * gmxcorr::sorted_pitch(nframes) * sizeof(float))) !=
+            hipSuccess)
+            return bail(hip_fail(e, "hipMalloc(sorted rows)"));
+        if ((e = hipMalloc(&pl->dSM, rows * gmxcorr::kSumPitch * sizeof(float))) != hipSuccess)
+            return bail(hip_fail(e, "hipMalloc(sorted-row summaries)"));
+    }
     if ((e = hipMalloc(&pl->dCounters, 4 * sizeof(unsigned))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(counters)"));
     if ((e = hipMalloc(&pl->dTraj, sizeof(gmxcorr::TrajDesc))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(descriptor)"));
     int rc = ensure_fix(pl, 65536);
     if (rc) return bail(rc);
-    pl->st.row_bytes = (long long)dbytes;
+    pl->st.row_bytes = (long long)(dbytes + sbytes);
     pl->st.nframes = nframes; pl->st.k = k; pl->st.natoms = natoms; pl->st.pitch = (int)pl->P;
     *out = pl;
     return GMXCORR_OK;
@@ -304,7 +344,8 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
     HIPCHK(hipMemcpyAsync(pl->dPsi, psi, (pl->N + 1) * sizeof(double), hipMemcpyHostToDevice,
                           pl->stream));
     HIPCHK(hipEventRecord(pl->ev[1], pl->stream));
-    HIPCHK(gmxcorr::launch_build_rows(pl->dX, pl->natoms, pl->N, pl->P, pl->dD, pl->stream));
+    HIPCHK(gmxcorr::launch_build_rows(pl->dX, pl->natoms, pl->N, pl->P, pl->dD, pl->dSD, pl->dSM,
+                                      pl->stream));
     HIPCHK(hipEventRecord(pl->ev[5], pl->stream));
     HIPCHK(hipStreamSynchronize(pl->stream));
     pl->st.ms_load = elapsed(pl->ev[0], pl->ev[1]);
@@ -313,6 +354,8 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
     gmxcorr::TrajDesc td{};
     td.X = pl->dX;
     td.D = pl->dD;
+    td.SD = pl->dSD;
+    td.SM = pl->dSM;
     td.psi = pl->dPsi;
     td.P = pl->P;
     td.N = pl->N;
@@ -339,9 +382,15 @@ int gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* pl)
             for (int a2 = 0; a2 < a1; ++a2, ++idx) { h1[idx] = a1; h2[idx] = a2; }
         HIPCHK(hipMemcpy(pl->dA1, h1.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(pl->dA2, h2.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
+        if (!pl->dQuads && npairs < (1LL << 31)) {
+            const std::vector<int> q = pair_quads((int)n);
+            HIPCHK(hipMalloc(&pl->dQuads, q.size() * sizeof(int)));
+            HIPCHK(hipMemcpy(pl->dQuads, q.data(), q.size() * sizeof(int), hipMemcpyHostToDevice));
+            pl->nquads = (long long)q.size() / 4;
+        }
         pl->all_pairs = true;
     }
-    return run_loaded(pl, npairs, false);
+    return run_loaded(pl, npairs, false, pl->dQuads != nullptr);
 }
 
 int gmxcorr_ksg_plan_run_pairs(gmxcorr_ksg_plan* pl, long long npairs, const int* a1,
@@ -366,7 +415,7 @@ int gmxcorr_ksg_plan_run_pairs(gmxcorr_ksg_plan* pl, long long npairs, const int
         HIPCHK(hipMalloc(&pl->dRcFlag, rows));
         pl->rc_cap = rows;
     }
-    rc = run_loaded(pl, npairs, detail);
+    rc = run_loaded(pl, npairs, detail, false);
     if (rc) return rc;
     if (nx) HIPCHK(hipMemcpy(nx, pl->dRcNx, rows * sizeof(int), hipMemcpyDeviceToHost));
     if (ny) HIPCHK(hipMemcpy(ny, pl->dRcNy, rows * sizeof(int), hipMemcpyDeviceToHost));
@@ -426,7 +475,7 @@ void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* pl)
 {
     if (!pl) return;
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dA1); dfree(pl->dA2);
+    dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dSD); dfree(pl->dSM); dfree(pl->dQuads); dfree(pl->dA1); dfree(pl->dA2);
     dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix);
     dfree(pl->dCounters); dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
     dfree(pl->dTraj);

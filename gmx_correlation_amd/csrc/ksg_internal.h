@@ -26,10 +26,12 @@ struct FixEntry {
 struct KsgArgs {
     const double* X;      // [3n][N] normalised coordinates (FP64)
     const float*  D;      // [n][N][P] per-atom max-norm distance rows (FP32)
+    const float*  SD;     // [n][N][sorted_pitch(N)] sorted rows (single-wave path)
+    const float*  SM;     // [n][N][kSumPitch] their summaries
     const double* psi;    // [N+1]
     const int*    a1;     // [npairs]
     const int*    a2;     // [npairs]
-    double*       partial;   // [npairs] sum of psi[nx]+psi[ny] over unflagged rows
+    double*       partial;   // [npairs][kRowSplit] sums of psi[nx]+psi[ny] over unflagged rows
     double*       fixsum;    // [npairs] same over FP64-recounted rows
     double*       mi;        // [npairs]
     FixEntry*     fixq;      // [fixcap]
@@ -39,7 +41,9 @@ struct KsgArgs {
     int*          rc_ny;     // optional [npairs][N]
     unsigned char* rc_flag;  // optional [npairs][N]
     long long     npairs;
-    long long     P;         // row pitch in floats (multiple of 64)
+    const int*    quads;     // optional [nquads][4] pair indices per block (-1: none);
+    long long     nquads;    // nullptr: pairs 4q..4q+3
+    long long     P;         // row pitch in floats, row_pitch(N)
     int           N;
     int           K;
     double        psiN_phiK; // psi[N] + (psi[K] - 1/K)
@@ -50,6 +54,8 @@ struct KsgArgs {
 struct TrajDesc {
     const double* X;     // [3n][N] normalised rows (FP64)
     const float* D;      // [n][N][P] distance rows (FP32)
+    const float* SD;     // sorted rows and summaries (kind 0 only)
+    const float* SM;
     const double* psi;   // [N+1]
     long long P;
     int N;
@@ -91,6 +97,24 @@ struct KsgBatchArgs {
 // Largest N handled by the pair kernels (cooperative kernel, 4 x 2048 frames).
 constexpr int kMaxFrames = 8192;
 
+// The single-trajectory pair kernels write kRowSplit partial sums per pair
+// (frame rows split into that many ranges); partial holds npairs*kRowSplit.
+constexpr int kRowSplit = 4;
+
+// Distance-row pitch in floats: a multiple of 256 frames, so the single-wave
+// kernels read whole 64-lane chunks of 4 frames without a per-lane bound.
+__host__ __device__ inline long long row_pitch(int N) { return ((long long)N + 255) / 256 * 256; }
+
+// Sorted distance rows (single-wave path, N <= kSortedMaxFrames): row pitch
+// sorted_pitch(N) floats (64 front pad, N values, >= 64 padding values) and a
+// summary of kSumPitch floats per row (every 64th sorted value).
+constexpr int kSortedMaxFrames = 2048;
+constexpr int kSumPitch = 64;
+__host__ __device__ inline long long sorted_pitch(int N)
+{
+    return 64 + ((long long)N + 64 + 63) / 64 * 64;
+}
+
 // Read-only view of a loaded plan's device data (for the surrogate features).
 struct PlanView {
     const double* X;   // [3n][N] normalised rows
@@ -100,8 +124,10 @@ struct PlanView {
     hipStream_t stream;
 };
 
+// SD/SM: sorted rows and their summaries for the single-wave path, or both
+// nullptr (required for N > kSortedMaxFrames).
 hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, float* D,
-                             hipStream_t s);
+                             float* SD, float* SM, hipStream_t s);
 hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
                             unsigned long long* queue, hipStream_t s);
 // kind 0: one wave per frame row, N <= 256*s4 (s4 in 1,2,3,4,5,6,8);
@@ -111,7 +137,7 @@ void default_variant(int N, int& kind, int& s4);
 hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind = -1, int s4 = 0);
 hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s);
 hipError_t launch_ksg_finalize(const FixupArgs& f, long long nitems, const double* partial,
-                               double* mi, hipStream_t s);
+                               int nsplit, double* mi, hipStream_t s);
 // Code paths compiled into the batched kernel.
 bool batch_variant_ok(int kind, int s4, int N);
 hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s);

@@ -147,6 +147,10 @@ int  gmxcorr_ksg_batch_create(gmxcorr_ksg_batch** out);
 /* Returns the trajectory's index (>= 0) or a negative status. */
 int  gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const double* psi,
                            int natoms, int nframes, int k, int kind, int s4);
+/* Device bytes that trajectory t's distance rows (and, on the one-wave-per-
+ * row path, sorted rows) take in a memory wave; max_row_bytes of
+ * gmxcorr_ksg_batch_run must be at least the largest of these. */
+long long gmxcorr_ksg_batch_traj_bytes(const gmxcorr_ksg_batch* b, int t);
 int  gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nblocks);
 /* MI of trajectory t's pairs (a1 = 1.., a2 < a1); ms_done: batch-relative
  * completion time of its wave; wave: its wave index (both optional). */

@@ -34,8 +34,7 @@ def test_batch_matches_single_plans(gpu, trajs, oracle_lib):
     for rows, psi, k in trajs:
         b.add(rows, psi, k)
     # a small row budget forces several memory waves
-    budget = max(r.shape[0] // 3 * r.shape[1] * ((r.shape[1] + 63) // 64 * 64) * 4
-                 for r, _, _ in trajs)
+    budget = max(b.traj_bytes(t) for t in range(len(trajs)))
     st = b.run(max_row_bytes=budget, nblocks=64)
     assert st["nwaves"] >= 2
     for t, (rows, psi, k) in enumerate(trajs):

@@ -15,7 +15,7 @@ def _declared_symbols():
     boundary symbols of include/kraskov_gpu.h."""
     names = set()
     src = open(os.path.join(ROOT, "include", "gmxcorr.h")).read()
-    for m in re.finditer(r"^\s*(?:const\s+)?[\w\*]+\s+\**(gmxcorr_\w+)\s*\(", src, re.M):
+    for m in re.finditer(r"^\s*(?:const\s+)?(?:[\w\*]+\s+)+\**(gmxcorr_\w+)\s*\(", src, re.M):
         names.add(m.group(1))
     return names
 
