@@ -127,7 +127,7 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_ksg_plan_destroy.argtypes = [ctypes.c_void_p]
     L.gmxcorr_ksg_plan_destroy.restype = None
     L.gmxcorr_ksg_plan_set_variant.argtypes = [ctypes.c_void_p, _i, _i]
-    L.gmxcorr_ksg_variant_fits.argtypes = [_i, _i, _i]
+    L.gmxcorr_ksg_variant_fits.argtypes = [_i, _i, _i, _i]
     L.gmxcorr_gauss_matrix.argtypes = [_dp, _i, _i, _dp, _dp, ctypes.POINTER(DenseStats)]
     L.gmxcorr_gcmi_matrix.argtypes = [_dp, _i, _i, _dp, _dp, _dp, ctypes.POINTER(DenseStats)]
     _fp = ctypes.POINTER(ctypes.c_float)

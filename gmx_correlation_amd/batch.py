@@ -19,13 +19,20 @@ from . import _lib
 from .ksg import KsgPlan
 
 # Code paths compiled into the persistent batch kernel: (kind, s4) with
-# kind 0 = one wave per frame row (N <= 256*s4), kind 1 = four waves per row
-# (N <= 1024*s4).
-BATCH_PATHS = [(0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (1, 4), (1, 6), (1, 8)]
+# kind 0 = one wave per frame row streaming both distance rows (N <= 256*s4),
+# kind 1 = four waves per row (N <= 1024*s4), kind 2 = one wave per row over
+# the 512 nearest frames of the sparser atom (N <= 2048, k + 1 <= 256).
+BATCH_PATHS = [(2, 1), (0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (1, 4), (1, 6), (1, 8)]
+PREFIX_MAX_FRAMES = 2048
+PREFIX_MAX_T = 256
 
 
-def fitting_paths(nframes: int) -> list[tuple[int, int]]:
-    return [(k, s) for k, s in BATCH_PATHS if nframes <= (256 if k == 0 else 1024) * s]
+def fitting_paths(nframes: int, k: int) -> list[tuple[int, int]]:
+    def fits(kind: int, s4: int) -> bool:
+        if kind == 2:
+            return nframes <= PREFIX_MAX_FRAMES and k + 1 <= PREFIX_MAX_T
+        return nframes <= (256 if kind == 0 else 1024) * s4
+    return [(kd, s) for kd, s in BATCH_PATHS if fits(kd, s)]
 
 
 def tune_variant(rows: np.ndarray, psi: np.ndarray, k: int, natoms_probe: int = 6,
@@ -35,7 +42,7 @@ def tune_variant(rows: np.ndarray, psi: np.ndarray, k: int, natoms_probe: int = 
     events) and return the fastest (kind, s4) with all timings in ms."""
     N = rows.shape[1]
     m = min(natoms_probe, rows.shape[0] // 3)
-    cands = paths if paths is not None else fitting_paths(N)
+    cands = paths if paths is not None else fitting_paths(N, k)
     # keep only the two smallest of each kind: larger register tiles only add work
     by_kind: dict[int, list[int]] = {}
     for kind, s4 in sorted(cands):

@@ -39,15 +39,17 @@ struct HostTraj {
     double psiN_phiK = 0.0;
     int wave = -1;
     double ms_done = 0.0;       // batch-relative completion time of its wave
-    bool sorted() const { return kind == 0; }
+    bool sorted() const { return gmxcorr::variant_sorted(kind); }
     long long d_bytes() const { return (long long)natoms * N * P * 4; }
     long long sd_bytes() const { return sorted() ? (long long)natoms * N * gmxcorr::sorted_pitch(N) * 4 : 0; }
     long long sm_bytes() const { return sorted() ? (long long)natoms * N * gmxcorr::kSumPitch * 4 : 0; }
-    // device bytes of the distance rows, sorted rows and summaries (256-aligned parts)
+    long long si_bytes() const { return sorted() ? (long long)natoms * N * gmxcorr::kPrefix * 2 : 0; }
+    // device bytes of the distance rows, sorted rows, summaries and prefix
+    // indices (256-aligned parts)
     long long row_bytes() const
     {
         auto al = [](long long v) { return (v + 255) / 256 * 256; };
-        return al(d_bytes()) + al(sd_bytes()) + al(sm_bytes());
+        return al(d_bytes()) + al(sd_bytes()) + al(sm_bytes()) + al(si_bytes());
     }
     long long npairs() const { return (long long)natoms * (natoms - 1) / 2; }
 };
@@ -86,17 +88,20 @@ int gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const doubl
     if (nframes > gmxcorr::kMaxFrames)
         return gmxcorr::set_error(GMXCORR_EUNSUP, "batch_add: nframes above the kernel limit");
     if (kind < 0) {
-        // smallest compiled batch path that covers the frame count
+        // the sorted-prefix path where it applies, else the smallest compiled
+        // batch path that covers the frame count
         static const int single[] = {2, 4, 6, 8};
         static const int coop[] = {3, 4, 6, 8};
         kind = -1;
-        for (int v : single)
-            if (nframes <= 256 * v) { kind = 0; s4 = v; break; }
+        if (gmxcorr::batch_variant_ok(2, 1, nframes, k)) { kind = 2; s4 = 1; }
+        if (kind < 0)
+            for (int v : single)
+                if (nframes <= 256 * v) { kind = 0; s4 = v; break; }
         if (kind < 0)
             for (int v : coop)
                 if (nframes <= 1024 * v) { kind = 1; s4 = v; break; }
     }
-    if (!gmxcorr::batch_variant_ok(kind, s4, nframes))
+    if (!gmxcorr::batch_variant_ok(kind, s4, nframes, k))
         return gmxcorr::set_error(GMXCORR_EUNSUP, "batch_add: variant not compiled into the batch kernel");
     const size_t nx = (size_t)3 * natoms * nframes;
     if (int rc = gmxcorr::check_rows(xrows, 3LL * natoms, nframes, "batch_add")) return rc;
@@ -213,6 +218,10 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                 float* dSM = h.sorted() ? reinterpret_cast<float*>(dArena + doff + align(h.d_bytes()) +
                                                                   align(h.sd_bytes()))
                                         : nullptr;
+                uint16_t* dSI =
+                    h.sorted() ? reinterpret_cast<uint16_t*>(dArena + doff + align(h.d_bytes()) +
+                                                             align(h.sd_bytes()) + align(h.sm_bytes()))
+                               : nullptr;
                 doff += align(h.row_bytes());
                 dX[q] = reinterpret_cast<double*>(xArena + xoff);
                 xoff += align((long long)h.rows.size() * 8);
@@ -222,8 +231,9 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                                       hipMemcpyHostToDevice, s));
                 HIPCHK(hipMemcpyAsync(dPsi[q], h.psi.data(), h.psi.size() * sizeof(double),
                                       hipMemcpyHostToDevice, s));
-                HIPCHK(gmxcorr::launch_build_rows(dX[q], h.natoms, h.N, h.P, dD[q], dSD, dSM, s));
-                desc[q] = gmxcorr::TrajDesc{dX[q], dD[q], dSD, dSM, dPsi[q], h.P, h.N, h.K,
+                HIPCHK(gmxcorr::launch_build_rows(dX[q], h.natoms, h.N, h.P, dD[q], dSD, dSM, dSI,
+                                                  s));
+                desc[q] = gmxcorr::TrajDesc{dX[q], dD[q], dSD, dSM, dSI, dPsi[q], h.P, h.N, h.K,
                                             h.psiN_phiK, h.kind, h.s4};
             }
             // items: trajectories by decreasing frame count, pairs a1 > a2 within each
@@ -254,11 +264,11 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             HIPCHK(hipMalloc(&dMI, nitems * sizeof(double)));
             HIPCHK(hipMalloc(&dCnt, 4 * sizeof(unsigned)));
             HIPCHK(hipMalloc(&dQueue, sizeof(unsigned long long)));
-            // recount queue for 0.2% of the wave's rows (0.04-0.07% measured)
+            // recount queue for 1% of the wave's rows (0.04-0.2% measured)
             long long rows_w = 0;
             for (size_t q = 0; q < ts.size(); ++q)
                 rows_w += b->traj[ts[q]].npairs() * (long long)b->traj[ts[q]].N;
-            unsigned fixcap = (unsigned)std::min<long long>(std::max<long long>(1 << 18, rows_w / 500),
+            unsigned fixcap = (unsigned)std::min<long long>(std::max<long long>(1 << 18, rows_w / 100),
                                                             64LL << 20);
             HIPCHK(hipMalloc(&dFix, (size_t)fixcap * sizeof(gmxcorr::FixEntry)));
             HIPCHK(hipMemcpyAsync(dDesc, desc.data(), desc.size() * sizeof(gmxcorr::TrajDesc),

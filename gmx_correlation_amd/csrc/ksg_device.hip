@@ -276,18 +276,22 @@ __device__ __forceinline__ void clear_hist(uint32_t* hist, int lane)
 // computed in FP64 exactly as the reference does (fabs of the difference,
 // lib/kraskov.cpp:131-132, 280-281), then rounded once.  j in [N, P) = 3072.
 // One block per row (a, i).  With SD != nullptr (N <= kSortedMaxFrames) the
-// row is also sorted in LDS (bitonic) and written as
+// row's (value, frame) pairs are also sorted in LDS (bitonic, ties in
+// ascending frame order) and written as
 //   SD[a][i][0..64)        0 (front pad)
 //   SD[a][i][64 + t]       t-th smallest value, t < N; 3072 up to the pitch
 //   SM[a][i][m]            SD value t = 64m + 63 (3072 past N), m < 64
-// so that #{j : D[a][i][j] <= x} is found with two 64-lane loads.
+//   SI[a][i][t]            frame of the t-th smallest value, t < kPrefix (0 past N)
+// so that #{j : D[a][i][j] <= x} is found with two 64-lane loads, and the
+// kPrefix nearest frames of i are known in order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __restrict__ X, int N,
                                                              long long P, float* __restrict__ D,
                                                              float* __restrict__ SD,
-                                                             float* __restrict__ SM)
+                                                             float* __restrict__ SM,
+                                                             uint16_t* __restrict__ SI)
 {
-    __shared__ uint32_t sv[kSortedMaxFrames];
+    __shared__ uint64_t sv[kSortedMaxFrames];   // (value bits << 32) | frame
     const long long row = blockIdx.x;
     const int a = (int)(row / N), i = (int)(row % N);
     const double* x0 = X + (size_t)(3 * a) * N;
@@ -314,20 +318,22 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
         if (SD) {
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                if (4 * c + e < kSortedMaxFrames) sv[4 * c + e] = __float_as_uint(v[e]);
+                if (4 * c + e < kSortedMaxFrames)
+                    sv[4 * c + e] = ((uint64_t)__float_as_uint(v[e]) << 32) | (uint32_t)(4 * c + e);
         }
     }
     if (!SD) return;
     int M = 64;
     while (M < N) M <<= 1;
-    for (int t = nc * 4 + threadIdx.x; t < M; t += blockDim.x) sv[t] = kPadBits;
+    for (int t = nc * 4 + threadIdx.x; t < M; t += blockDim.x)
+        sv[t] = ((uint64_t)kPadBits << 32) | (uint32_t)t;
     __syncthreads();
-    // bitonic sort of sv[0..M), ascending as unsigned (non-negative floats)
+    // bitonic sort of sv[0..M), ascending (non-negative float bits, then frame)
     for (int k = 2; k <= M; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
                 const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
-                const uint32_t x = sv[lo], y = sv[hi];
+                const uint64_t x = sv[lo], y = sv[hi];
                 if ((x > y) == ((lo & k) == 0)) {
                     sv[lo] = y;
                     sv[hi] = x;
@@ -339,12 +345,15 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
     uint32_t* srow = reinterpret_cast<uint32_t*>(SD) + row * ps;
     for (int t = threadIdx.x; t < ps; t += blockDim.x) {
         const int q = t - 64;
-        srow[t] = q < 0 ? 0u : q < N ? sv[q] : kPadBits;
+        srow[t] = q < 0 ? 0u : q < N ? (uint32_t)(sv[q] >> 32) : kPadBits;
     }
     if (threadIdx.x < kSumPitch) {
         const int q = 64 * threadIdx.x + 63;
-        reinterpret_cast<uint32_t*>(SM)[row * kSumPitch + threadIdx.x] = q < N ? sv[q] : kPadBits;
+        reinterpret_cast<uint32_t*>(SM)[row * kSumPitch + threadIdx.x] =
+            q < N ? (uint32_t)(sv[q] >> 32) : kPadBits;
     }
+    for (int t = threadIdx.x; t < kPrefix; t += blockDim.x)
+        SI[row * kPrefix + t] = t < N ? (uint16_t)(sv[t] & 0xffffu) : (uint16_t)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -547,6 +556,255 @@ __device__ __forceinline__ double rows_single(const float* Da, const float* Db, 
             sb = ld_lane<uint32_t>(sr.SMb + (size_t)i * kSumPitch, lane16 >> 2);
             r1 = i; ex1 = ex; ey1 = ey; eps1 = eps_key;
         }
+    }
+    return acc + (px + py);
+}
+
+// ---------------------------------------------------------------------------
+// Sorted-prefix rows (kind 2): frame rows i0, i0+step, ... < i1 of one pair,
+// one wave, at most kPrefix rows.  For row i, let P be the atom whose sorted
+// row is larger at position 255 (the sparser neighbourhood) and Q the other.
+// Lane l holds the sorted P values at positions 8l..8l+7 and gathers the Q
+// distances of the same frames from Q's row (staged in LDS), so the wave
+// sees the joint keys J = max(dP, dQ) of the kPrefix nearest frames in P.
+// Every frame outside that prefix has dP >= H = SP[kPrefix - 1], hence
+// J >= H: if the (k+1)-th smallest key of the prefix is below H it is the
+// row's, and the prefix holds every frame at or below it.  Otherwise the
+// row is handed on unresolved (FP64 recount).
+//   histogram  256 bins over [L, H), L = SP[k] (the (k+1)-th smallest dP, a
+//              lower bound of the key), bin 0 below L, bin 255 above H;
+//   sweep B    frames below the target bin are in S; its frames go to LDS;
+//   select     ballot radix over the bits below the bin.
+// nP = #{dP <= epsP} is counted in registers (epsP < H), nQ from Q's sorted
+// row (summary, then one block, consumed one row later).  The next row's
+// prefix, Q summary and whole Q row are loaded while a row is processed.
+// ---------------------------------------------------------------------------
+constexpr int kPBins = 256;
+
+struct PrefixRows {
+    const uint32_t* Da;    // distance rows of atom a (row i at i * P)
+    const uint32_t* Db;
+    const uint32_t* SDa;   // sorted rows (row i at i * PS, value t at 64 + t)
+    const uint32_t* SDb;
+    const uint32_t* SMa;   // summaries (row i at i * kSumPitch)
+    const uint32_t* SMb;
+    const uint16_t* SIa;   // prefix frame indices (row i at i * kPrefix)
+    const uint16_t* SIb;
+    long long P, PS;
+};
+
+// Per-wave LDS of the sorted-prefix path: histogram / candidate slots, then
+// the Q distance row of the current frame.
+struct PrefixLds {
+    uint32_t hist[kPBins];
+    uint32_t qrow[kSortedMaxFrames];
+};
+
+__device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T, int i0, int i1,
+                                              int step, long long item, PrefixLds& lds, int lane,
+                                              const RowOut& o)
+{
+    // R <= kPrefix holds for every caller (N <= kSortedMaxFrames rows split
+    // in at least kRowSplit = 4 parts); the side bits cover kPrefix rows.
+    const int R = min((i1 - i0 + step - 1) / step, kPrefix);
+    const int nblk = (int)(pr.P / 256);   // 64-lane chunks of 4 frames in a row
+    uint32_t* hist = lds.hist;
+    const uint32_t hbase = lds_addr(hist);
+    uint32_t lane4 = (uint32_t)lane * 4u;
+    // side bits: bit u of lane l is set when row t = l + 64u takes atom a as P
+    uint32_t sidebits = 0u;
+#pragma unroll
+    for (int u = 0; u < kPrefix / kWave; ++u) {
+        const int t = lane + kWave * u;
+        if (t < R) {
+            const long long row = i0 + (long long)t * step;
+            const uint32_t sa = pr.SMa[row * kSumPitch + 3];
+            const uint32_t sb = pr.SMb[row * kSumPitch + 3];
+            sidebits |= (sa >= sb ? 1u : 0u) << u;
+        }
+    }
+    auto side_of = [&](int t) -> bool {
+        return ((uint32_t)readlane((int)sidebits, t & (kWave - 1)) >> (t >> 6)) & 1u;
+    };
+    // the next row: P values and frame indices, SP[T-1], Q summary, Q row
+    uint4 nv0 = make_uint4(0u, 0u, 0u, 0u), nv1 = nv0, ni = nv0;
+    uint4 nq[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) nq[m] = nv0;
+    uint32_t nsm = 0u, nL = 0u;
+    auto issue = [&](int t) {
+        const long long row = i0 + (long long)t * step;
+        const bool sa = side_of(t);
+        const uint32_t* base = (sa ? pr.SDa : pr.SDb) + row * pr.PS + 64;
+        nv0 = ld_lane<uint4>(base, lane4 * 8u);
+        nv1 = ld_lane<uint4>(base, lane4 * 8u + 16u);
+        ni = ld_lane<uint4>((sa ? pr.SIa : pr.SIb) + row * kPrefix, lane4 * 4u);
+        nL = ld_lane<uint32_t>(base, (uint32_t)(T - 1) * 4u);
+        nsm = ld_lane<uint32_t>((sa ? pr.SMb : pr.SMa) + row * kSumPitch, lane4);
+        const uint32_t* rq = (sa ? pr.Db : pr.Da) + row * pr.P;
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            if (m < nblk) nq[m] = ld_lane<uint4>(rq, lane4 * 4u + m * 1024u);
+    };
+    double acc = 0.0, px = 0.0, py = 0.0;
+    // the previous row, waiting for its Q block
+    long long prow = -1;
+    bool pside = false, ptie = false;
+    int pnp = 0, pc1 = 0;
+    uint32_t pexq = 0u, peps = 0u;
+    uint2 pblk = make_uint2(0u, 0u);
+    if (R > 0) issue(0);
+    for (int t = 0;; ++t) {
+        const bool have = t < R;
+        asm volatile("" : "+v"(lane4));
+        uint32_t v[8], q[8];
+        uint32_t sm = 0u, L = 0u;
+        bool sideA = false;
+        long long row = 0;
+        if (have) {
+            row = i0 + (long long)t * step;
+            sideA = side_of(t);
+            v[0] = nv0.x; v[1] = nv0.y; v[2] = nv0.z; v[3] = nv0.w;
+            v[4] = nv1.x; v[5] = nv1.y; v[6] = nv1.z; v[7] = nv1.w;
+            const uint32_t idx[4] = {ni.x, ni.y, ni.z, ni.w};
+            sm = nsm;
+            L = (uint32_t)__builtin_amdgcn_readfirstlane((int)nL);
+            // stage the Q row (frames 256m + 4l + c at lane l, chunk m)
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+                if (m < nblk) reinterpret_cast<uint4*>(lds.qrow)[m * kWave + lane] = nq[m];
+            wave_sync();
+#pragma unroll
+            for (int e = 0; e < 8; ++e) q[e] = lds.qrow[(idx[e >> 1] >> (16 * (e & 1))) & 0xffffu];
+            if (t + 1 < R) issue(t + 1);
+        }
+        // ---- previous row: Q count from its block, psi loads ---------------
+        acc += px + py;
+        px = py = 0.0;
+        if (prow >= 0) {
+            bool tq;
+            const int nq_ = sorted_count(pblk, pc1, pexq, lane, tq) - 1;
+            const int np = pnp - 1;
+            const bool flag = tq || ptie;
+            const int nx = pside ? np : nq_, ny = pside ? nq_ : np;
+            if (!flag) {
+                px = o.psi[nx];
+                py = o.psi[ny];
+            }
+            if (lane == 0) {
+                if (flag) queue_row(o, item, (int)prow, peps);
+                if (o.rc_nx) {
+                    const size_t off = (size_t)item * N + prow;
+                    o.rc_nx[off] = flag ? -1 : nx;
+                    o.rc_ny[off] = flag ? -1 : ny;
+                    o.rc_flag[off] = 0;
+                }
+            }
+            prow = -1;
+        }
+        if (!have) break;
+        // ---- histogram of the prefix's joint keys over [L, H) --------------
+        const uint32_t H = min((uint32_t)readlane((int)v[7], kWave - 1), kPadBits);
+        // smallest sh with ((H - L - 1) >> sh) <= kPBins - 3
+        const uint32_t span1 = H > L ? H - L - 1u : 0u;
+        int sh = max(0, 24 - (int)__builtin_clz(span1 | 1u));
+        if ((span1 >> sh) > (uint32_t)(kPBins - 3)) ++sh;
+        const uint32_t Lm = L - min(L, 1u << sh);
+        reinterpret_cast<uint4*>(hist)[lane] = make_uint4(0u, 0u, 0u, 0u);
+        wave_sync();
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t key = max(v[e], q[e]);
+            const uint32_t bin = min(__builtin_elementwise_sub_sat(key, Lm) >> sh,
+                                     (uint32_t)(kPBins - 1));
+            __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+        wave_sync();
+        int b1, r0, pop;
+        find_bin<kPBins / kWave>(hist, lane, T, b1, r0, pop);
+        wave_sync();
+        bool queued = b1 == 0 || b1 == kPBins - 1 || pop > kWave;
+        uint32_t eps_key = kUnknownEps, ex = 0u, ey = 0u;
+        if (!queued) {
+            // ---- sweep B: sure members' marginal maxima; the target bin's
+            //      frames appended, compacted, to LDS
+            const uint32_t lo = Lm + ((uint32_t)b1 << sh);
+            const uint32_t width = 1u << sh;
+            uint2* cand = reinterpret_cast<uint2*>(hist);
+            int nstored = 0;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t key = max(v[e], q[e]);
+                const bool sure = key < lo;
+                ex = max(ex, sure ? v[e] : 0u);
+                ey = max(ey, sure ? q[e] : 0u);
+                const bool inbin = key - lo < width;
+                const uint64_t cm = __ballot(inbin);
+                if (cm) {
+                    if (inbin) cand[nstored + mbcnt(cm)] = make_uint2(v[e], q[e]);
+                    nstored += __popcll(cm);
+                }
+            }
+            wave_sync();
+            // ---- select: r0-th smallest of the candidates' offsets in the bin
+            const bool act = lane < pop;
+            uint32_t ca = 0u, cb = 0u, low = 0xffffffffu;
+            if (act) {
+                const uint2 c = cand[lane];
+                ca = c.x;
+                cb = c.y;
+                low = max(ca, cb) - lo;   // < width
+            }
+            uint64_t live = __ballot(act);
+            uint32_t pre = 0u;
+            int r = r0;
+            for (int bit = sh - 1; bit >= 0 && __popcll(live) > 1; --bit) {
+                const uint64_t z = __ballot((low >> bit) == (pre << 1));
+                const int c0 = __popcll(z);
+                if (r > c0) {
+                    r -= c0;
+                    pre = (pre << 1) | 1u;
+                    live &= ~z;
+                } else {
+                    pre <<= 1;
+                    live = z;
+                }
+            }
+            const uint32_t sel = (uint32_t)readlane((int)low, __ffsll((unsigned long long)live) - 1);
+            eps_key = lo + sel;
+            const bool in = act && low <= sel;
+            queued = __popcll(__ballot(in)) != r0 || eps_key >= H;
+            ex = wave_max_u(max(ex, in ? ca : 0u));
+            ey = wave_max_u(max(ey, in ? cb : 0u));
+            wave_sync();
+        }
+        if (queued) {
+            if (lane == 0) {
+                queue_row(o, item, (int)row, eps_key < H ? eps_key : kUnknownEps);
+                if (o.rc_nx) {
+                    const size_t off = (size_t)item * N + row;
+                    o.rc_nx[off] = -1;
+                    o.rc_ny[off] = -1;
+                    o.rc_flag[off] = 0;
+                }
+            }
+            continue;
+        }
+        // ---- nP from the prefix in registers (epsP < H); Q block load -------
+        int c = 0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) c += (v[e] <= ex ? 1 : 0) + (v[e] == ex ? 0x10000 : 0);
+        c = wave_sum(c);
+        pnp = c & 0xffff;
+        ptie = (c >> 16) > 1;
+        pc1 = __popcll(__ballot(sm <= ey));
+        const uint32_t* sq = (sideA ? pr.SDb : pr.SDa) + row * pr.PS + 64 * pc1;
+        pblk = ld_lane<uint2>(sq, lane4 * 2u);
+        prow = row;
+        pside = sideA;
+        pexq = ey;
+        peps = eps_key;
     }
     return acc + (px + py);
 }
@@ -766,6 +1024,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
 }
 
+__device__ __forceinline__ PrefixRows prefix_rows(const float* D, const float* SD, const float* SM,
+                                                  const uint16_t* SI, long long P, int N, int a1,
+                                                  int a2)
+{
+    PrefixRows pr;
+    pr.P = P;
+    pr.PS = sorted_pitch(N);
+    pr.Da = reinterpret_cast<const uint32_t*>(D + (size_t)a1 * N * P);
+    pr.Db = reinterpret_cast<const uint32_t*>(D + (size_t)a2 * N * P);
+    pr.SDa = reinterpret_cast<const uint32_t*>(SD + (size_t)a1 * N * pr.PS);
+    pr.SDb = reinterpret_cast<const uint32_t*>(SD + (size_t)a2 * N * pr.PS);
+    pr.SMa = reinterpret_cast<const uint32_t*>(SM + (size_t)a1 * N * kSumPitch);
+    pr.SMb = reinterpret_cast<const uint32_t*>(SM + (size_t)a2 * N * kSumPitch);
+    pr.SIa = SI + (size_t)a1 * N * kPrefix;
+    pr.SIb = SI + (size_t)a2 * N * kPrefix;
+    return pr;
+}
+
+// Sorted-prefix pair kernel (kind 2): block = (4 pairs, one range of frame
+// rows) as ksg_pair_kernel, each wave one pair.
+__global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
+{
+    __shared__ __attribute__((aligned(4 * kPBins))) PrefixLds lds[kPairWaves];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
+    const long long item = xcd_pair(nquads * kRowSplit);
+    if (item >= nquads * kRowSplit) return;
+    const long long q = item / kRowSplit;
+    const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
+    const int part = (int)(item % kRowSplit);
+    if (p < 0 || p >= args.npairs) return;   // no block barriers below
+    const int N = args.N, T = args.K + 1;
+    const int i0 = (int)((long long)N * part / kRowSplit);
+    const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
+    const PrefixRows pr = prefix_rows(args.D, args.SD, args.SM, args.SI, args.P, N, args.a1[p],
+                                      args.a2[p]);
+    const double acc = rows_prefix(pr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
+    if (lane == 0) args.partial[p * kRowSplit + part] = acc;
+}
+
 template <int S4>
 __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
 {
@@ -791,6 +1090,7 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
 // ---------------------------------------------------------------------------
 union __attribute__((aligned(4 * kBins))) BatchLds {
     uint32_t single[kPairWaves][kBins];
+    PrefixLds prefix[kPairWaves];
     CoopLds coop;
 };
 
@@ -807,6 +1107,17 @@ __device__ __forceinline__ double batch_single(const TrajDesc& d, const float* D
     sr.SMb = d.SM + (Db - d.D) / d.P * kSumPitch;
     const double acc = rows_single<S4>(Da, Db, sr, d.P, d.N, d.K + 1, w, d.N, kPairWaves, item,
                                        u.single[w], lane, o);
+    if (lane == 0) wsum[w] = acc;
+    __syncthreads();
+    return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+
+__device__ __forceinline__ double batch_prefix(const TrajDesc& d, int a1, int a2, long long item,
+                                               BatchLds& u, double* wsum, int lane, int w,
+                                               const RowOut& o)
+{
+    const PrefixRows pr = prefix_rows(d.D, d.SD, d.SM, d.SI, d.P, d.N, a1, a2);
+    const double acc = rows_prefix(pr, d.N, d.K + 1, w, d.N, kPairWaves, item, u.prefix[w], lane, o);
     if (lane == 0) wsum[w] = acc;
     __syncthreads();
     return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
@@ -848,7 +1159,9 @@ __global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
         const float* Da = d.D + (size_t)b.a1[item] * d.N * d.P;
         const float* Db = d.D + (size_t)b.a2[item] * d.N * d.P;
         double acc = 0.0;
-        if (d.kind == 0) {
+        if (d.kind == 2) {
+            acc = batch_prefix(d, b.a1[item], b.a2[item], item, u, wsum, lane, w, o);
+        } else if (d.kind == 0) {
             switch (d.s4) {
                 case 2: acc = batch_single<2>(d, Da, Db, item, u, wsum, lane, w, o); break;
                 case 4: acc = batch_single<4>(d, Da, Db, item, u, wsum, lane, w, o); break;
@@ -1050,13 +1363,14 @@ __global__ void ksg_selftest_kernel(int* out)
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, float* D,
-                             float* SD, float* SM, hipStream_t s)
+                             float* SD, float* SM, uint16_t* SI, hipStream_t s)
 {
-    if ((SD == nullptr) != (SM == nullptr) || (SD && N > kSortedMaxFrames))
+    if ((SD == nullptr) != (SM == nullptr) || (SD == nullptr) != (SI == nullptr) ||
+        (SD && N > kSortedMaxFrames))
         return hipErrorInvalidValue;
     const long long rows = (long long)natoms * N;
     hipLaunchKernelGGL(ksg_build_rows_kernel, dim3((unsigned)rows), dim3(256), 0, s, X, N, P, D,
-                       SD, SM);
+                       SD, SM, SI);
     return hipGetLastError();
 }
 
@@ -1071,17 +1385,19 @@ hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
 
 // Kernel variants: kind 0 = one wave per row (N <= 256*S4), kind 1 = four
 // waves per row (N <= 1024*S4).  default_variant picks the smallest that fits.
-bool variant_fits(int kind, int s4, int N)
+bool variant_fits(int kind, int s4, int N, int K)
 {
+    if (kind == 2) return N <= kSortedMaxFrames && K + 1 <= kPrefixMaxT;
     const long long cap = (kind == 0 ? 256LL : 1024LL) * s4;
     const bool inst = kind == 0 ? (s4 == 1 || s4 == 2 || s4 == 3 || s4 == 4 || s4 == 5 ||
                                    s4 == 6 || s4 == 8)
-                                : (s4 >= 2 && s4 <= 8);
+                                : kind == 1 && (s4 >= 2 && s4 <= 8);
     return inst && N <= cap;
 }
 
-void default_variant(int N, int& kind, int& s4)
+void default_variant(int N, int K, int& kind, int& s4)
 {
+    if (variant_fits(2, 1, N, K)) { kind = 2; s4 = 1; return; }
     static const int single[] = {1, 2, 3, 4, 5, 6, 8};
     for (int v : single)
         if (N <= 256 * v) { kind = 0; s4 = v; return; }
@@ -1092,15 +1408,18 @@ void default_variant(int N, int& kind, int& s4)
 
 hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4)
 {
-    if (kind < 0) default_variant(a.N, kind, s4);
-    if (!variant_fits(kind, s4, a.N)) return hipErrorInvalidValue;
+    if (kind < 0) default_variant(a.N, a.K, kind, s4);
+    if (!variant_fits(kind, s4, a.N, a.K)) return hipErrorInvalidValue;
+    if (variant_sorted(kind) && !(a.SD && a.SM && a.SI)) return hipErrorInvalidValue;
     const long long nitems =
-        kind == 0 ? (a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves) * kRowSplit
+        kind != 1 ? (a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves) * kRowSplit
                   : a.npairs;
     const dim3 grid((unsigned)(((nitems + 7) / 8) * 8)), block(kWave * kPairWaves);
 #define GMX_SINGLE(V) case V: hipLaunchKernelGGL(ksg_pair_kernel<V>, grid, block, 0, s, a); break;
 #define GMX_COOP(V) case V: hipLaunchKernelGGL(ksg_pair_coop_kernel<V>, grid, block, 0, s, a); break;
-    if (kind == 0) {
+    if (kind == 2) {
+        hipLaunchKernelGGL(ksg_pair_prefix_kernel, grid, block, 0, s, a);
+    } else if (kind == 0) {
         switch (s4) {
             GMX_SINGLE(1) GMX_SINGLE(2) GMX_SINGLE(3) GMX_SINGLE(4) GMX_SINGLE(5) GMX_SINGLE(6)
             GMX_SINGLE(8)
@@ -1134,8 +1453,9 @@ hipError_t launch_ksg_finalize(const FixupArgs& f, long long nitems, const doubl
     return hipGetLastError();
 }
 
-bool batch_variant_ok(int kind, int s4, int N)
+bool batch_variant_ok(int kind, int s4, int N, int K)
 {
+    if (kind == 2) return variant_fits(2, s4, N, K);
     if (kind == 0) return (s4 == 2 || s4 == 4 || s4 == 6 || s4 == 8) && N <= 256 * s4;
     if (kind == 1) return (s4 == 3 || s4 == 4 || s4 == 6 || s4 == 8) && N <= 1024 * s4;
     return false;

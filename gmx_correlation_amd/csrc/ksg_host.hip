@@ -108,8 +108,9 @@ struct gmxcorr_ksg_plan {
     float* dD = nullptr;
     int* dQuads = nullptr;  // all-pairs run: [nquads][4] pair indices (2x2 atom tiles)
     long long nquads = 0;
-    float* dSD = nullptr;   // sorted rows and summaries (N <= kSortedMaxFrames)
+    float* dSD = nullptr;   // sorted rows, summaries, prefix indices (N <= kSortedMaxFrames)
     float* dSM = nullptr;
+    uint16_t* dSI = nullptr;
     int* dA1 = nullptr;
     int* dA2 = nullptr;
     long long pair_cap = 0;
@@ -193,16 +194,17 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
 {
     KsgArgs a{};
     if (tiles) { a.quads = pl->dQuads; a.nquads = pl->nquads; }
-    a.X = pl->dX; a.D = pl->dD; a.SD = pl->dSD; a.SM = pl->dSM; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
+    a.X = pl->dX; a.D = pl->dD; a.SD = pl->dSD; a.SM = pl->dSM; a.SI = pl->dSI; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
     a.partial = pl->dPartial; a.fixsum = pl->dFixsum; a.mi = pl->dMI;
     a.counters = pl->dCounters;
     a.npairs = npairs; a.P = pl->P; a.N = pl->N; a.K = pl->K; a.psiN_phiK = pl->psiN_phiK;
     if (detail) { a.rc_nx = pl->dRcNx; a.rc_ny = pl->dRcNy; a.rc_flag = pl->dRcFlag; }
-    // recount queue sized for 0.2% of the rows up front (runs measured here
-    // flag 0.04-0.07%); an overflow still triggers a full re-run
+    // recount queue sized for 1% of the rows up front (runs measured here
+    // queue 0.04-0.2%, the sorted-prefix path included); an overflow still
+    // triggers a full re-run
     {
         const long long rows = npairs * (long long)pl->N;
-        const long long want = std::min<long long>(std::max<long long>(65536, rows / 500),
+        const long long want = std::min<long long>(std::max<long long>(65536, rows / 100),
                                                    64LL << 20);
         int rc = ensure_fix(pl, (unsigned)want);
         if (rc) return rc;
@@ -315,12 +317,15 @@ int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** o
     size_t sbytes = 0;
     if (nframes <= gmxcorr::kSortedMaxFrames) {
         const size_t rows = (size_t)natoms * nframes;
-        sbytes = rows * (gmxcorr::sorted_pitch(nframes) + gmxcorr::kSumPitch) * sizeof(float);
+        sbytes = rows * ((gmxcorr::sorted_pitch(nframes) + gmxcorr::kSumPitch) * sizeof(float) +
+                         gmxcorr::kPrefix * sizeof(uint16_t));
         if ((e = hipMalloc(&pl->dSD, rows * gmxcorr::sorted_pitch(nframes) * sizeof(float))) !=
             hipSuccess)
             return bail(hip_fail(e, "hipMalloc(sorted rows)"));
         if ((e = hipMalloc(&pl->dSM, rows * gmxcorr::kSumPitch * sizeof(float))) != hipSuccess)
             return bail(hip_fail(e, "hipMalloc(sorted-row summaries)"));
+        if ((e = hipMalloc(&pl->dSI, rows * gmxcorr::kPrefix * sizeof(uint16_t))) != hipSuccess)
+            return bail(hip_fail(e, "hipMalloc(sorted-row prefix indices)"));
     }
     if ((e = hipMalloc(&pl->dCounters, 4 * sizeof(unsigned))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(counters)"));
@@ -345,7 +350,7 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
                           pl->stream));
     HIPCHK(hipEventRecord(pl->ev[1], pl->stream));
     HIPCHK(gmxcorr::launch_build_rows(pl->dX, pl->natoms, pl->N, pl->P, pl->dD, pl->dSD, pl->dSM,
-                                      pl->stream));
+                                      pl->dSI, pl->stream));
     HIPCHK(hipEventRecord(pl->ev[5], pl->stream));
     HIPCHK(hipStreamSynchronize(pl->stream));
     pl->st.ms_load = elapsed(pl->ev[0], pl->ev[1]);
@@ -356,6 +361,7 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
     td.D = pl->dD;
     td.SD = pl->dSD;
     td.SM = pl->dSM;
+    td.SI = pl->dSI;
     td.psi = pl->dPsi;
     td.P = pl->P;
     td.N = pl->N;
@@ -452,16 +458,16 @@ int gmxcorr_ksg_plan_matrix(gmxcorr_ksg_plan* pl, double* mat)
 int gmxcorr_ksg_plan_set_variant(gmxcorr_ksg_plan* pl, int kind, int s4)
 {
     if (!pl) return fail(GMXCORR_EINVAL, "set_variant: null plan");
-    if (kind >= 0 && !gmxcorr::variant_fits(kind, s4, pl->N))
-        return fail(GMXCORR_EUNSUP, "kernel variant does not cover this frame count");
+    if (kind >= 0 && !gmxcorr::variant_fits(kind, s4, pl->N, pl->K))
+        return fail(GMXCORR_EUNSUP, "kernel variant does not cover this frame count and k");
     pl->var_kind = kind;
     pl->var_s4 = s4;
     return GMXCORR_OK;
 }
 
-int gmxcorr_ksg_variant_fits(int kind, int s4, int nframes)
+int gmxcorr_ksg_variant_fits(int kind, int s4, int nframes, int k)
 {
-    return gmxcorr::variant_fits(kind, s4, nframes) ? 1 : 0;
+    return gmxcorr::variant_fits(kind, s4, nframes, k) ? 1 : 0;
 }
 
 int gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* pl, gmxcorr_ksg_stats* st)
@@ -475,7 +481,7 @@ void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* pl)
 {
     if (!pl) return;
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dSD); dfree(pl->dSM); dfree(pl->dQuads); dfree(pl->dA1); dfree(pl->dA2);
+    dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dSD); dfree(pl->dSM); dfree(pl->dSI); dfree(pl->dQuads); dfree(pl->dA1); dfree(pl->dA2);
     dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix);
     dfree(pl->dCounters); dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
     dfree(pl->dTraj);

@@ -26,8 +26,9 @@ struct FixEntry {
 struct KsgArgs {
     const double* X;      // [3n][N] normalised coordinates (FP64)
     const float*  D;      // [n][N][P] per-atom max-norm distance rows (FP32)
-    const float*  SD;     // [n][N][sorted_pitch(N)] sorted rows (single-wave path)
+    const float*  SD;     // [n][N][sorted_pitch(N)] sorted rows (N <= kSortedMaxFrames)
     const float*  SM;     // [n][N][kSumPitch] their summaries
+    const uint16_t* SI;   // [n][N][kPrefix] frame indices of the kPrefix smallest values
     const double* psi;    // [N+1]
     const int*    a1;     // [npairs]
     const int*    a2;     // [npairs]
@@ -54,8 +55,9 @@ struct KsgArgs {
 struct TrajDesc {
     const double* X;     // [3n][N] normalised rows (FP64)
     const float* D;      // [n][N][P] distance rows (FP32)
-    const float* SD;     // sorted rows and summaries (kind 0 only)
+    const float* SD;     // sorted rows, summaries and prefix indices (kinds 0 and 2)
     const float* SM;
+    const uint16_t* SI;
     const double* psi;   // [N+1]
     long long P;
     int N;
@@ -105,14 +107,20 @@ constexpr int kRowSplit = 4;
 // kernels read whole 64-lane chunks of 4 frames without a per-lane bound.
 __host__ __device__ inline long long row_pitch(int N) { return ((long long)N + 255) / 256 * 256; }
 
-// Sorted distance rows (single-wave path, N <= kSortedMaxFrames): row pitch
-// sorted_pitch(N) floats (64 front pad, N values, >= 64 padding values) and a
-// summary of kSumPitch floats per row (every 64th sorted value).
+// Sorted distance rows (single-wave paths, N <= kSortedMaxFrames): row pitch
+// sorted_pitch(N) floats (64 front pad, N values, padding up to at least
+// kPrefix values and 64 more), a summary of kSumPitch floats per row (every
+// 64th sorted value) and the frame indices of the kPrefix smallest values
+// (ties in ascending frame order; 0 past N).
 constexpr int kSortedMaxFrames = 2048;
 constexpr int kSumPitch = 64;
+constexpr int kPrefix = 512;
+// The sorted-prefix path (kind 2) is used for k + 1 <= kPrefixMaxT.
+constexpr int kPrefixMaxT = 256;
 __host__ __device__ inline long long sorted_pitch(int N)
 {
-    return 64 + ((long long)N + 64 + 63) / 64 * 64;
+    const long long n = N > kPrefix ? N : kPrefix;
+    return 64 + (n + 64 + 63) / 64 * 64;
 }
 
 // Read-only view of a loaded plan's device data (for the surrogate features).
@@ -124,22 +132,28 @@ struct PlanView {
     hipStream_t stream;
 };
 
-// SD/SM: sorted rows and their summaries for the single-wave path, or both
-// nullptr (required for N > kSortedMaxFrames).
+// SD/SM/SI: sorted rows, their summaries and prefix indices for the
+// single-wave paths, or all nullptr (required for N > kSortedMaxFrames).
 hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, float* D,
-                             float* SD, float* SM, hipStream_t s);
+                             float* SD, float* SM, uint16_t* SI, hipStream_t s);
 hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
                             unsigned long long* queue, hipStream_t s);
-// kind 0: one wave per frame row, N <= 256*s4 (s4 in 1,2,3,4,5,6,8);
-// kind 1: four waves per row, N <= 1024*s4 (s4 in 2..8); kind < 0: default.
-bool variant_fits(int kind, int s4, int N);
-void default_variant(int N, int& kind, int& s4);
+// kind 0: one wave per frame row streaming both distance rows, N <= 256*s4
+//         (s4 in 1,2,3,4,5,6,8);
+// kind 1: four waves per row, N <= 1024*s4 (s4 in 2..8);
+// kind 2: one wave per row over the sorted prefix of the sparser atom,
+//         N <= kSortedMaxFrames and K + 1 <= kPrefixMaxT (s4 ignored, 1);
+// kind < 0: default.
+bool variant_fits(int kind, int s4, int N, int K);
+void default_variant(int N, int K, int& kind, int& s4);
 hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind = -1, int s4 = 0);
 hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s);
 hipError_t launch_ksg_finalize(const FixupArgs& f, long long nitems, const double* partial,
                                int nsplit, double* mi, hipStream_t s);
 // Code paths compiled into the batched kernel.
-bool batch_variant_ok(int kind, int s4, int N);
+bool batch_variant_ok(int kind, int s4, int N, int K);
+// Whether a variant reads the sorted rows (and needs SD, SM, SI built).
+inline bool variant_sorted(int kind) { return kind == 0 || kind == 2; }
 hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s);
 hipError_t launch_selftest(int* out, hipStream_t s);
 

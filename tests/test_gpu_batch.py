@@ -56,7 +56,7 @@ def test_batch_explicit_paths(gpu, trajs):
     """Every compiled batch path that covers a trajectory gives the same MI."""
     rows, psi, k = trajs[1]                      # N = 900
     ref = None
-    for kind, s4 in batch.fitting_paths(rows.shape[1]):
+    for kind, s4 in batch.fitting_paths(rows.shape[1], k):
         b = batch.KsgBatch()
         b.add(rows, psi, k, kind, s4)
         b.run(nblocks=32)
@@ -69,7 +69,7 @@ def test_batch_explicit_paths(gpu, trajs):
 def test_tuner_returns_a_fitting_path(gpu, trajs):
     rows, psi, k = trajs[3]                      # N = 5000
     (kind, s4), timings = batch.tune_variant(rows, psi, k, natoms_probe=4, reps=1)
-    assert (kind, s4) in batch.fitting_paths(rows.shape[1])
+    assert (kind, s4) in batch.fitting_paths(rows.shape[1], k)
     assert all(v > 0 for v in timings.values())
 
 

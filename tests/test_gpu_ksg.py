@@ -160,6 +160,48 @@ def test_readme_workload_matches_reference(gpu, oracle_lib):
         assert np.array_equal(d["ny"][p], ref_d["ny"])
 
 
+@pytest.mark.parametrize("kind,s4", [(0, 8), (2, 1)])
+def test_single_wave_variants_bitexact_2000(gpu, oracle_lib, kind, s4):
+    """Both one-wave-per-row paths at 2000 frames, k = 100 (kind 0 streams
+    both distance rows, kind 2 reads the sorted prefix of the sparser atom):
+    per-frame counts equal the FP64 brute-force recount."""
+    c = synth.generate(synth.TrajSpec(natoms=6, nframes=2000, nmodes=5, seed=7))
+    rows, psi = ksg.kraskov_prepare(ksg.center(c))
+    a1 = np.array([1, 5, 3]); a2 = np.array([0, 2, 4])
+    with ksg.KsgPlan(6, 2000, 100) as plan:
+        plan.load(rows, psi)
+        plan.set_variant(kind, s4)
+        mi, d = plan.run_pairs(a1, a2, detail=True)
+    for p in range(len(a1)):
+        v, ref = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), 100, method="brute",
+                                     detail=True)
+        assert np.array_equal(d["nx"][p], ref["nx"]), p
+        assert np.array_equal(d["ny"][p], ref["ny"]), p
+        assert abs(mi[p] - v) <= 1e-12
+
+
+def test_prefix_overflow_rows_take_fp64_recount(gpu, oracle_lib):
+    """Independent atoms at k = 200: the (k+1)-th joint neighbour lies beyond
+    the 512 nearest frames of either atom in most rows (about sqrt(201*2000)
+    = 634 frames are needed), so the sorted-prefix path hands those rows to
+    the FP64 recount unresolved; counts still equal the FP64 recount."""
+    rng = np.random.default_rng(11)
+    coords = rng.standard_normal((2000, 4, 3)).astype(np.float32)
+    rows, psi = ksg.kraskov_prepare(ksg.center(coords))
+    a1 = np.array([1, 3]); a2 = np.array([0, 2])
+    with ksg.KsgPlan(4, 2000, 200) as plan:
+        plan.load(rows, psi)
+        mi, d = plan.run_pairs(a1, a2, detail=True)
+        st = plan.stats()
+    assert st["fixup_rows"] > 1000
+    for p in range(len(a1)):
+        v, ref = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), 200, method="brute",
+                                     detail=True)
+        assert np.array_equal(d["nx"][p], ref["nx"]), p
+        assert np.array_equal(d["ny"][p], ref["ny"]), p
+        assert abs(mi[p] - v) <= 1e-12
+
+
 @pytest.mark.parametrize("N,k,npairs", [(3000, 50, 2), (5000, 100, 1), (8000, 10, 1)])
 def test_long_trajectories_cooperative_kernel(gpu, oracle_lib, N, k, npairs):
     """N > 2048 runs the 4-waves-per-row kernel; counts and MI against the
