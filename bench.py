@@ -420,10 +420,11 @@ def main() -> int:
                    "nframes": N, "k": k, "npairs": npairs},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "ksg_pair_kernel", "kernel_ms": kern_s * 1e3,
+                     "kernel": "ksg_pair_prefix_kernel", "kernel_ms": kern_s * 1e3,
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": cpu,
         "detail": {"fixup_ms": float(np.mean(fix_ms)), "fixup_rows": st["fixup_rows"],
+                   "long_rows": st["long_rows"],
                    "exact_ties": st["ties"], "row_build_ms": st_load["ms_build"],
                    "row_bytes": st_load["row_bytes"], "parity": parity},
     }

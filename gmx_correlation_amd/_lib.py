@@ -32,6 +32,7 @@ class KsgStats(ctypes.Structure):
         ("ms_fixup", ctypes.c_double),
         ("npairs", _ll),
         ("fixup_rows", _ll),
+        ("long_rows", _ll),
         ("ties", _ll),
         ("row_bytes", _ll),
         ("nframes", _i),

@@ -230,6 +230,8 @@ struct RowOut {
     unsigned* counters;
     FixEntry* fixq;
     unsigned fixcap;
+    FixEntry* longq;    // nullptr: long rows go to the FP64 recount unresolved
+    unsigned longcap;
     int* rc_nx;
     int* rc_ny;
     unsigned char* rc_flag;
@@ -248,6 +250,24 @@ __device__ __forceinline__ void queue_row(const RowOut& o, long long item, int r
     }
 }
 
+// A row whose (k+1)-th joint neighbour lies beyond the sorted prefix.
+__device__ __forceinline__ void queue_long(const RowOut& o, long long item, int row)
+{
+    if (!o.longq) {
+        queue_row(o, item, row, kUnknownEps);
+        return;
+    }
+    const unsigned slot = atomicAdd(&o.counters[2], 1u);
+    if (slot < o.longcap) {
+        FixEntry fe;
+        fe.pair = (int)item;
+        fe.row = row;
+        fe.eps_bits = kUnknownEps;
+        fe.pad = 0;
+        o.longq[slot] = fe;
+    }
+}
+
 __device__ __forceinline__ RowOut row_out(const KsgArgs& a)
 {
     RowOut o;
@@ -255,6 +275,8 @@ __device__ __forceinline__ RowOut row_out(const KsgArgs& a)
     o.counters = a.counters;
     o.fixq = a.fixq;
     o.fixcap = a.fixcap;
+    o.longq = a.longq;
+    o.longcap = a.longcap;
     o.rc_nx = a.rc_nx;
     o.rc_ny = a.rc_ny;
     o.rc_flag = a.rc_flag;
@@ -564,13 +586,15 @@ __device__ __forceinline__ double rows_single(const float* Da, const float* Db, 
 // Sorted-prefix rows (kind 2): frame rows i0, i0+step, ... < i1 of one pair,
 // one wave, at most kPrefix rows.  For row i, let P be the atom whose sorted
 // row is larger at position 255 (the sparser neighbourhood) and Q the other.
-// Lane l holds the sorted P values at positions 8l..8l+7 and gathers the Q
-// distances of the same frames from Q's row (staged in LDS), so the wave
+// Lane l holds the sorted P values at positions 4l+c and 256+4l+c (c < 4)
+// and gathers the Q distances of the same frames from Q's row (staged in
+// LDS), so the wave
 // sees the joint keys J = max(dP, dQ) of the kPrefix nearest frames in P.
 // Every frame outside that prefix has dP >= H = SP[kPrefix - 1], hence
 // J >= H: if the (k+1)-th smallest key of the prefix is below H it is the
 // row's, and the prefix holds every frame at or below it.  Otherwise the
-// row is handed on unresolved (FP64 recount).
+// row goes to the long-row queue (streaming path) or, without one, to the
+// FP64 recount unresolved.
 //   histogram  256 bins over [L, H), L = SP[k] (the (k+1)-th smallest dP, a
 //              lower bound of the key), bin 0 below L, bin 255 above H;
 //   sweep B    frames below the target bin are in S; its frames go to LDS;
@@ -636,10 +660,13 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         const long long row = i0 + (long long)t * step;
         const bool sa = side_of(t);
         const uint32_t* base = (sa ? pr.SDa : pr.SDb) + row * pr.PS + 64;
-        nv0 = ld_lane<uint4>(base, lane4 * 8u);
-        nv1 = ld_lane<uint4>(base, lane4 * 8u + 16u);
-        ni = ld_lane<uint4>((sa ? pr.SIa : pr.SIb) + row * kPrefix, lane4 * 4u);
-        nL = ld_lane<uint32_t>(base, (uint32_t)(T - 1) * 4u);
+        nv0 = ld_lane<uint4>(base, lane4 * 4u);
+        nv1 = ld_lane<uint4>(base, lane4 * 4u + 1024u);
+        const uint16_t* ib = (sa ? pr.SIa : pr.SIb) + row * kPrefix;
+        const uint2 i01 = ld_lane<uint2>(ib, lane4 * 2u);
+        const uint2 i23 = ld_lane<uint2>(ib, lane4 * 2u + 512u);
+        ni = make_uint4(i01.x, i01.y, i23.x, i23.y);
+        if (lane == 0) nL = ld_lane<uint32_t>(base, (uint32_t)(T - 1) * 4u);
         nsm = ld_lane<uint32_t>((sa ? pr.SMb : pr.SMa) + row * kSumPitch, lane4);
         const uint32_t* rq = (sa ? pr.Db : pr.Da) + row * pr.P;
 #pragma unroll
@@ -668,7 +695,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             v[4] = nv1.x; v[5] = nv1.y; v[6] = nv1.z; v[7] = nv1.w;
             const uint32_t idx[4] = {ni.x, ni.y, ni.z, ni.w};
             sm = nsm;
-            L = (uint32_t)__builtin_amdgcn_readfirstlane((int)nL);
+            L = (uint32_t)readlane((int)nL, 0);
             // stage the Q row (frames 256m + 4l + c at lane l, chunk m)
 #pragma unroll
             for (int m = 0; m < 8; ++m)
@@ -687,7 +714,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             const int np = pnp - 1;
             const bool flag = tq || ptie;
             const int nx = pside ? np : nq_, ny = pside ? nq_ : np;
-            if (!flag) {
+            if (!flag && lane == 0) {   // the sum is kept on lane 0
                 px = o.psi[nx];
                 py = o.psi[ny];
             }
@@ -780,9 +807,15 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             wave_sync();
         }
         if (queued) {
+            // beyond the prefix: the target is the overflow bin or the key
+            // is not below H; otherwise an FP32 tie or a crowded bin
+            const bool beyond = b1 == kPBins - 1 || (eps_key != kUnknownEps && eps_key >= H);
             if (lane == 0) {
-                queue_row(o, item, (int)row, eps_key < H ? eps_key : kUnknownEps);
-                if (o.rc_nx) {
+                if (beyond)
+                    queue_long(o, item, (int)row);
+                else
+                    queue_row(o, item, (int)row, eps_key);
+                if (o.rc_nx && !(beyond && o.longq)) {
                     const size_t off = (size_t)item * N + row;
                     o.rc_nx[off] = -1;
                     o.rc_ny[off] = -1;
@@ -1065,6 +1098,32 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
 }
 
+// Rows the sorted-prefix kernel queued to longq: one wave per entry runs the
+// streaming one-wave path (rows_single) on that row, with the same recount
+// rules; the row's psi[nx] + psi[ny] is added to fixsum[pair].
+__global__ __launch_bounds__(256) void ksg_long_rows_kernel(KsgArgs args, unsigned nent)
+{
+    __shared__ __attribute__((aligned(4 * kBins))) uint32_t lds[kPairWaves][kBins];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const unsigned e = blockIdx.x * kPairWaves + w;
+    if (e >= nent) return;   // no block barriers below
+    const FixEntry fe = args.longq[e];
+    const long long p = fe.pair;
+    const int N = args.N;
+    const int a1 = args.a1[p], a2 = args.a2[p];
+    SortedRows sr;
+    sr.PS = sorted_pitch(N);
+    sr.SDa = args.SD + (size_t)a1 * N * sr.PS;
+    sr.SDb = args.SD + (size_t)a2 * N * sr.PS;
+    sr.SMa = args.SM + (size_t)a1 * N * kSumPitch;
+    sr.SMb = args.SM + (size_t)a2 * N * kSumPitch;
+    const double acc = rows_single<kSortedMaxFrames / 256>(
+        args.D + (size_t)a1 * N * args.P, args.D + (size_t)a2 * N * args.P, sr, args.P, N,
+        args.K + 1, fe.row, fe.row + 1, 1, p, lds[w], lane, row_out(args));
+    if (lane == 0) atomicAdd(&args.fixsum[p], acc);
+}
+
 template <int S4>
 __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
 {
@@ -1145,6 +1204,8 @@ __global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
     o.counters = b.counters;
     o.fixq = b.fixq;
     o.fixcap = b.fixcap;
+    o.longq = nullptr;
+    o.longcap = 0u;
     o.rc_nx = nullptr;
     o.rc_ny = nullptr;
     o.rc_flag = nullptr;
@@ -1326,7 +1387,7 @@ __global__ void ksg_clear_kernel(double* fixsum, long long npairs, unsigned* cou
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < npairs) fixsum[t] = 0.0;
-    if (t < 2) counters[t] = 0u;
+    if (t < 3) counters[t] = 0u;
     if (t == 0 && queue) *queue = 0ull;
 }
 
@@ -1377,7 +1438,7 @@ hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, fl
 hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
                             unsigned long long* queue, hipStream_t s)
 {
-    const long long m = n > 2 ? n : 2;
+    const long long m = n > 3 ? n : 3;
     hipLaunchKernelGGL(ksg_clear_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
                        fixsum, n, counters, queue);
     return hipGetLastError();
@@ -1442,6 +1503,15 @@ hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s)
     const unsigned per_block = 4;
     hipLaunchKernelGGL(ksg_fixup_kernel, dim3((nqueued + per_block - 1) / per_block),
                        dim3(kWave * per_block), 0, s, f, nqueued);
+    return hipGetLastError();
+}
+
+hipError_t launch_ksg_long_rows(const KsgArgs& a, unsigned nlong, hipStream_t s)
+{
+    if (nlong == 0) return hipSuccess;
+    if (!a.longq || !a.SD || !a.SM || a.N > kSortedMaxFrames) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ksg_long_rows_kernel, dim3((nlong + kPairWaves - 1) / kPairWaves),
+                       dim3(kWave * kPairWaves), 0, s, a, nlong);
     return hipGetLastError();
 }
 

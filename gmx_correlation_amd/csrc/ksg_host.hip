@@ -119,6 +119,8 @@ struct gmxcorr_ksg_plan {
     double* dMI = nullptr;
     FixEntry* dFix = nullptr;
     unsigned fixcap = 0;
+    FixEntry* dLong = nullptr;   // rows beyond the sorted prefix (kind 2)
+    unsigned longcap = 0;
     unsigned* dCounters = nullptr;
     int* dRcNx = nullptr;
     int* dRcNy = nullptr;
@@ -154,6 +156,15 @@ int ensure_fix(gmxcorr_ksg_plan* pl, unsigned cap)
     dfree(pl->dFix);
     HIPCHK(hipMalloc(&pl->dFix, (size_t)cap * sizeof(FixEntry)));
     pl->fixcap = cap;
+    return GMXCORR_OK;
+}
+
+int ensure_long(gmxcorr_ksg_plan* pl, unsigned cap)
+{
+    if (cap <= pl->longcap) return GMXCORR_OK;
+    dfree(pl->dLong);
+    HIPCHK(hipMalloc(&pl->dLong, (size_t)cap * sizeof(FixEntry)));
+    pl->longcap = cap;
     return GMXCORR_OK;
 }
 
@@ -199,14 +210,14 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
     a.counters = pl->dCounters;
     a.npairs = npairs; a.P = pl->P; a.N = pl->N; a.K = pl->K; a.psiN_phiK = pl->psiN_phiK;
     if (detail) { a.rc_nx = pl->dRcNx; a.rc_ny = pl->dRcNy; a.rc_flag = pl->dRcFlag; }
-    // recount queue sized for 1% of the rows up front (runs measured here
-    // queue 0.04-0.2%, the sorted-prefix path included); an overflow still
-    // triggers a full re-run
+    // recount and long-row queues sized for 1% of the rows up front (runs
+    // measured here queue 0.04-0.2%); an overflow triggers a full re-run
     {
         const long long rows = npairs * (long long)pl->N;
         const long long want = std::min<long long>(std::max<long long>(65536, rows / 100),
                                                    64LL << 20);
         int rc = ensure_fix(pl, (unsigned)want);
+        if (!rc) rc = ensure_long(pl, (unsigned)want);
         if (rc) return rc;
     }
     gmxcorr::FixupArgs f{};
@@ -217,17 +228,31 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
     f.fixsum = pl->dFixsum;
     f.counters = pl->dCounters;
     if (detail) { f.rc_nx = pl->dRcNx; f.rc_ny = pl->dRcNy; f.rc_flag = pl->dRcFlag; }
-    for (int attempt = 0; attempt < 2; ++attempt) {
+    for (int attempt = 0; attempt < 3; ++attempt) {
         a.fixq = pl->dFix; a.fixcap = pl->fixcap;
+        a.longq = pl->dLong; a.longcap = pl->longcap;
         f.fixq = pl->dFix;
         HIPCHK(gmxcorr::launch_ksg_clear(pl->dFixsum, npairs, pl->dCounters, nullptr, pl->stream));
         HIPCHK(hipEventRecord(pl->ev[2], pl->stream));
         HIPCHK(gmxcorr::launch_ksg_pairs(a, pl->stream, pl->var_kind, pl->var_s4));
         HIPCHK(hipEventRecord(pl->ev[3], pl->stream));
-        unsigned counters[2] = {0, 0};
+        unsigned counters[3] = {0, 0, 0};
         HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,
                               pl->stream));
         HIPCHK(hipStreamSynchronize(pl->stream));
+        if (counters[2] > pl->longcap) {
+            int rc = ensure_long(pl, counters[2] + counters[2] / 4 + 1024);
+            if (rc) return rc;
+            continue;
+        }
+        const unsigned nlong = counters[2];
+        if (nlong) {
+            // the streaming path may queue FP32-ambiguous rows of its own
+            HIPCHK(gmxcorr::launch_ksg_long_rows(a, nlong, pl->stream));
+            HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters),
+                                  hipMemcpyDeviceToHost, pl->stream));
+            HIPCHK(hipStreamSynchronize(pl->stream));
+        }
         if (counters[0] > pl->fixcap) {
             int rc = ensure_fix(pl, counters[0] + counters[0] / 4 + 1024);
             if (rc) return rc;
@@ -244,11 +269,12 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
         pl->st.ms_fixup = elapsed(pl->ev[3], pl->ev[4]);
         pl->st.npairs = npairs;
         pl->st.fixup_rows = counters[0];
+        pl->st.long_rows = nlong;
         pl->st.ties = counters[1];
         pl->last_npairs = npairs;
         return GMXCORR_OK;
     }
-    return fail(GMXCORR_EHIP, "FP64 recount queue overflowed twice");
+    return fail(GMXCORR_EHIP, "recount queues overflowed on every attempt");
 }
 
 }  // namespace
@@ -482,7 +508,7 @@ void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* pl)
     if (!pl) return;
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
     dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dSD); dfree(pl->dSM); dfree(pl->dSI); dfree(pl->dQuads); dfree(pl->dA1); dfree(pl->dA2);
-    dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix);
+    dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix); dfree(pl->dLong);
     dfree(pl->dCounters); dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
     dfree(pl->dTraj);
     for (auto& ev : pl->ev)

@@ -36,8 +36,10 @@ struct KsgArgs {
     double*       fixsum;    // [npairs] same over FP64-recounted rows
     double*       mi;        // [npairs]
     FixEntry*     fixq;      // [fixcap]
-    unsigned*     counters;  // [0] queued rows, [1] exact ties
+    unsigned*     counters;  // [0] queued rows, [1] exact ties, [2] long rows
     unsigned      fixcap;
+    FixEntry*     longq;     // optional [longcap]: rows beyond the sorted prefix (kind 2)
+    unsigned      longcap;
     int*          rc_nx;     // optional [npairs][N]
     int*          rc_ny;     // optional [npairs][N]
     unsigned char* rc_flag;  // optional [npairs][N]
@@ -148,6 +150,9 @@ bool variant_fits(int kind, int s4, int N, int K);
 void default_variant(int N, int K, int& kind, int& s4);
 hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind = -1, int s4 = 0);
 hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s);
+// Rows that the sorted-prefix path queued to a.longq (k-th neighbour beyond
+// the prefix): the streaming one-wave path per row, psi sums into a.fixsum.
+hipError_t launch_ksg_long_rows(const KsgArgs& a, unsigned nlong, hipStream_t s);
 hipError_t launch_ksg_finalize(const FixupArgs& f, long long nitems, const double* partial,
                                int nsplit, double* mi, hipStream_t s);
 // Code paths compiled into the batched kernel.

@@ -57,9 +57,11 @@ typedef struct {
     double ms_load;        /* host->device copy of rows and psi */
     double ms_build;       /* per-atom distance-row build */
     double ms_pairs;       /* main pair kernel (HIP events on the plan stream) */
-    double ms_fixup;       /* FP64 recount of flagged rows + finalise */
+    double ms_fixup;       /* long rows + FP64 recount of flagged rows + finalise */
     long long npairs;      /* pairs in the last run */
     long long fixup_rows;  /* rows recounted in FP64 (FP32 near-ties) */
+    long long long_rows;   /* rows whose k-th neighbour lay beyond the sorted
+                              prefix, rerun on the streaming path */
     long long ties;        /* rows with an exact FP64 tie at the k-th distance */
     long long row_bytes;   /* bytes of distance rows resident in HBM */
     int nframes, k, natoms, pitch;

@@ -180,11 +180,12 @@ def test_single_wave_variants_bitexact_2000(gpu, oracle_lib, kind, s4):
         assert abs(mi[p] - v) <= 1e-12
 
 
-def test_prefix_overflow_rows_take_fp64_recount(gpu, oracle_lib):
+def test_prefix_overflow_rows_take_streaming_path(gpu, oracle_lib):
     """Independent atoms at k = 200: the (k+1)-th joint neighbour lies beyond
     the 512 nearest frames of either atom in most rows (about sqrt(201*2000)
     = 634 frames are needed), so the sorted-prefix path hands those rows to
-    the FP64 recount unresolved; counts still equal the FP64 recount."""
+    the long-row kernel (streaming path); counts still equal the FP64
+    recount."""
     rng = np.random.default_rng(11)
     coords = rng.standard_normal((2000, 4, 3)).astype(np.float32)
     rows, psi = ksg.kraskov_prepare(ksg.center(coords))
@@ -193,7 +194,7 @@ def test_prefix_overflow_rows_take_fp64_recount(gpu, oracle_lib):
         plan.load(rows, psi)
         mi, d = plan.run_pairs(a1, a2, detail=True)
         st = plan.stats()
-    assert st["fixup_rows"] > 1000
+    assert st["long_rows"] > 1000
     for p in range(len(a1)):
         v, ref = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), 200, method="brute",
                                      detail=True)
