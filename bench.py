@@ -421,7 +421,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "ksg_pair_prefix_kernel", "kernel_ms": kern_s * 1e3,
-                     "algorithmic_bytes_per_launch": algo_bytes},
+                     "algorithmic_bytes_per_launch": algo_bytes,
+                     "traffic_gbs": traffic / kern_s / 1e9 if traffic else None,
+                     "traffic_frac": traffic / kern_s / 1e9 / HBM_PEAK_GBS if traffic else None},
         "cpu_baseline": cpu,
         "detail": {"fixup_ms": float(np.mean(fix_ms)), "fixup_rows": st["fixup_rows"],
                    "long_rows": st["long_rows"],

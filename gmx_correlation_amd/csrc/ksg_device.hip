@@ -61,6 +61,16 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Orders a wave's accesses to its own LDS region: the LDS unit executes a
+// wave's LDS instructions in issue order, so only compiler reordering has to
+// be prevented (no s_waitcnt is emitted).
+__device__ __forceinline__ void lds_order()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // DPP lane exchange; lanes without a valid source (or outside row_mask) read 0.
 template <int CTRL, int ROW_MASK = 0xf>
 __device__ __forceinline__ int dpp(int v)
@@ -602,6 +612,7 @@ __device__ __forceinline__ double rows_single(const float* Da, const float* Db, 
 // nP = #{dP <= epsP} is counted in registers (epsP < H), nQ from Q's sorted
 // row (summary, then one block, consumed one row later).  The next row's
 // prefix, Q summary and whole Q row are loaded while a row is processed.
+// All LDS traffic is the wave's own region, ordered by lds_order().
 // ---------------------------------------------------------------------------
 constexpr int kPBins = 256;
 
@@ -700,7 +711,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
 #pragma unroll
             for (int m = 0; m < 8; ++m)
                 if (m < nblk) reinterpret_cast<uint4*>(lds.qrow)[m * kWave + lane] = nq[m];
-            wave_sync();
+            lds_order();
 #pragma unroll
             for (int e = 0; e < 8; ++e) q[e] = lds.qrow[(idx[e >> 1] >> (16 * (e & 1))) & 0xffffu];
             if (t + 1 < R) issue(t + 1);
@@ -738,7 +749,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         if ((span1 >> sh) > (uint32_t)(kPBins - 3)) ++sh;
         const uint32_t Lm = L - min(L, 1u << sh);
         reinterpret_cast<uint4*>(hist)[lane] = make_uint4(0u, 0u, 0u, 0u);
-        wave_sync();
+        lds_order();
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const uint32_t key = max(v[e], q[e]);
@@ -747,10 +758,10 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
-        wave_sync();
+        lds_order();
         int b1, r0, pop;
         find_bin<kPBins / kWave>(hist, lane, T, b1, r0, pop);
-        wave_sync();
+        lds_order();
         bool queued = b1 == 0 || b1 == kPBins - 1 || pop > kWave;
         uint32_t eps_key = kUnknownEps, ex = 0u, ey = 0u;
         if (!queued) {
@@ -773,7 +784,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
                     nstored += __popcll(cm);
                 }
             }
-            wave_sync();
+            lds_order();
             // ---- select: r0-th smallest of the candidates' offsets in the bin
             const bool act = lane < pop;
             uint32_t ca = 0u, cb = 0u, low = 0xffffffffu;
@@ -804,7 +815,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             queued = __popcll(__ballot(in)) != r0 || eps_key >= H;
             ex = wave_max_u(max(ex, in ? ca : 0u));
             ey = wave_max_u(max(ey, in ? cb : 0u));
-            wave_sync();
+            lds_order();
         }
         if (queued) {
             // beyond the prefix: the target is the overflow bin or the key
