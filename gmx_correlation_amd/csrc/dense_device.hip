@@ -254,6 +254,68 @@ __global__ __launch_bounds__(256) void dense_copula_kernel(const double* __restr
     }
 }
 
+// Order-preserving 64-bit key of a finite double; v + 0.0 maps -0.0 to +0.0,
+// so values that compare equal share one key.
+__device__ __forceinline__ uint64_t order_key(double v)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v + 0.0);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// ---------------------------------------------------------------------------
+// Copula scores by sorting (N <= kCopulaSortFrames): block per component row,
+// the row's keys bitonic-sorted in LDS (M = power of two >= N, padding keys
+// all-ones sort last), then per frame lt = #{g: v_g < v_f} and
+// eq = #{g: v_g == v_f} by binary search.  Same scores as
+// dense_copula_kernel, in O(N log N) per row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void dense_copula_sort_kernel(const double* __restrict__ V,
+                                                                 int N, long long ldv,
+                                                                 double* __restrict__ Z,
+                                                                 long long ldz)
+{
+    extern __shared__ uint64_t keys[];
+    const long long c = blockIdx.x;
+    const double* row = V + c * ldv;
+    int M = 64;
+    while (M < N) M <<= 1;
+    for (int f = threadIdx.x; f < M; f += blockDim.x) keys[f] = f < N ? order_key(row[f]) : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= M; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
+                const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
+                const uint64_t x = keys[lo], y = keys[hi];
+                if ((x > y) == ((lo & k) == 0)) {
+                    keys[lo] = y;
+                    keys[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    for (int f = threadIdx.x; f < N; f += blockDim.x) {
+        const uint64_t key = order_key(row[f]);
+        int lo = 0, hi = N;
+        while (lo < hi) {   // first position with keys >= key
+            const int mid = (lo + hi) >> 1;
+            if (keys[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        const int lt = lo;
+        hi = N;
+        while (lo < hi) {   // first position with keys > key
+            const int mid = (lo + hi) >> 1;
+            if (keys[mid] <= key) lo = mid + 1; else hi = mid;
+        }
+        const int eq = lo - lt;
+        double z = 0.0;
+        for (int t = lt; t < lt + eq; ++t) {
+            const double pr = (t + 0.5) / N;
+            z += sqrt(2.0) * erfinv_acklam(2.0 * pr - 1.0);
+        }
+        Z[c * ldz + f] = z / eq;
+    }
+}
+
 // traj [n][N][3] -> component rows [3n][N]
 __global__ void dense_rows_kernel(const double* __restrict__ traj, int natoms, int N,
                                   double* __restrict__ rows)
@@ -306,6 +368,17 @@ hipError_t launch_dense_copula(const double* V, int M, int N, long long ldv, dou
                                long long ldz, hipStream_t s)
 {
     if (N > kCopulaMaxFrames) return hipErrorInvalidValue;
+    if (N <= kCopulaSortFrames) {
+        int Mp = 64;
+        while (Mp < N) Mp <<= 1;
+        const size_t kbytes = (size_t)Mp * sizeof(uint64_t);
+        hipError_t e = hipFuncSetAttribute((const void*)dense_copula_sort_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kbytes);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(dense_copula_sort_kernel, dim3(M), dim3(1024), kbytes, s, V, N, ldv, Z,
+                           ldz);
+        return hipGetLastError();
+    }
     const size_t bytes = (size_t)N * sizeof(double);
     hipError_t e = hipFuncSetAttribute((const void*)dense_copula_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);

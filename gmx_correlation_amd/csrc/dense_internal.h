@@ -7,6 +7,8 @@ namespace gmxcorr {
 // Largest frame count whose component row is staged whole in LDS by the
 // copula kernel (8 bytes per frame within 160 KB).
 constexpr int kCopulaMaxFrames = 19456;
+// Up to this frame count the copula sorts each row in LDS (64-bit keys).
+constexpr int kCopulaSortFrames = 16384;
 
 hipError_t launch_dense_rows(const double* traj, int natoms, int N, double* rows, hipStream_t s);
 hipError_t launch_dense_split(const double* V, int M, int N, long long ldv, int Kp, double scale,
