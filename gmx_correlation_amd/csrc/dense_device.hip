@@ -11,6 +11,8 @@
 // separate FP32 accumulators.  The per-pair log-determinants, the entropies
 // and the copula's inverse normal stay in FP64.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "dense_internal.h"
@@ -120,51 +122,165 @@ __global__ void dense_split_kernel(const double* __restrict__ V, int N, long lon
 }
 
 // ---------------------------------------------------------------------------
+// Split-precision second-moment GEMM on the matrix cores.
 // C = (hi.hi + (hi.lo + lo.hi) 2^-11) * out_scale over the upper-triangle
-// 64x64 tiles (ti <= tj), mirrored into the lower triangle.  Block = 4 waves,
-// wave w owns the 32x32 sub-tile (w/2, w%2) and issues
-// v_mfma_f32_32x32x16_f16: lane l supplies row l%32 and the 8 consecutive
-// frames 8*(l/32).. of each 16-frame step for both operands (both read the
-// component-major rows, since B[k][j] = V[k][j] = rows[j][k]); the result
-// register q of lane l is row 8*(q/4) + 4*(l/32) + q%4, column l%32.
+// 128x128 tiles (ti <= tj), K split into nsplit ranges whose partial tiles go
+// to Cp[split] (mirrored into the lower triangle off the diagonal);
+// dense_cov_reduce_kernel sums them in split order.
+// Block = 4 waves; wave w owns the 64x64 sub-tile (w/2, w%2) as 2x2 blocks of
+// v_mfma_f32_32x32x16_f16.  Each 32-frame step of the four operand panels
+// (A hi/lo, B hi/lo: 128 components x 32 frames) is staged in LDS, double
+// buffered, one barrier per step; the next step's panels are loaded into
+// registers while the current one is multiplied.  MFMA operands: lane l
+// supplies component l%32 and the 8 consecutive frames 8*(l/32).. of each
+// 16-frame step (both operands read the component-major rows, since
+// B[k][j] = V[k][j] = rows[j][k]); result register q of lane l is row
+// 8*(q/4) + 4*(l/32) + q%4, column l%32 of the 32x32 block.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void dense_cov_split_kernel(const _Float16* __restrict__ hi,
-                                                              const _Float16* __restrict__ lo,
-                                                              int M, int Kp, int ntile,
-                                                              float out_scale,
-                                                              float* __restrict__ C)
+constexpr int kGTile = 128;            // block tile (components)
+constexpr int kGStep = 32;             // frames per staged step
+constexpr int kGPitch = kGStep + 8;    // LDS row pitch in halves (80 B: no bank conflicts)
+constexpr int kGPanel = kGTile * kGPitch;
+
+__global__ __launch_bounds__(256, 2) void dense_cov_tile_kernel(const _Float16* __restrict__ hi,
+                                                             const _Float16* __restrict__ lo,
+                                                             int M, int Kp, int ntile, int nsplit,
+                                                             float out_scale,
+                                                             float* __restrict__ Cp)
 {
-    // triangular decode of the tile pair
-    int t = blockIdx.x, ti = 0;
+    __shared__ __attribute__((aligned(16))) _Float16 sm[2][4][kGPanel];
+    int t = blockIdx.x / nsplit;
+    const int split = blockIdx.x % nsplit;
+    int ti = 0;
     while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
     const int tj = ti + t;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int row0 = ti * 64 + (w >> 1) * 32, col0 = tj * 64 + (w & 1) * 32;
-    const int r = lane & 31, kq = (lane >> 5) * 8;
-    const int arow = min(row0 + r, M - 1), bcol = min(col0 + r, M - 1);
-    const _Float16* ah = hi + (size_t)arow * Kp + kq;
-    const _Float16* al = lo + (size_t)arow * Kp + kq;
-    const _Float16* bh = hi + (size_t)bcol * Kp + kq;
-    const _Float16* bl = lo + (size_t)bcol * Kp + kq;
-    float16v acc_hh = {}, acc_x = {};
-    for (int k0 = 0; k0 < Kp; k0 += 16) {
-        const half8 vah = *reinterpret_cast<const half8*>(ah + k0);
-        const half8 val = *reinterpret_cast<const half8*>(al + k0);
-        const half8 vbh = *reinterpret_cast<const half8*>(bh + k0);
-        const half8 vbl = *reinterpret_cast<const half8*>(bl + k0);
-        acc_hh = __builtin_amdgcn_mfma_f32_32x32x16_f16(vah, vbh, acc_hh, 0, 0, 0);
-        acc_x = __builtin_amdgcn_mfma_f32_32x32x16_f16(vah, vbl, acc_x, 0, 0, 0);
-        acc_x = __builtin_amdgcn_mfma_f32_32x32x16_f16(val, vbh, acc_x, 0, 0, 0);
-    }
-    const int j = col0 + (lane & 31);
+    const int nsteps = Kp / kGStep;
+    const int s0 = (int)((long long)nsteps * split / nsplit);
+    const int s1 = (int)((long long)nsteps * (split + 1) / nsplit);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // loader: chunks c = tid, tid + 256 of each panel, chunk c = component
+    // c/4, frames 8*(c%4).. of the step
+    // element offsets into hi/lo (M * Kp < 2^31 is checked by the launcher)
+    uint32_t aoff[2], boff[2];
+    int soff[2];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int i = row0 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-        if (i < M && j < M) {
-            const float v = (acc_hh[q] + acc_x[q] * (1.0f / 2048.0f)) * out_scale;
-            C[(size_t)i * M + j] = v;
-            C[(size_t)j * M + i] = v;
-        }
+    for (int h = 0; h < 2; ++h) {
+        const int c = tid + 256 * h;
+        const int ra = min(ti * kGTile + c / 4, M - 1), rb = min(tj * kGTile + c / 4, M - 1);
+        aoff[h] = (uint32_t)ra * (uint32_t)Kp + 8u * (c % 4);
+        boff[h] = (uint32_t)rb * (uint32_t)Kp + 8u * (c % 4);
+        soff[h] = (c / 4) * kGPitch + 8 * (c % 4);
+    }
+    const int wr = (w >> 1) * 64, wc = (w & 1) * 64;   // wave sub-tile in the block tile
+    const int r = lane & 31, kq = (lane >> 5) * 8;
+    float16v h00 = {}, h01 = {}, h10 = {}, h11 = {}, x00 = {}, x01 = {}, x10 = {}, x11 = {};
+    // registers of the next step's panels: (A hi, A lo, B hi, B lo) x (chunk 0, 1)
+    uint4 g0, g1, g2, g3, g4, g5, g6, g7;
+#define GMX_FETCH(step)                                                                  \
+    do {                                                                                 \
+        const uint32_t k_ = (uint32_t)(step) * kGStep;                                   \
+        g0 = *reinterpret_cast<const uint4*>(hi + aoff[0] + k_);                         \
+        g1 = *reinterpret_cast<const uint4*>(hi + aoff[1] + k_);                         \
+        g2 = *reinterpret_cast<const uint4*>(lo + aoff[0] + k_);                         \
+        g3 = *reinterpret_cast<const uint4*>(lo + aoff[1] + k_);                         \
+        g4 = *reinterpret_cast<const uint4*>(hi + boff[0] + k_);                         \
+        g5 = *reinterpret_cast<const uint4*>(hi + boff[1] + k_);                         \
+        g6 = *reinterpret_cast<const uint4*>(lo + boff[0] + k_);                         \
+        g7 = *reinterpret_cast<const uint4*>(lo + boff[1] + k_);                         \
+    } while (0)
+#define GMX_STAGE(buf)                                                                   \
+    do {                                                                                 \
+        *reinterpret_cast<uint4*>(&sm[buf][0][soff[0]]) = g0;                            \
+        *reinterpret_cast<uint4*>(&sm[buf][0][soff[1]]) = g1;                            \
+        *reinterpret_cast<uint4*>(&sm[buf][1][soff[0]]) = g2;                            \
+        *reinterpret_cast<uint4*>(&sm[buf][1][soff[1]]) = g3;                            \
+        *reinterpret_cast<uint4*>(&sm[buf][2][soff[0]]) = g4;                            \
+        *reinterpret_cast<uint4*>(&sm[buf][2][soff[1]]) = g5;                            \
+        *reinterpret_cast<uint4*>(&sm[buf][3][soff[0]]) = g6;                            \
+        *reinterpret_cast<uint4*>(&sm[buf][3][soff[1]]) = g7;                            \
+    } while (0)
+#define GMX_STEP(buf)                                                                    \
+    do {                                                                                 \
+        const _Float16* pa = sm[buf][0];                                                 \
+        _Pragma("unroll") for (int k16 = 0; k16 < kGStep; k16 += 16)                     \
+        {                                                                                \
+            const int ao = (wr + r) * kGPitch + k16 + kq;                                \
+            const int bo = (wc + r) * kGPitch + k16 + kq;                                \
+            const half8 ah0 = *reinterpret_cast<const half8*>(pa + ao);                  \
+            const half8 ah1 = *reinterpret_cast<const half8*>(pa + ao + 32 * kGPitch);   \
+            const half8 al0 = *reinterpret_cast<const half8*>(pa + kGPanel + ao);        \
+            const half8 al1 =                                                            \
+                *reinterpret_cast<const half8*>(pa + kGPanel + ao + 32 * kGPitch);       \
+            const half8 bh0 = *reinterpret_cast<const half8*>(pa + 2 * kGPanel + bo);    \
+            const half8 bh1 =                                                            \
+                *reinterpret_cast<const half8*>(pa + 2 * kGPanel + bo + 32 * kGPitch);   \
+            const half8 bl0 = *reinterpret_cast<const half8*>(pa + 3 * kGPanel + bo);    \
+            const half8 bl1 =                                                            \
+                *reinterpret_cast<const half8*>(pa + 3 * kGPanel + bo + 32 * kGPitch);   \
+            h00 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bh0, h00, 0, 0, 0);        \
+            h01 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bh1, h01, 0, 0, 0);        \
+            h10 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bh0, h10, 0, 0, 0);        \
+            h11 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bh1, h11, 0, 0, 0);        \
+            x00 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bl0, x00, 0, 0, 0);        \
+            x01 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bl1, x01, 0, 0, 0);        \
+            x10 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bl0, x10, 0, 0, 0);        \
+            x11 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bl1, x11, 0, 0, 0);        \
+            x00 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al0, bh0, x00, 0, 0, 0);        \
+            x01 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al0, bh1, x01, 0, 0, 0);        \
+            x10 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al1, bh0, x10, 0, 0, 0);        \
+            x11 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al1, bh1, x11, 0, 0, 0);        \
+        }                                                                                \
+    } while (0)
+    GMX_FETCH(s0);
+    GMX_STAGE(0);
+    __syncthreads();
+    // steps s0 .. s1-2 stage the next step while multiplying; the last one
+    // only multiplies
+    for (int step = s0; step < s1 - 1; ++step) {
+        const int cur = (step - s0) & 1;
+        GMX_FETCH(step + 1);
+        GMX_STEP(cur);
+        GMX_STAGE(cur ^ 1);
+        __syncthreads();
+    }
+    GMX_STEP((s1 - 1 - s0) & 1);
+#undef GMX_STEP
+#undef GMX_FETCH
+#undef GMX_STAGE
+    float* C = Cp + (size_t)split * M * M;
+    const bool mirror = ti != tj;
+    // one 32x32 block at a time: result register q of lane l is row
+    // 8*(q/4) + 4*(l/32) + q%4, column l%32
+#define GMX_STORE(H, X, U, V)                                                            \
+    do {                                                                                 \
+        const int j_ = tj * kGTile + wc + 32 * (V) + (lane & 31);                        \
+        _Pragma("unroll") for (int q = 0; q < 16; ++q)                                   \
+        {                                                                                \
+            const int i_ = ti * kGTile + wr + 32 * (U) + 8 * (q >> 2) + 4 * (lane >> 5) + \
+                           (q & 3);                                                      \
+            if (i_ < M && j_ < M) {                                                      \
+                const float val = ((H)[q] + (X)[q] * (1.0f / 2048.0f)) * out_scale;      \
+                C[(size_t)i_ * M + j_] = val;                                            \
+                if (mirror) C[(size_t)j_ * M + i_] = val;                                \
+            }                                                                            \
+        }                                                                                \
+    } while (0)
+    GMX_STORE(h00, x00, 0, 0);
+    GMX_STORE(h01, x01, 0, 1);
+    GMX_STORE(h10, x10, 1, 0);
+    GMX_STORE(h11, x11, 1, 1);
+#undef GMX_STORE
+}
+
+// C[e] = sum over splits of Cp[s][e], in split order.
+__global__ void dense_cov_reduce_kernel(const float* __restrict__ Cp, long long n, int nsplit,
+                                        float* __restrict__ C)
+{
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        float v = Cp[e];
+        for (int s = 1; s < nsplit; ++s) v += Cp[(size_t)s * n + e];
+        C[e] = v;
     }
 }
 
@@ -346,13 +462,28 @@ hipError_t launch_dense_split(const double* V, int M, int N, long long ldv, int 
     return hipGetLastError();
 }
 
-hipError_t launch_dense_cov(const _Float16* hi, const _Float16* lo, int M, int Kp,
-                            float out_scale, float* C, hipStream_t s)
+int dense_cov_splits(int M, int Kp)
 {
-    const int ntile = (M + 63) / 64;
-    const unsigned nblk = (unsigned)(ntile * (ntile + 1) / 2);
-    hipLaunchKernelGGL(dense_cov_split_kernel, dim3(nblk), dim3(256), 0, s, hi, lo, M, Kp, ntile,
-                       out_scale, C);
+    const long long ntile = (M + kGTile - 1) / kGTile;
+    const long long tiles = ntile * (ntile + 1) / 2;
+    int ns = (int)((1024 + tiles - 1) / tiles);
+    ns = std::min(ns, std::min(8, std::max(1, Kp / kGStep / 4)));
+    return std::max(ns, 1);
+}
+
+hipError_t launch_dense_cov(const _Float16* hi, const _Float16* lo, int M, int Kp,
+                            float out_scale, float* Cp, int nsplit, float* C, hipStream_t s)
+{
+    if (Kp % kGStep != 0 || nsplit < 1 || Kp / kGStep < nsplit ||
+        (long long)M * Kp >= (1LL << 31))
+        return hipErrorInvalidValue;
+    const int ntile = (M + kGTile - 1) / kGTile;
+    const unsigned nblk = (unsigned)(ntile * (ntile + 1) / 2 * nsplit);
+    hipLaunchKernelGGL(dense_cov_tile_kernel, dim3(nblk), dim3(256), 0, s, hi, lo, M, Kp, ntile,
+                       nsplit, out_scale, Cp);
+    if (nsplit == 1 && Cp == C) return hipGetLastError();
+    const long long n = (long long)M * M;
+    hipLaunchKernelGGL(dense_cov_reduce_kernel, dim3(1024), dim3(256), 0, s, Cp, n, nsplit, C);
     return hipGetLastError();
 }
 

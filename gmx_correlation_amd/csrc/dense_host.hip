@@ -30,6 +30,7 @@ struct DenseBuffers {
     _Float16* hi = nullptr;
     _Float16* lo = nullptr;
     float* C = nullptr;
+    float* Cp = nullptr;   // split-K partial matrices (nullptr: one split into C)
     double* mat = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t ev[6] = {};
@@ -37,7 +38,7 @@ struct DenseBuffers {
     {
         if (s) (void)hipStreamSynchronize(s);
         for (void* p : {(void*)traj, (void*)rows, (void*)z, (void*)hi, (void*)lo, (void*)C,
-                        (void*)mat})
+                        (void*)Cp, (void*)mat})
             if (p) (void)hipFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -77,6 +78,8 @@ int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* 
     HIPCHK(hipMalloc(&b.hi, (size_t)M * Kp * sizeof(_Float16)));
     HIPCHK(hipMalloc(&b.lo, (size_t)M * Kp * sizeof(_Float16)));
     HIPCHK(hipMalloc(&b.C, (size_t)M * M * sizeof(float)));
+    const int nsplit = gmxcorr::dense_cov_splits(M, Kp);
+    if (nsplit > 1) HIPCHK(hipMalloc(&b.Cp, (size_t)nsplit * M * M * sizeof(float)));
     HIPCHK(hipMalloc(&b.mat, (size_t)natoms * natoms * sizeof(double)));
 
     // power-of-two scale putting the largest |value| near 2^10 for FP16
@@ -102,7 +105,8 @@ int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* 
     HIPCHK(hipEventRecord(b.ev[2], b.s));
     HIPCHK(gmxcorr::launch_dense_split(V, M, N, N, Kp, scale, b.hi, b.lo, b.s));
     HIPCHK(hipEventRecord(b.ev[3], b.s));
-    HIPCHK(gmxcorr::launch_dense_cov(b.hi, b.lo, M, Kp, out_scale, b.C, b.s));
+    HIPCHK(gmxcorr::launch_dense_cov(b.hi, b.lo, M, Kp, out_scale, nsplit > 1 ? b.Cp : b.C, nsplit,
+                                     b.C, b.s));
     HIPCHK(hipEventRecord(b.ev[4], b.s));
     HIPCHK(gmxcorr::launch_dense_pairs(b.C, natoms, mode, b.mat, b.s));
     HIPCHK(hipEventRecord(b.ev[5], b.s));

@@ -78,13 +78,15 @@ int gmxcorr_ksg_plan_features(gmxcorr_ksg_plan* plan, long long npairs, const in
     HIPCHK(hipEventCreate(&e1));
     DevBuf<double> Vc, kurt;
     DevBuf<_Float16> hi, lo;
-    DevBuf<float> C, rad, fk, q50, q90, dfeat;
+    DevBuf<float> C, Cp, rad, fk, q50, q90, dfeat;
     DevBuf<int> d1, d2;
     HIPCHK(Vc.alloc((size_t)M * N));
     HIPCHK(kurt.alloc(M));
     HIPCHK(hi.alloc((size_t)M * Kp));
     HIPCHK(lo.alloc((size_t)M * Kp));
     HIPCHK(C.alloc((size_t)M * M));
+    const int nsplit = gmxcorr::dense_cov_splits(M, Kp);
+    if (nsplit > 1) HIPCHK(Cp.alloc((size_t)nsplit * M * M));
     HIPCHK(rad.alloc((size_t)n * nsample));
     HIPCHK(fk.alloc(n));
     HIPCHK(q50.alloc(n));
@@ -99,7 +101,8 @@ int gmxcorr_ksg_plan_features(gmxcorr_ksg_plan* plan, long long npairs, const in
     // |z| of a standardised sample is below sqrt(N); scale the FP16 split to 2^10
     const double scale = std::ldexp(1.0, 9 - std::ilogb(std::sqrt((double)N) + 1.0));
     HIPCHK(gmxcorr::launch_dense_split(Vc.p, M, N, N, Kp, scale, hi.p, lo.p, s));
-    HIPCHK(gmxcorr::launch_dense_cov(hi.p, lo.p, M, Kp, (float)(1.0 / (scale * scale * N)), C.p, s));
+    HIPCHK(gmxcorr::launch_dense_cov(hi.p, lo.p, M, Kp, (float)(1.0 / (scale * scale * N)),
+                                     nsplit > 1 ? Cp.p : C.p, nsplit, C.p, s));
     HIPCHK(gmxcorr::launch_atom_knn_radius(pv.D, n, N, pv.P, pv.K, nsample, rad.p, s));
     std::vector<double> hk(M);
     std::vector<float> hr((size_t)n * nsample);
