@@ -1096,9 +1096,11 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
     const long long item = xcd_pair(nquads * kRowSplit);
     if (item >= nquads * kRowSplit) return;
-    const long long q = item / kRowSplit;
+    // row-range major: the blocks in flight on one XCD are consecutive tiles
+    // (sharing their first two atoms) over the same row range
+    const long long q = item % nquads;
     const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
-    const int part = (int)(item % kRowSplit);
+    const int part = (int)(item / nquads);
     if (p < 0 || p >= args.npairs) return;   // no block barriers below
     const int N = args.N, T = args.K + 1;
     const int i0 = (int)((long long)N * part / kRowSplit);

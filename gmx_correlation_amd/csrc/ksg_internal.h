@@ -103,7 +103,7 @@ constexpr int kMaxFrames = 8192;
 
 // The single-trajectory pair kernels write kRowSplit partial sums per pair
 // (frame rows split into that many ranges); partial holds npairs*kRowSplit.
-constexpr int kRowSplit = 4;
+constexpr int kRowSplit = 16;
 
 // Distance-row pitch in floats: a multiple of 256 frames, so the single-wave
 // kernels read whole 64-lane chunks of 4 frames without a per-lane bound.
