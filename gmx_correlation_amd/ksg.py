@@ -204,3 +204,63 @@ def read_matrix(path: str) -> np.ndarray:
     if len(vals) != n * n:
         raise ValueError(f"expected {n * n} values, found {len(vals)}")
     return np.array(vals).reshape(n, n).T.copy()
+
+
+def write_xpm_matrix(mat: np.ndarray, path: str, title: str, legend: str) -> None:
+    """XPM image of the matrix (lib/correlation_core.cpp:227-278): 80 grey
+    levels '!'..'p' spanning [min, max] of the off-diagonal values (0..1 when
+    they are all equal), white at the minimum; row i of the image holds
+    mat[j*n+i] for j, as correl.dat does."""
+    m = np.asarray(mat, dtype=np.float64)
+    n = m.shape[0]
+    off = m.T[~np.eye(n, dtype=bool)] if n > 1 else np.array([])
+    lo = float(off.min()) if off.size else 1e100
+    hi = float(off.max()) if off.size else -1e100
+    if lo >= hi:
+        lo, hi = 0.0, 1.0
+    nlev = 80
+    out = ["/* XPM */\nstatic char *gmx_correlation_xpm[] = {\n", f'"{n} {n} {nlev} 1",\n']
+    for lev in range(nlev):
+        shade = 255 - int(255.0 * lev / (nlev - 1))
+        out.append('"%c c #%02x%02x%02x",\n' % (33 + lev, shade, shade, shade))
+    out.append(f"/* title: {title} */\n/* legend: {legend} */\n")
+    for i in range(n):
+        # truncation toward zero of the C cast, then the clamp
+        lev = np.trunc((m[:, i] - lo) / (hi - lo) * (nlev - 1))
+        lev = np.clip(lev, 0, nlev - 1).astype(int)
+        out.append('"' + "".join(chr(33 + int(v)) for v in lev) + '"' + ("" if i + 1 == n else ",") + "\n")
+    out.append("};\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+def write_dump(traj: np.ndarray, path: str) -> None:
+    """The -dump text of the mean-centred trajectory
+    (src/gmx_correlation.cpp:225-244): "natoms nframes", then per frame, per
+    atom, "%.10g %.10g %.10g"."""
+    t = np.asarray(traj, dtype=np.float64)
+    natoms, nframes = t.shape[0], t.shape[1]
+    lines = [f"{natoms} {nframes}\n"]
+    for fr in range(nframes):
+        for a in range(natoms):
+            x = t[a, fr]
+            lines.append("%.10g %.10g %.10g\n" % (x[0], x[1], x[2]))
+    with open(path, "w") as f:
+        f.write("".join(lines))
+
+
+def read_dump(path: str) -> np.ndarray:
+    """Inverse of write_dump: t_traj.x as [natoms][nframes][3] float64, so a
+    trajectory dumped by the reference tool feeds kraskov_prepare and the GPU
+    path without GROMACS.  Raises ValueError on a malformed file."""
+    with open(path) as f:
+        head = f.readline().split()
+        if len(head) != 2:
+            raise ValueError(f"{path}: first line must be 'natoms nframes'")
+        natoms, nframes = int(head[0]), int(head[1])
+        if natoms < 1 or nframes < 1:
+            raise ValueError(f"{path}: natoms and nframes must be positive")
+        vals = np.array(f.read().split(), dtype=np.float64)
+    if vals.size != natoms * nframes * 3:
+        raise ValueError(f"{path}: expected {natoms * nframes * 3} values, found {vals.size}")
+    return np.ascontiguousarray(vals.reshape(nframes, natoms, 3).transpose(1, 0, 2))

@@ -101,6 +101,7 @@ def ref():
         R.ref_te_matrix.argtypes = [_fp, _i, _i, _i, _i, _i, _dp]
         R.ref_pearsify.argtypes = [_dp, _i]
         R.ref_write_matrix.argtypes = [_dp, _i, ctypes.c_char_p]
+        R.ref_write_xpm.argtypes = [_dp, _i, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
         _ref = R
     return _ref
 
@@ -317,3 +318,8 @@ def ref_te_matrix(coords, k, lag, nthreads=0):
 def ref_write_matrix(mat, path):
     m = np.ascontiguousarray(mat, dtype=np.float64)
     ref().ref_write_matrix(_ptr(m, _dp), m.shape[0], path.encode())
+
+
+def ref_write_xpm(mat, path, title, legend):
+    m = np.ascontiguousarray(mat, dtype=np.float64)
+    ref().ref_write_xpm(_ptr(m, _dp), m.shape[0], path.encode(), title.encode(), legend.encode())

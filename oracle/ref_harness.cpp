@@ -134,6 +134,12 @@ void ref_pearsify(double* mat, int n) { pearsify(mat, n, DIM); }
 
 void ref_write_matrix(const double* mat, int n, const char* path) { write_matrix(mat, n, path); }
 
+void ref_write_xpm(const double* mat, int n, const char* path, const char* title,
+                   const char* legend)
+{
+    write_xpm_matrix(mat, n, path, title, legend);
+}
+
 } // extern "C"
 
 #include <omp.h>

@@ -72,6 +72,14 @@ def main() -> None:
     m = O.pearsify(O.ref_ksg_matrix(c0[:, :8], 10))
     path = os.path.join(OUT, "small8_correl.dat")
     O.ref_write_matrix(m, path)
+    # the same r(MI) matrix as XPM (write_xpm_matrix), and a 5-atom matrix
+    # whose off-diagonal values are all equal (the writer's 0..1 fallback)
+    O.ref_write_xpm(m, os.path.join(OUT, "small8_correl.xpm"), "C-alpha correlation matrix",
+                    "r(MI)")
+    flat = np.full((5, 5), 0.25)
+    np.fill_diagonal(flat, 1.0)
+    O.ref_write_xpm(flat, os.path.join(OUT, "flat5.xpm"), "flat correlation matrix",
+                    "MI (bits)")
 
     if not args.skip_large:
         c1 = synth.generate(synth.README_247)

@@ -104,3 +104,63 @@ def test_batched_specs_deterministic():
     assert a == b and len(a) == 32
     assert all(50 <= s.natoms <= 600 and 1000 <= s.nframes <= 8000 and k in (10, 50, 100)
                for s, k in a)
+
+
+@pytest.mark.parametrize("case", ["small8", "flat5"])
+def test_write_xpm_byte_exact_vs_reference(tmp_path, oracle_lib, case):
+    """XPM written by the reference's write_xpm_matrix (fixtures) and by the
+    mirror from the same matrix are identical files; flat5 has equal
+    off-diagonal values (the writer's 0..1 range fallback)."""
+    if case == "small8":
+        c = synth.generate(synth.SMALL_CPU)[:, :8]
+        rows, psi = ksg.kraskov_prepare(ksg.center(c))
+        m = oracle_lib.pearsify(oracle_lib.ksg_matrix(rows, psi, 8, 10))
+        title, legend, fixture = "C-alpha correlation matrix", "r(MI)", "small8_correl.xpm"
+    else:
+        m = np.full((5, 5), 0.25)
+        np.fill_diagonal(m, 1.0)
+        title, legend, fixture = "flat correlation matrix", "MI (bits)", "flat5.xpm"
+    path = tmp_path / "out.xpm"
+    ksg.write_xpm_matrix(m, str(path), title, legend)
+    assert path.read_bytes() == open(os.path.join(GOLDEN, fixture), "rb").read()
+
+
+def test_dump_format_and_round_trip(tmp_path):
+    """-dump text (src/gmx_correlation.cpp:225-244): header, then frame-major
+    lines of an atom's three fluctuations in %.10g; read_dump inverts it."""
+    t = np.array([[[0.1, -2.0, 1.0 / 3.0], [1e-12, 5.0, -0.25]],
+                  [[7.0, 8.0, 9.0], [-1.5, 0.0, 123456789.123]]])   # [natoms=2][nframes=2][3]
+    path = tmp_path / "traj.dump"
+    ksg.write_dump(t, str(path))
+    assert path.read_text().splitlines() == [
+        "2 2",
+        "0.1 -2 0.3333333333",
+        "7 8 9",
+        "1e-12 5 -0.25",
+        "-1.5 0 123456789.1",
+    ]
+    back = ksg.read_dump(str(path))
+    assert back.shape == (2, 2, 3)
+    assert np.array_equal(back, np.vectorize(lambda v: float("%.10g" % v))(t))
+
+
+def test_dump_feeds_prepare(tmp_path):
+    """A dumped trajectory read back gives the same normalised rows to the
+    precision of the text (10 significant digits)."""
+    traj = ksg.center(synth.generate(synth.SMALL_CPU)[:, :6])
+    path = tmp_path / "small.dump"
+    ksg.write_dump(traj, str(path))
+    rows_a, psi_a = ksg.kraskov_prepare(traj)
+    rows_b, psi_b = ksg.kraskov_prepare(ksg.read_dump(str(path)))
+    assert np.array_equal(psi_a, psi_b)
+    assert np.max(np.abs(rows_a - rows_b)) < 1e-7
+
+
+def test_read_dump_rejects_malformed(tmp_path):
+    p = tmp_path / "bad.dump"
+    p.write_text("2 3\n1 2 3\n")
+    with pytest.raises(ValueError):
+        ksg.read_dump(str(p))
+    p.write_text("oops\n")
+    with pytest.raises(ValueError):
+        ksg.read_dump(str(p))
