@@ -124,6 +124,36 @@ def bench_dense(args) -> int:
     print(json.dumps(out), flush=True)
     return 0
 
+def bench_noise(args) -> int:
+    """Sampling noise of the exact KSG r(MI) itself: two replicas of one
+    200-atom system (same chain, mode loadings and coupled pairs; independent
+    frames) at 5000 frames, k = 100, the frame count and k of the surrogate's
+    inference set.  The spread of r between replicas is the estimator's own
+    noise, which no predictor built from a pair's distribution can follow."""
+    from gmx_correlation_amd import ksg, surrogate, synth
+    k, n, T = 100, 200, 5000
+    ns, nr = synth.coupled_counts(n)
+    rs = []
+    for fs in (1, 2):
+        spec = synth.TrajSpec(natoms=n, nframes=T, nmodes=20, seed=33, n_switch=ns, n_rotor=nr,
+                              frame_seed=fs)
+        log(f"bench: replica {spec}")
+        rows, psi = ksg.kraskov_prepare(ksg.center(synth.generate(spec)))
+        with ksg.KsgPlan(n, T, k) as plan:
+            plan.load(rows, psi)
+            rs.append(surrogate.mi_to_r(plan.run_all()))
+    d = rs[0] - rs[1]
+    ad = np.abs(d)
+    out = {"workload": f"KSG r(MI) noise: 2 replicas of {n} atoms x {T} frames, k={k}",
+           "pairs": int(len(d)), "r_mean": float(np.mean(rs[0])),
+           "sigma_r_per_replica": float(np.std(d) / np.sqrt(2.0)),
+           "abs_dr_between_replicas_p50_p90_p99_max": [float(np.percentile(ad, q))
+                                                       for q in (50, 90, 99)] + [float(ad.max())],
+           "frac_pairs_abs_dr_le_1e-3": float(np.mean(ad <= 1e-3))}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_hybrid(args) -> int:
     """Surrogate + KSG fallback: train on the exact r(MI) of 500 atoms x 4000
     frames (k=100), run on 2000 atoms x 5000 frames (k=100).  The plain GPU
@@ -308,7 +338,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU-baseline work (0 disables)")
-    ap.add_argument("--workload", default="ksg", choices=["ksg", "dense", "hybrid", "batched", "te"],
+    ap.add_argument("--workload", default="ksg",
+                    choices=["ksg", "dense", "hybrid", "batched", "te", "noise"],
                     help="ksg: the headline metric; dense: Gaussian/GCMI GEMM path; "
                          "hybrid: surrogate + KSG fallback; batched: uneven trajectories")
     ap.add_argument("--epochs", type=int, default=120, help="surrogate training epochs")
@@ -329,6 +360,8 @@ def main() -> int:
         return bench_batched(args)
     if args.workload == "te":
         return bench_te(args)
+    if args.workload == "noise":
+        return bench_noise(args)
 
     from gmx_correlation_amd import _lib, ksg, synth
 

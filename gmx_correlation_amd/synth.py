@@ -39,6 +39,11 @@ class TrajSpec:
     n_switch: int = 0
     n_rotor: int = 0
     student_nu: float | None = None
+    # None: one stream for everything.  Otherwise the per-frame draws (mode
+    # amplitudes, noise, switch states, rotor angles) come from this seed and
+    # the structure (chain, loadings, coupled pairs) from `seed`: replicas of
+    # one system with independent frames.
+    frame_seed: int | None = None
 
 
 def _unit_vectors(rng: np.random.Generator, n: int) -> np.ndarray:
@@ -63,12 +68,13 @@ def generate(spec: TrajSpec) -> np.ndarray:
     mean = np.concatenate([np.zeros((1, 3)), np.cumsum(steps, axis=0)], axis=0)
     sig = np.linspace(0.05, 0.3, M) if M > 1 else np.array([0.05])
     loads = np.stack([_unit_vectors(rng, n) for _ in range(M)])       # [M][n][3]
-    amps = rng.standard_normal((T, M)) * sig[None, :]                  # [T][M]
+    fr = rng if spec.frame_seed is None else np.random.Generator(np.random.PCG64(spec.frame_seed))
+    amps = fr.standard_normal((T, M)) * sig[None, :]                   # [T][M]
     disp = np.einsum("tm,mnd->tnd", amps, loads)
     if spec.student_nu is None:
-        disp += rng.standard_normal((T, n, 3)) * 0.05
+        disp += fr.standard_normal((T, n, 3)) * 0.05
     else:
-        disp += rng.standard_t(spec.student_nu, size=(T, n, 3)) * 0.05
+        disp += fr.standard_t(spec.student_nu, size=(T, n, 3)) * 0.05
     pos = mean[None, :, :] + disp
 
     need = 2 * (spec.n_switch + spec.n_rotor)
@@ -80,14 +86,14 @@ def generate(spec: TrajSpec) -> np.ndarray:
         rot = perm[2 * spec.n_switch:].reshape(-1, 2)
         for a, b in sw:
             direction = _unit_vectors(rng, 1)[0]
-            s = (rng.random(T) < 0.3).astype(np.float64)
+            s = (fr.random(T) < 0.3).astype(np.float64)
             shift = 0.4 * s[:, None] * direction[None, :]
             pos[:, a, :] += shift
             pos[:, b, :] += shift
         for a, b in rot:
             axis = _unit_vectors(rng, 1)[0]
-            sign = np.where(rng.random(T) < 0.5, -1.0, 1.0)
-            theta = 0.5 * rng.normal(0.6 * sign, 0.1)
+            sign = np.where(fr.random(T) < 0.5, -1.0, 1.0)
+            theta = 0.5 * fr.normal(0.6 * sign, 0.1)
             mid = 0.5 * (pos[:, a, :] + pos[:, b, :])
             pos[:, a, :] = mid + _rotate(pos[:, a, :] - mid, axis, theta)
             pos[:, b, :] = mid + _rotate(pos[:, b, :] - mid, axis, theta)
