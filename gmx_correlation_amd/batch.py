@@ -21,9 +21,11 @@ from .ksg import KsgPlan
 # Code paths compiled into the persistent batch kernel: (kind, s4) with
 # kind 0 = one wave per frame row streaming both distance rows (N <= 256*s4),
 # kind 1 = four waves per row (N <= 1024*s4), kind 2 = one wave per row over
-# the 512 nearest frames of the sparser atom (N <= 2048, k + 1 <= 256).
-BATCH_PATHS = [(2, 1), (0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (1, 4), (1, 6), (1, 8)]
+# the 512 nearest frames of the sparser atom (N <= 2048, k + 1 <= 256), kind 3
+# = the same over 1024-frame prefix stores (2048 < N <= 8192, k + 1 <= 256).
+BATCH_PATHS = [(2, 1), (3, 1), (0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (1, 4), (1, 6), (1, 8)]
 PREFIX_MAX_FRAMES = 2048
+LONG_MAX_FRAMES = 8192
 PREFIX_MAX_T = 256
 
 
@@ -31,6 +33,8 @@ def fitting_paths(nframes: int, k: int) -> list[tuple[int, int]]:
     def fits(kind: int, s4: int) -> bool:
         if kind == 2:
             return nframes <= PREFIX_MAX_FRAMES and k + 1 <= PREFIX_MAX_T
+        if kind == 3:
+            return PREFIX_MAX_FRAMES < nframes <= LONG_MAX_FRAMES and k + 1 <= PREFIX_MAX_T
         return nframes <= (256 if kind == 0 else 1024) * s4
     return [(kd, s) for kd, s in BATCH_PATHS if fits(kd, s)]
 

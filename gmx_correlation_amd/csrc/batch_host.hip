@@ -44,12 +44,17 @@ struct HostTraj {
     long long sd_bytes() const { return sorted() ? (long long)natoms * N * gmxcorr::sorted_pitch(N) * 4 : 0; }
     long long sm_bytes() const { return sorted() ? (long long)natoms * N * gmxcorr::kSumPitch * 4 : 0; }
     long long si_bytes() const { return sorted() ? (long long)natoms * N * gmxcorr::kPrefix * 2 : 0; }
+    bool stored() const { return gmxcorr::variant_prefix_store(kind); }
+    long long lv_bytes() const { return stored() ? (long long)natoms * N * gmxcorr::kLongPitch * 4 : 0; }
+    long long li_bytes() const { return stored() ? (long long)natoms * N * gmxcorr::kLongPrefix * 2 : 0; }
+    long long ls_bytes() const { return stored() ? (long long)natoms * N * gmxcorr::kLongSum * 4 : 0; }
     // device bytes of the distance rows, sorted rows, summaries and prefix
     // indices (256-aligned parts)
     long long row_bytes() const
     {
         auto al = [](long long v) { return (v + 255) / 256 * 256; };
-        return al(d_bytes()) + al(sd_bytes()) + al(sm_bytes()) + al(si_bytes());
+        return al(d_bytes()) + al(sd_bytes()) + al(sm_bytes()) + al(si_bytes()) + al(lv_bytes()) +
+               al(li_bytes()) + al(ls_bytes());
     }
     long long npairs() const { return (long long)natoms * (natoms - 1) / 2; }
 };
@@ -94,6 +99,7 @@ int gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const doubl
         static const int coop[] = {3, 4, 6, 8};
         kind = -1;
         if (gmxcorr::batch_variant_ok(2, 1, nframes, k)) { kind = 2; s4 = 1; }
+        if (kind < 0 && gmxcorr::batch_variant_ok(3, 1, nframes, k)) { kind = 3; s4 = 1; }
         if (kind < 0)
             for (int v : single)
                 if (nframes <= 256 * v) { kind = 0; s4 = v; break; }
@@ -222,6 +228,15 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                     h.sorted() ? reinterpret_cast<uint16_t*>(dArena + doff + align(h.d_bytes()) +
                                                              align(h.sd_bytes()) + align(h.sm_bytes()))
                                : nullptr;
+                const long long loff = doff + align(h.d_bytes()) + align(h.sd_bytes()) +
+                                       align(h.sm_bytes()) + align(h.si_bytes());
+                float* dLV = h.stored() ? reinterpret_cast<float*>(dArena + loff) : nullptr;
+                uint16_t* dLI = h.stored()
+                                    ? reinterpret_cast<uint16_t*>(dArena + loff + align(h.lv_bytes()))
+                                    : nullptr;
+                float* dLS = h.stored() ? reinterpret_cast<float*>(dArena + loff + align(h.lv_bytes()) +
+                                                                   align(h.li_bytes()))
+                                        : nullptr;
                 doff += align(h.row_bytes());
                 dX[q] = reinterpret_cast<double*>(xArena + xoff);
                 xoff += align((long long)h.rows.size() * 8);
@@ -233,8 +248,10 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                                       hipMemcpyHostToDevice, s));
                 HIPCHK(gmxcorr::launch_build_rows(dX[q], h.natoms, h.N, h.P, dD[q], dSD, dSM, dSI,
                                                   s));
-                desc[q] = gmxcorr::TrajDesc{dX[q], dD[q], dSD, dSM, dSI, dPsi[q], h.P, h.N, h.K,
-                                            h.psiN_phiK, h.kind, h.s4};
+                if (h.stored())
+                    HIPCHK(gmxcorr::launch_build_prefix(dD[q], h.natoms, h.N, h.P, dLV, dLI, dLS, s));
+                desc[q] = gmxcorr::TrajDesc{dX[q], dD[q], dSD, dSM, dSI, dLV, dLI, dLS, dPsi[q],
+                                            h.P, h.N, h.K, h.psiN_phiK, h.kind, h.s4};
             }
             // items: trajectories by decreasing frame count, pairs a1 > a2 within each
             std::vector<int> tq(ts.size());

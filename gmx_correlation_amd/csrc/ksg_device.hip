@@ -318,13 +318,15 @@ __device__ __forceinline__ void clear_hist(uint32_t* hist, int lane)
 // kPrefix nearest frames of i are known in order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __restrict__ X, int N,
-                                                             long long P, float* __restrict__ D,
+                                                             long long P, long long nrows,
+                                                             float* __restrict__ D,
                                                              float* __restrict__ SD,
                                                              float* __restrict__ SM,
                                                              uint16_t* __restrict__ SI)
 {
     __shared__ uint64_t sv[kSortedMaxFrames];   // (value bits << 32) | frame
-    const long long row = blockIdx.x;
+    // grid-stride over rows: a launch's work-item count must stay below 2^32
+    for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {
     const int a = (int)(row / N), i = (int)(row % N);
     const double* x0 = X + (size_t)(3 * a) * N;
     const double* x1 = x0 + N;
@@ -354,7 +356,7 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
                     sv[4 * c + e] = ((uint64_t)__float_as_uint(v[e]) << 32) | (uint32_t)(4 * c + e);
         }
     }
-    if (!SD) return;
+    if (!SD) continue;
     int M = 64;
     while (M < N) M <<= 1;
     for (int t = nc * 4 + threadIdx.x; t < M; t += blockDim.x)
@@ -386,6 +388,55 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
     }
     for (int t = threadIdx.x; t < kPrefix; t += blockDim.x)
         SI[row * kPrefix + t] = t < N ? (uint16_t)(sv[t] & 0xffffu) : (uint16_t)0;
+    __syncthreads();   // sv is rewritten by the next row
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Prefix stores of the long-trajectory path from the distance rows: one block
+// per row (a, i), the row's (value, frame) pairs bitonic-sorted in dynamic
+// LDS (M = power of two >= N; ties in frame order), then
+//   LV[a][i][0..64) = 0, LV[a][i][64 + t] = t-th smallest value, t < kLongPrefix
+//   LI[a][i][t]     = its frame
+//   LS[a][i][m]     = LV value 64m + 63
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void ksg_build_prefix_kernel(const float* __restrict__ D, int N,
+                                                               long long P, long long nrows,
+                                                               float* __restrict__ LV,
+                                                               uint16_t* __restrict__ LI,
+                                                               float* __restrict__ LS)
+{
+    extern __shared__ uint64_t sk[];
+    // grid-stride over rows: a launch's work-item count must stay below 2^32
+    for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(D) + row * P;
+    int M = 64;
+    while (M < N) M <<= 1;
+    for (int t = threadIdx.x; t < M; t += blockDim.x)
+        sk[t] = t < N ? ((uint64_t)d[t] << 32) | (uint32_t)t : ((uint64_t)kPadBits << 32) | (uint32_t)t;
+    __syncthreads();
+    for (int k = 2; k <= M; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
+                const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
+                const uint64_t x = sk[lo], y = sk[hi];
+                if ((x > y) == ((lo & k) == 0)) {
+                    sk[lo] = y;
+                    sk[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    uint32_t* lv = reinterpret_cast<uint32_t*>(LV) + row * kLongPitch;
+    for (int t = threadIdx.x; t < kLongPitch; t += blockDim.x)
+        lv[t] = t < 64 ? 0u : (uint32_t)(sk[t - 64] >> 32);
+    for (int t = threadIdx.x; t < kLongPrefix; t += blockDim.x)
+        LI[row * kLongPrefix + t] = (uint16_t)(sk[t] & 0xffffu);
+    if (threadIdx.x < kLongSum)
+        reinterpret_cast<uint32_t*>(LS)[row * kLongSum + threadIdx.x] =
+            (uint32_t)(sk[64 * threadIdx.x + 63] >> 32);
+    __syncthreads();   // sk is rewritten by the next row
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -854,6 +905,275 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
 }
 
 // ---------------------------------------------------------------------------
+// Long-trajectory prefix rows (kind 3, kSortedMaxFrames < N <= kMaxFrames):
+// the sorted-prefix method of rows_prefix over the kLongPrefix prefix stores.
+// Lane l holds P's prefix positions 256c + 4l + e (c, e < 4, 16 per lane) and
+// gathers Q's distances of those frames straight from Q's distance row.
+// nQ = #{dQ <= epsQ} comes from Q's prefix store (its 16-value summary picks
+// a 128-value block) when epsQ is below Q's 1024th value, else from a
+// streaming count over Q's row; either count is consumed one row later.
+// ---------------------------------------------------------------------------
+struct LongRows {
+    const uint32_t* Da;    // distance rows (row i at i * P)
+    const uint32_t* Db;
+    const uint32_t* LVa;   // prefix values (row i at i * kLongPitch, value t at 64 + t)
+    const uint32_t* LVb;
+    const uint16_t* LIa;   // prefix frames (row i at i * kLongPrefix)
+    const uint16_t* LIb;
+    const uint32_t* LSa;   // prefix summaries (row i at i * kLongSum)
+    const uint32_t* LSb;
+    long long P;
+};
+
+__device__ __forceinline__ LongRows long_rows(const float* D, const float* LV, const uint16_t* LI,
+                                              const float* LS, long long P, int N, int a1, int a2)
+{
+    LongRows lr;
+    lr.P = P;
+    lr.Da = reinterpret_cast<const uint32_t*>(D + (size_t)a1 * N * P);
+    lr.Db = reinterpret_cast<const uint32_t*>(D + (size_t)a2 * N * P);
+    lr.LVa = reinterpret_cast<const uint32_t*>(LV + (size_t)a1 * N * kLongPitch);
+    lr.LVb = reinterpret_cast<const uint32_t*>(LV + (size_t)a2 * N * kLongPitch);
+    lr.LIa = LI + (size_t)a1 * N * kLongPrefix;
+    lr.LIb = LI + (size_t)a2 * N * kLongPrefix;
+    lr.LSa = reinterpret_cast<const uint32_t*>(LS + (size_t)a1 * N * kLongSum);
+    lr.LSb = reinterpret_cast<const uint32_t*>(LS + (size_t)a2 * N * kLongSum);
+    return lr;
+}
+
+__device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, int i0, int i1,
+                                            int step, long long item, uint32_t* hist, int lane,
+                                            const RowOut& o)
+{
+    // Callers pass at most kPrefix rows (the side bits cover kPrefix rows).
+    const int R = min((i1 - i0 + step - 1) / step, kPrefix);
+    const int nblk = (int)(lr.P / 256);
+    const uint32_t hbase = lds_addr(hist);
+    uint32_t lane4 = (uint32_t)lane * 4u;
+    uint32_t sidebits = 0u;
+#pragma unroll
+    for (int u = 0; u < kPrefix / kWave; ++u) {
+        const int t = lane + kWave * u;
+        if (t < R) {
+            const long long row = i0 + (long long)t * step;
+            const uint32_t sa = lr.LVa[row * kLongPitch + 64 + 255];
+            const uint32_t sb = lr.LVb[row * kLongPitch + 64 + 255];
+            sidebits |= (sa >= sb ? 1u : 0u) << u;
+        }
+    }
+    auto side_of = [&](int t) -> bool {
+        return ((uint32_t)readlane((int)sidebits, t & (kWave - 1)) >> (t >> 6)) & 1u;
+    };
+    // the next row: P prefix values and frames, SP[T-1], Q summary
+    uint4 nv[4];
+    uint2 ni[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        nv[c] = make_uint4(0u, 0u, 0u, 0u);
+        ni[c] = make_uint2(0u, 0u);
+    }
+    uint32_t nL = 0u, nls = 0xffffffffu;
+    auto issue = [&](int t) {
+        const long long row = i0 + (long long)t * step;
+        const bool sa = side_of(t);
+        const uint32_t* base = (sa ? lr.LVa : lr.LVb) + row * kLongPitch + 64;
+        const uint16_t* ib = (sa ? lr.LIa : lr.LIb) + row * kLongPrefix;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            nv[c] = ld_lane<uint4>(base, lane4 * 4u + c * 1024u);
+            ni[c] = ld_lane<uint2>(ib, lane4 * 2u + c * 512u);
+        }
+        if (lane == 0) nL = ld_lane<uint32_t>(base, (uint32_t)(T - 1) * 4u);
+        if (lane < kLongSum) nls = ld_lane<uint32_t>((sa ? lr.LSb : lr.LSa) + row * kLongSum, lane4);
+    };
+    double acc = 0.0, px = 0.0, py = 0.0;
+    // the previous row: its P count, and either its Q block (pending) or its
+    // Q count from the streaming path (ready)
+    long long prow = -1;
+    bool pside = false, ptie = false, pready = false, pqtie = false;
+    int pnp = 0, pc1 = 0, pnq = 0;
+    uint32_t pexq = 0u, peps = 0u;
+    uint2 pblk = make_uint2(0u, 0u);
+    if (R > 0) issue(0);
+    for (int t = 0;; ++t) {
+        const bool have = t < R;
+        asm volatile("" : "+v"(lane4));
+        uint32_t v[16], q[16];
+        uint32_t lsq = 0xffffffffu, L = 0u;
+        bool sideA = false;
+        long long row = 0;
+        const uint32_t* rq = nullptr;
+        if (have) {
+            row = i0 + (long long)t * step;
+            sideA = side_of(t);
+            uint32_t idx[8];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                v[4 * c + 0] = nv[c].x; v[4 * c + 1] = nv[c].y;
+                v[4 * c + 2] = nv[c].z; v[4 * c + 3] = nv[c].w;
+                idx[2 * c] = ni[c].x;
+                idx[2 * c + 1] = ni[c].y;
+            }
+            lsq = nls;
+            L = (uint32_t)readlane((int)nL, 0);
+            rq = (sideA ? lr.Db : lr.Da) + row * lr.P;
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                q[e] = ld_lane<uint32_t>(rq, ((idx[e >> 1] >> (16 * (e & 1))) & 0xffffu) * 4u);
+            if (t + 1 < R) issue(t + 1);
+        }
+        // ---- previous row: Q count, psi loads ------------------------------
+        acc += px + py;
+        px = py = 0.0;
+        if (prow >= 0) {
+            bool tq = pqtie;
+            const int nq_ = (pready ? pnq : sorted_count(pblk, pc1, pexq, lane, tq)) - 1;
+            const int np = pnp - 1;
+            const bool flag = tq || ptie;
+            const int nx = pside ? np : nq_, ny = pside ? nq_ : np;
+            if (!flag && lane == 0) {
+                px = o.psi[nx];
+                py = o.psi[ny];
+            }
+            if (lane == 0) {
+                if (flag) queue_row(o, item, (int)prow, peps);
+                if (o.rc_nx) {
+                    const size_t off = (size_t)item * N + prow;
+                    o.rc_nx[off] = flag ? -1 : nx;
+                    o.rc_ny[off] = flag ? -1 : ny;
+                    o.rc_flag[off] = 0;
+                }
+            }
+            prow = -1;
+        }
+        if (!have) break;
+        // ---- histogram of the prefix's joint keys over [L, H) --------------
+        const uint32_t H = min((uint32_t)readlane((int)v[15], kWave - 1), kPadBits);
+        const uint32_t span1 = H > L ? H - L - 1u : 0u;
+        int sh = max(0, 24 - (int)__builtin_clz(span1 | 1u));
+        if ((span1 >> sh) > (uint32_t)(kPBins - 3)) ++sh;
+        const uint32_t Lm = L - min(L, 1u << sh);
+        reinterpret_cast<uint4*>(hist)[lane] = make_uint4(0u, 0u, 0u, 0u);
+        lds_order();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t key = max(v[e], q[e]);
+            const uint32_t bin = min(__builtin_elementwise_sub_sat(key, Lm) >> sh,
+                                     (uint32_t)(kPBins - 1));
+            __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+        lds_order();
+        int b1, r0, pop;
+        find_bin<kPBins / kWave>(hist, lane, T, b1, r0, pop);
+        lds_order();
+        bool queued = b1 == 0 || b1 == kPBins - 1 || pop > kWave;
+        uint32_t eps_key = kUnknownEps, ex = 0u, ey = 0u;
+        if (!queued) {
+            const uint32_t lo = Lm + ((uint32_t)b1 << sh);
+            const uint32_t width = 1u << sh;
+            uint2* cand = reinterpret_cast<uint2*>(hist);
+            int nstored = 0;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const uint32_t key = max(v[e], q[e]);
+                const bool sure = key < lo;
+                ex = max(ex, sure ? v[e] : 0u);
+                ey = max(ey, sure ? q[e] : 0u);
+                const bool inbin = key - lo < width;
+                const uint64_t cm = __ballot(inbin);
+                if (cm) {
+                    if (inbin) cand[nstored + mbcnt(cm)] = make_uint2(v[e], q[e]);
+                    nstored += __popcll(cm);
+                }
+            }
+            lds_order();
+            const bool act = lane < pop;
+            uint32_t ca = 0u, cb = 0u, low = 0xffffffffu;
+            if (act) {
+                const uint2 c = cand[lane];
+                ca = c.x;
+                cb = c.y;
+                low = max(ca, cb) - lo;
+            }
+            uint64_t live = __ballot(act);
+            uint32_t pre = 0u;
+            int r = r0;
+            for (int bit = sh - 1; bit >= 0 && __popcll(live) > 1; --bit) {
+                const uint64_t z = __ballot((low >> bit) == (pre << 1));
+                const int c0 = __popcll(z);
+                if (r > c0) {
+                    r -= c0;
+                    pre = (pre << 1) | 1u;
+                    live &= ~z;
+                } else {
+                    pre <<= 1;
+                    live = z;
+                }
+            }
+            const uint32_t sel = (uint32_t)readlane((int)low, __ffsll((unsigned long long)live) - 1);
+            eps_key = lo + sel;
+            const bool in = act && low <= sel;
+            queued = __popcll(__ballot(in)) != r0 || eps_key >= H;
+            ex = wave_max_u(max(ex, in ? ca : 0u));
+            ey = wave_max_u(max(ey, in ? cb : 0u));
+            lds_order();
+        }
+        if (queued) {
+            const bool beyond = b1 == kPBins - 1 || (eps_key != kUnknownEps && eps_key >= H);
+            if (lane == 0) {
+                if (beyond)
+                    queue_long(o, item, (int)row);
+                else
+                    queue_row(o, item, (int)row, eps_key);
+                if (o.rc_nx && !(beyond && o.longq)) {
+                    const size_t off = (size_t)item * N + row;
+                    o.rc_nx[off] = -1;
+                    o.rc_ny[off] = -1;
+                    o.rc_flag[off] = 0;
+                }
+            }
+            continue;
+        }
+        // ---- nP from the prefix in registers; nQ from Q's prefix store or
+        //      a streaming count over Q's row ---------------------------------
+        int c = 0;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) c += (v[e] <= ex ? 1 : 0) + (v[e] == ex ? 0x10000 : 0);
+        c = wave_sum(c);
+        pnp = c & 0xffff;
+        ptie = (c >> 16) > 1;
+        const uint32_t qtop = (uint32_t)readlane((int)lsq, kLongSum - 1);
+        if (ey < qtop) {
+            pready = false;
+            pqtie = false;
+            pc1 = __popcll(__ballot(lane < kLongSum && lsq <= ey));
+            const uint32_t* sq = (sideA ? lr.LVb : lr.LVa) + row * kLongPitch + 64 * pc1;
+            pblk = ld_lane<uint2>(sq, lane4 * 2u);
+        } else {
+            // frames [N, P) hold padding above every distance
+            int cq = 0;
+            for (int m = 0; m < nblk; ++m) {
+                const uint4 w = ld_lane<uint4>(rq, lane4 * 4u + (uint32_t)m * 1024u);
+                cq += (w.x <= ey ? 1 : 0) + (w.x == ey ? 0x10000 : 0);
+                cq += (w.y <= ey ? 1 : 0) + (w.y == ey ? 0x10000 : 0);
+                cq += (w.z <= ey ? 1 : 0) + (w.z == ey ? 0x10000 : 0);
+                cq += (w.w <= ey ? 1 : 0) + (w.w == ey ? 0x10000 : 0);
+            }
+            cq = wave_sum(cq);
+            pready = true;
+            pnq = cq & 0xffff;
+            pqtie = (cq >> 16) > 1;
+        }
+        prow = row;
+        pside = sideA;
+        pexq = ey;
+        peps = eps_key;
+    }
+    return acc + (px + py);
+}
+
+// ---------------------------------------------------------------------------
 // One frame row processed by the 4 waves of a block together, wave w holding
 // frames [w*256*S4, (w+1)*256*S4) (N <= 1024*S4).  Same three sweeps with a
 // block-shared histogram; wave 0 resolves the boundary candidates.  Five
@@ -1022,13 +1342,14 @@ __device__ __forceinline__ double row_coop(const float* Da, const float* Db, lon
     return contrib;
 }
 
-// Pair index of a block: each XCD (block id mod 8) receives a contiguous
-// range of pairs, so pairs sharing their first atom keep its rows in that
-// XCD's L2.
-__device__ __forceinline__ long long xcd_pair(long long npairs)
+// Work item of virtual block vb (the kernels loop vb = blockIdx.x, +gridDim.x,
+// .. below roundup(n, 8), the grid a multiple of 8): each XCD (vb mod 8)
+// receives a contiguous range of items, so neighbouring items keep their
+// shared atoms' rows in that XCD's L2.
+__device__ __forceinline__ long long xcd_item(long long vb, long long n)
 {
-    const long long per_xcd = (npairs + 7) / 8;
-    return (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const long long per_xcd = (n + 7) / 8;
+    return (vb % 8) * per_xcd + vb / 8;
 }
 
 // ---------------------------------------------------------------------------
@@ -1046,12 +1367,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // read by two waves at about the same time), or pair 4q+w (consecutive
     // pairs share their first atom).
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
-    const long long item = xcd_pair(nquads * kRowSplit);
-    if (item >= nquads * kRowSplit) return;
+    const long long nitems = nquads * kRowSplit;
+    // grid-stride over virtual blocks: a launch's work-item count must stay
+    // below 2^32
+    for (long long vb = blockIdx.x; vb < (nitems + 7) / 8 * 8; vb += gridDim.x) {
+    const long long item = xcd_item(vb, nitems);
+    if (item >= nitems) continue;
     const long long q = item / kRowSplit;
     const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
     const int part = (int)(item % kRowSplit);
-    if (p < 0 || p >= args.npairs) return;   // no block barriers below
+    if (p < 0 || p >= args.npairs) continue;   // no block barriers below
     const int N = args.N, T = args.K + 1;
     const int i0 = (int)((long long)N * part / kRowSplit);
     const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
@@ -1066,6 +1391,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     sr.SMb = args.SM + (size_t)args.a2[p] * N * kSumPitch;
     const double acc = rows_single<S4>(Da, Db, sr, args.P, N, T, i0, i1, 1, p, lds[w], lane, o);
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
+    }
 }
 
 __device__ __forceinline__ PrefixRows prefix_rows(const float* D, const float* SD, const float* SM,
@@ -1094,14 +1420,18 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
-    const long long item = xcd_pair(nquads * kRowSplit);
-    if (item >= nquads * kRowSplit) return;
+    const long long nitems = nquads * kRowSplit;
+    // grid-stride over virtual blocks: a launch's work-item count must stay
+    // below 2^32
+    for (long long vb = blockIdx.x; vb < (nitems + 7) / 8 * 8; vb += gridDim.x) {
+    const long long item = xcd_item(vb, nitems);
+    if (item >= nitems) continue;
     // row-range major: the blocks in flight on one XCD are consecutive tiles
     // (sharing their first two atoms) over the same row range
     const long long q = item % nquads;
     const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
     const int part = (int)(item / nquads);
-    if (p < 0 || p >= args.npairs) return;   // no block barriers below
+    if (p < 0 || p >= args.npairs) continue;   // no block barriers below
     const int N = args.N, T = args.K + 1;
     const int i0 = (int)((long long)N * part / kRowSplit);
     const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
@@ -1109,6 +1439,55 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
                                       args.a2[p]);
     const double acc = rows_prefix(pr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
+    }
+}
+
+// Long-trajectory prefix kernel (kind 3): block and order as
+// ksg_pair_prefix_kernel; 1 KB of LDS per wave.
+__global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
+{
+    __shared__ __attribute__((aligned(4 * kPBins))) uint32_t lds[kPairWaves][kPBins];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
+    const long long nitems = nquads * kRowSplit;
+    // grid-stride over virtual blocks: a launch's work-item count must stay
+    // below 2^32
+    for (long long vb = blockIdx.x; vb < (nitems + 7) / 8 * 8; vb += gridDim.x) {
+    const long long item = xcd_item(vb, nitems);
+    if (item >= nitems) continue;
+    const long long q = item % nquads;
+    const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
+    const int part = (int)(item / nquads);
+    if (p < 0 || p >= args.npairs) continue;   // no block barriers below
+    const int N = args.N, T = args.K + 1;
+    const int i0 = (int)((long long)N * part / kRowSplit);
+    const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
+    const LongRows lr = long_rows(args.D, args.LV, args.LI, args.LS, args.P, N, args.a1[p],
+                                  args.a2[p]);
+    const double acc = rows_long(lr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
+    if (lane == 0) args.partial[p * kRowSplit + part] = acc;
+    }
+}
+
+// Rows the long-trajectory kernel queued to longq (the (k+1)-th neighbour
+// beyond the 1024-frame prefix): one block of four waves per entry runs the
+// cooperative streaming row (row_coop), its psi sum added to fixsum[pair].
+__global__ __launch_bounds__(256) void ksg_long_rows_coop_kernel(KsgArgs args, unsigned nent)
+{
+    __shared__ CoopLds sh;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    for (unsigned e = blockIdx.x; e < nent; e += gridDim.x) {   // uniform over the block
+    const FixEntry fe = args.longq[e];
+    const long long p = fe.pair;
+    const int N = args.N;
+    const float* Da = args.D + (size_t)args.a1[p] * N * args.P;
+    const float* Db = args.D + (size_t)args.a2[p] * N * args.P;
+    const double acc = row_coop<kMaxFrames / 1024>(Da, Db, args.P, N, args.K + 1, fe.row, p, sh,
+                                                   lane, w, row_out(args));
+    if (threadIdx.x == 0) atomicAdd(&args.fixsum[p], acc);
+    }
 }
 
 // Rows the sorted-prefix kernel queued to longq: one wave per entry runs the
@@ -1119,8 +1498,7 @@ __global__ __launch_bounds__(256) void ksg_long_rows_kernel(KsgArgs args, unsign
     __shared__ __attribute__((aligned(4 * kBins))) uint32_t lds[kPairWaves][kBins];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const unsigned e = blockIdx.x * kPairWaves + w;
-    if (e >= nent) return;   // no block barriers below
+    for (unsigned e = blockIdx.x * kPairWaves + w; e < nent; e += gridDim.x * kPairWaves) {
     const FixEntry fe = args.longq[e];
     const long long p = fe.pair;
     const int N = args.N;
@@ -1135,6 +1513,7 @@ __global__ __launch_bounds__(256) void ksg_long_rows_kernel(KsgArgs args, unsign
         args.D + (size_t)a1 * N * args.P, args.D + (size_t)a2 * N * args.P, sr, args.P, N,
         args.K + 1, fe.row, fe.row + 1, 1, p, lds[w], lane, row_out(args));
     if (lane == 0) atomicAdd(&args.fixsum[p], acc);
+    }
 }
 
 template <int S4>
@@ -1143,8 +1522,9 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
     __shared__ CoopLds sh;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    const long long p = xcd_pair(args.npairs);
-    if (p >= args.npairs) return;
+    for (long long vb = blockIdx.x; vb < (args.npairs + 7) / 8 * 8; vb += gridDim.x) {
+    const long long p = xcd_item(vb, args.npairs);
+    if (p >= args.npairs) continue;   // uniform over the block
     const int N = args.N, T = args.K + 1;
     const float* Da = args.D + (size_t)args.a1[p] * N * args.P;
     const float* Db = args.D + (size_t)args.a2[p] * N * args.P;
@@ -1152,6 +1532,7 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
     double acc = 0.0;
     for (int i = 0; i < N; ++i) acc += row_coop<S4>(Da, Db, args.P, N, T, i, p, sh, lane, w, o);
     if (threadIdx.x < kRowSplit) args.partial[p * kRowSplit + threadIdx.x] = threadIdx.x == 0 ? acc : 0.0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1190,6 +1571,22 @@ __device__ __forceinline__ double batch_prefix(const TrajDesc& d, int a1, int a2
 {
     const PrefixRows pr = prefix_rows(d.D, d.SD, d.SM, d.SI, d.P, d.N, a1, a2);
     const double acc = rows_prefix(pr, d.N, d.K + 1, w, d.N, kPairWaves, item, u.prefix[w], lane, o);
+    if (lane == 0) wsum[w] = acc;
+    __syncthreads();
+    return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+
+// Kind 3 in the batched kernel: wave w runs rows w, w+4, ... in chunks of at
+// most kPrefix rows.
+__device__ __forceinline__ double batch_long(const TrajDesc& d, int a1, int a2, long long item,
+                                             BatchLds& u, double* wsum, int lane, int w,
+                                             const RowOut& o)
+{
+    const LongRows lr = long_rows(d.D, d.LV, d.LI, d.LS, d.P, d.N, a1, a2);
+    double acc = 0.0;
+    for (int c0 = 0; c0 < d.N; c0 += kPairWaves * kPrefix)
+        acc += rows_long(lr, d.N, d.K + 1, c0 + w, min(d.N, c0 + kPairWaves * kPrefix),
+                         kPairWaves, item, u.single[w], lane, o);
     if (lane == 0) wsum[w] = acc;
     __syncthreads();
     return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
@@ -1235,6 +1632,8 @@ __global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
         double acc = 0.0;
         if (d.kind == 2) {
             acc = batch_prefix(d, b.a1[item], b.a2[item], item, u, wsum, lane, w, o);
+        } else if (d.kind == 3) {
+            acc = batch_long(d, b.a1[item], b.a2[item], item, u, wsum, lane, w, o);
         } else if (d.kind == 0) {
             switch (d.s4) {
                 case 2: acc = batch_single<2>(d, Da, Db, item, u, wsum, lane, w, o); break;
@@ -1269,8 +1668,8 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned
     __shared__ uint32_t lds[4][kBins];
     const int lane = threadIdx.x & (kWave - 1);
     uint32_t* hist = lds[threadIdx.x / kWave];
-    const unsigned e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (e >= nent) return;
+    for (unsigned e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; e < nent;
+         e += gridDim.x * (blockDim.x / kWave)) {
     const FixEntry fe = args.fixq[e];
     const long long p = fe.pair;
     const int i = fe.row;
@@ -1393,6 +1792,7 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned
             args.rc_flag[o] = tie ? 2 : 1;
         }
     }
+    }
 }
 
 __global__ void ksg_clear_kernel(double* fixsum, long long npairs, unsigned* counters,
@@ -1436,6 +1836,16 @@ __global__ void ksg_selftest_kernel(int* out)
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+
+// Grid for a grid-stride kernel over n blocks' worth of work: at most 2^22
+// blocks (2^30 work-items at 256 threads, under the 2^32 work-items a
+// dispatch can carry), a multiple of 8 (the XCD mapping of xcd_item).
+unsigned capped_grid(long long nblocks)
+{
+    const long long cap = 1LL << 22;
+    const long long g = (nblocks + 7) / 8 * 8;
+    return (unsigned)(g < cap ? (g > 0 ? g : 8) : cap);
+}
 hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, float* D,
                              float* SD, float* SM, uint16_t* SI, hipStream_t s)
 {
@@ -1443,8 +1853,25 @@ hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, fl
         (SD && N > kSortedMaxFrames))
         return hipErrorInvalidValue;
     const long long rows = (long long)natoms * N;
-    hipLaunchKernelGGL(ksg_build_rows_kernel, dim3((unsigned)rows), dim3(256), 0, s, X, N, P, D,
-                       SD, SM, SI);
+    hipLaunchKernelGGL(ksg_build_rows_kernel, dim3(capped_grid(rows)), dim3(256), 0, s, X, N, P,
+                       rows, D, SD, SM, SI);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_prefix(const float* D, int natoms, int N, long long P, float* LV,
+                               uint16_t* LI, float* LS, hipStream_t s)
+{
+    if (!D || !LV || !LI || !LS || N <= kSortedMaxFrames || N > kMaxFrames)
+        return hipErrorInvalidValue;
+    int M = 64;
+    while (M < N) M <<= 1;
+    const size_t bytes = (size_t)M * sizeof(uint64_t);
+    hipError_t e = hipFuncSetAttribute((const void*)ksg_build_prefix_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    const long long rows = (long long)natoms * N;
+    hipLaunchKernelGGL(ksg_build_prefix_kernel, dim3(capped_grid(rows)), dim3(512), bytes, s, D, N,
+                       P, rows, LV, LI, LS);
     return hipGetLastError();
 }
 
@@ -1462,6 +1889,7 @@ hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
 bool variant_fits(int kind, int s4, int N, int K)
 {
     if (kind == 2) return N <= kSortedMaxFrames && K + 1 <= kPrefixMaxT;
+    if (kind == 3) return N > kSortedMaxFrames && N <= kMaxFrames && K + 1 <= kPrefixMaxT;
     const long long cap = (kind == 0 ? 256LL : 1024LL) * s4;
     const bool inst = kind == 0 ? (s4 == 1 || s4 == 2 || s4 == 3 || s4 == 4 || s4 == 5 ||
                                    s4 == 6 || s4 == 8)
@@ -1472,6 +1900,7 @@ bool variant_fits(int kind, int s4, int N, int K)
 void default_variant(int N, int K, int& kind, int& s4)
 {
     if (variant_fits(2, 1, N, K)) { kind = 2; s4 = 1; return; }
+    if (variant_fits(3, 1, N, K)) { kind = 3; s4 = 1; return; }
     static const int single[] = {1, 2, 3, 4, 5, 6, 8};
     for (int v : single)
         if (N <= 256 * v) { kind = 0; s4 = v; return; }
@@ -1482,16 +1911,26 @@ void default_variant(int N, int K, int& kind, int& s4)
 
 hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4)
 {
-    if (kind < 0) default_variant(a.N, a.K, kind, s4);
+    if (kind < 0) {
+        default_variant(a.N, a.K, kind, s4);
+        if (kind == 3 && !(a.LV && a.LI && a.LS)) {   // prefix stores not built
+            kind = 1;
+            s4 = (a.N + 1023) / 1024;
+            if (s4 < 2) s4 = 2;
+        }
+    }
     if (!variant_fits(kind, s4, a.N, a.K)) return hipErrorInvalidValue;
     if (variant_sorted(kind) && !(a.SD && a.SM && a.SI)) return hipErrorInvalidValue;
+    if (variant_prefix_store(kind) && !(a.LV && a.LI && a.LS)) return hipErrorInvalidValue;
     const long long nitems =
         kind != 1 ? (a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves) * kRowSplit
                   : a.npairs;
-    const dim3 grid((unsigned)(((nitems + 7) / 8) * 8)), block(kWave * kPairWaves);
+    const dim3 grid(capped_grid(nitems)), block(kWave * kPairWaves);
 #define GMX_SINGLE(V) case V: hipLaunchKernelGGL(ksg_pair_kernel<V>, grid, block, 0, s, a); break;
 #define GMX_COOP(V) case V: hipLaunchKernelGGL(ksg_pair_coop_kernel<V>, grid, block, 0, s, a); break;
-    if (kind == 2) {
+    if (kind == 3) {
+        hipLaunchKernelGGL(ksg_pair_long_kernel, grid, block, 0, s, a);
+    } else if (kind == 2) {
         hipLaunchKernelGGL(ksg_pair_prefix_kernel, grid, block, 0, s, a);
     } else if (kind == 0) {
         switch (s4) {
@@ -1514,7 +1953,7 @@ hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s)
 {
     if (nqueued == 0) return hipSuccess;
     const unsigned per_block = 4;
-    hipLaunchKernelGGL(ksg_fixup_kernel, dim3((nqueued + per_block - 1) / per_block),
+    hipLaunchKernelGGL(ksg_fixup_kernel, dim3(capped_grid((nqueued + per_block - 1) / per_block)),
                        dim3(kWave * per_block), 0, s, f, nqueued);
     return hipGetLastError();
 }
@@ -1522,8 +1961,14 @@ hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s)
 hipError_t launch_ksg_long_rows(const KsgArgs& a, unsigned nlong, hipStream_t s)
 {
     if (nlong == 0) return hipSuccess;
-    if (!a.longq || !a.SD || !a.SM || a.N > kSortedMaxFrames) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(ksg_long_rows_kernel, dim3((nlong + kPairWaves - 1) / kPairWaves),
+    if (!a.longq) return hipErrorInvalidValue;
+    if (a.N > kSortedMaxFrames) {
+        hipLaunchKernelGGL(ksg_long_rows_coop_kernel, dim3(capped_grid(nlong)),
+                           dim3(kWave * kPairWaves), 0, s, a, nlong);
+        return hipGetLastError();
+    }
+    if (!a.SD || !a.SM) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ksg_long_rows_kernel, dim3(capped_grid((nlong + kPairWaves - 1) / kPairWaves)),
                        dim3(kWave * kPairWaves), 0, s, a, nlong);
     return hipGetLastError();
 }
@@ -1538,7 +1983,7 @@ hipError_t launch_ksg_finalize(const FixupArgs& f, long long nitems, const doubl
 
 bool batch_variant_ok(int kind, int s4, int N, int K)
 {
-    if (kind == 2) return variant_fits(2, s4, N, K);
+    if (kind == 2 || kind == 3) return variant_fits(kind, s4, N, K);
     if (kind == 0) return (s4 == 2 || s4 == 4 || s4 == 6 || s4 == 8) && N <= 256 * s4;
     if (kind == 1) return (s4 == 3 || s4 == 4 || s4 == 6 || s4 == 8) && N <= 1024 * s4;
     return false;

@@ -111,6 +111,9 @@ struct gmxcorr_ksg_plan {
     float* dSD = nullptr;   // sorted rows, summaries, prefix indices (N <= kSortedMaxFrames)
     float* dSM = nullptr;
     uint16_t* dSI = nullptr;
+    float* dLV = nullptr;   // prefix stores (kSortedMaxFrames < N <= kMaxFrames, k + 1 <= 256)
+    uint16_t* dLI = nullptr;
+    float* dLS = nullptr;
     int* dA1 = nullptr;
     int* dA2 = nullptr;
     long long pair_cap = 0;
@@ -205,7 +208,7 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
 {
     KsgArgs a{};
     if (tiles) { a.quads = pl->dQuads; a.nquads = pl->nquads; }
-    a.X = pl->dX; a.D = pl->dD; a.SD = pl->dSD; a.SM = pl->dSM; a.SI = pl->dSI; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
+    a.X = pl->dX; a.D = pl->dD; a.SD = pl->dSD; a.SM = pl->dSM; a.SI = pl->dSI; a.LV = pl->dLV; a.LI = pl->dLI; a.LS = pl->dLS; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
     a.partial = pl->dPartial; a.fixsum = pl->dFixsum; a.mi = pl->dMI;
     a.counters = pl->dCounters;
     a.npairs = npairs; a.P = pl->P; a.N = pl->N; a.K = pl->K; a.psiN_phiK = pl->psiN_phiK;
@@ -359,6 +362,36 @@ int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** o
         return bail(hip_fail(e, "hipMalloc(descriptor)"));
     int rc = ensure_fix(pl, 65536);
     if (rc) return bail(rc);
+    // the recount and long-row queues of an all-pairs run (1% of its rows
+    // each, as run_loaded sizes them), before the prefix stores take HBM
+    {
+        const long double run_rows = (long double)natoms * (natoms - 1) / 2 * nframes;
+        const long long want = (long long)std::min<long double>(
+            std::max<long double>(65536, run_rows / 100), (long double)(64LL << 20));
+        int rc2 = ensure_fix(pl, (unsigned)want);
+        if (!rc2 && gmxcorr::variant_fits(3, 1, nframes, k)) rc2 = ensure_long(pl, (unsigned)want);
+        if (rc2) return bail(rc2);
+    }
+    // prefix stores for the long-trajectory path when they fit with 4 GB to
+    // spare; otherwise the plan runs the cooperative path
+    if (gmxcorr::variant_fits(3, 1, nframes, k)) {
+        const size_t rows = (size_t)natoms * nframes;
+        const size_t lbytes = rows * (gmxcorr::kLongPitch * sizeof(float) +
+                                      gmxcorr::kLongPrefix * sizeof(uint16_t) +
+                                      gmxcorr::kLongSum * sizeof(float));
+        const size_t reserve = 4ull << 30;
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > lbytes + reserve) {
+            if (hipMalloc(&pl->dLV, rows * gmxcorr::kLongPitch * sizeof(float)) != hipSuccess ||
+                hipMalloc(&pl->dLI, rows * gmxcorr::kLongPrefix * sizeof(uint16_t)) != hipSuccess ||
+                hipMalloc(&pl->dLS, rows * gmxcorr::kLongSum * sizeof(float)) != hipSuccess) {
+                (void)hipGetLastError();
+                dfree(pl->dLV); dfree(pl->dLI); dfree(pl->dLS);
+            } else {
+                sbytes += lbytes;
+            }
+        }
+    }
     pl->st.row_bytes = (long long)(dbytes + sbytes);
     pl->st.nframes = nframes; pl->st.k = k; pl->st.natoms = natoms; pl->st.pitch = (int)pl->P;
     *out = pl;
@@ -377,6 +410,9 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
     HIPCHK(hipEventRecord(pl->ev[1], pl->stream));
     HIPCHK(gmxcorr::launch_build_rows(pl->dX, pl->natoms, pl->N, pl->P, pl->dD, pl->dSD, pl->dSM,
                                       pl->dSI, pl->stream));
+    if (pl->dLV)
+        HIPCHK(gmxcorr::launch_build_prefix(pl->dD, pl->natoms, pl->N, pl->P, pl->dLV, pl->dLI,
+                                            pl->dLS, pl->stream));
     HIPCHK(hipEventRecord(pl->ev[5], pl->stream));
     HIPCHK(hipStreamSynchronize(pl->stream));
     pl->st.ms_load = elapsed(pl->ev[0], pl->ev[1]);
@@ -388,6 +424,9 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
     td.SD = pl->dSD;
     td.SM = pl->dSM;
     td.SI = pl->dSI;
+    td.LV = pl->dLV;
+    td.LI = pl->dLI;
+    td.LS = pl->dLS;
     td.psi = pl->dPsi;
     td.P = pl->P;
     td.N = pl->N;
@@ -507,7 +546,7 @@ void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* pl)
 {
     if (!pl) return;
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dSD); dfree(pl->dSM); dfree(pl->dSI); dfree(pl->dQuads); dfree(pl->dA1); dfree(pl->dA2);
+    dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dSD); dfree(pl->dSM); dfree(pl->dSI); dfree(pl->dLV); dfree(pl->dLI); dfree(pl->dLS); dfree(pl->dQuads); dfree(pl->dA1); dfree(pl->dA2);
     dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix); dfree(pl->dLong);
     dfree(pl->dCounters); dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
     dfree(pl->dTraj);

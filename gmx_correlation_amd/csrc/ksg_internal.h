@@ -29,6 +29,9 @@ struct KsgArgs {
     const float*  SD;     // [n][N][sorted_pitch(N)] sorted rows (N <= kSortedMaxFrames)
     const float*  SM;     // [n][N][kSumPitch] their summaries
     const uint16_t* SI;   // [n][N][kPrefix] frame indices of the kPrefix smallest values
+    const float*  LV;     // [n][N][kLongPitch] prefix stores (kind 3, N > kSortedMaxFrames):
+    const uint16_t* LI;   //   the kLongPrefix smallest values (64 leading zeros), their
+    const float*  LS;     //   frames, and every 64th of those values
     const double* psi;    // [N+1]
     const int*    a1;     // [npairs]
     const int*    a2;     // [npairs]
@@ -60,6 +63,9 @@ struct TrajDesc {
     const float* SD;     // sorted rows, summaries and prefix indices (kinds 0 and 2)
     const float* SM;
     const uint16_t* SI;
+    const float* LV;     // prefix stores (kind 3)
+    const uint16_t* LI;
+    const float* LS;
     const double* psi;   // [N+1]
     long long P;
     int N;
@@ -125,6 +131,14 @@ __host__ __device__ inline long long sorted_pitch(int N)
     return 64 + (n + 64 + 63) / 64 * 64;
 }
 
+// Prefix stores of the long-trajectory path (kind 3, kSortedMaxFrames < N <=
+// kMaxFrames): per row the kLongPrefix smallest values in order after 64
+// zeros (pitch kLongPitch floats), their frames (u16), and kLongSum summary
+// values (value 64m + 63).  6.4 KB per row instead of a full sorted row.
+constexpr int kLongPrefix = 1024;
+constexpr int kLongPitch = 64 + kLongPrefix;
+constexpr int kLongSum = kLongPrefix / 64;
+
 // Read-only view of a loaded plan's device data (for the surrogate features).
 struct PlanView {
     const double* X;   // [3n][N] normalised rows
@@ -138,6 +152,9 @@ struct PlanView {
 // single-wave paths, or all nullptr (required for N > kSortedMaxFrames).
 hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, float* D,
                              float* SD, float* SM, uint16_t* SI, hipStream_t s);
+// LV/LI/LS from the distance rows D (N in (kSortedMaxFrames, kMaxFrames]).
+hipError_t launch_build_prefix(const float* D, int natoms, int N, long long P, float* LV,
+                               uint16_t* LI, float* LS, hipStream_t s);
 hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
                             unsigned long long* queue, hipStream_t s);
 // kind 0: one wave per frame row streaming both distance rows, N <= 256*s4
@@ -145,6 +162,9 @@ hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
 // kind 1: four waves per row, N <= 1024*s4 (s4 in 2..8);
 // kind 2: one wave per row over the sorted prefix of the sparser atom,
 //         N <= kSortedMaxFrames and K + 1 <= kPrefixMaxT (s4 ignored, 1);
+// kind 3: the same over the kLongPrefix prefix stores, gathering the other
+//         atom's distances from its row, kSortedMaxFrames < N <= kMaxFrames
+//         and K + 1 <= kPrefixMaxT (s4 ignored, 1);
 // kind < 0: default.
 bool variant_fits(int kind, int s4, int N, int K);
 void default_variant(int N, int K, int& kind, int& s4);
@@ -157,8 +177,10 @@ hipError_t launch_ksg_finalize(const FixupArgs& f, long long nitems, const doubl
                                int nsplit, double* mi, hipStream_t s);
 // Code paths compiled into the batched kernel.
 bool batch_variant_ok(int kind, int s4, int N, int K);
-// Whether a variant reads the sorted rows (and needs SD, SM, SI built).
+// Whether a variant reads the sorted rows (and needs SD, SM, SI built), or
+// the prefix stores (LV, LI, LS).
 inline bool variant_sorted(int kind) { return kind == 0 || kind == 2; }
+inline bool variant_prefix_store(int kind) { return kind == 3; }
 hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s);
 hipError_t launch_selftest(int* out, hipStream_t s);
 

@@ -52,9 +52,11 @@ def test_batch_matches_single_plans(gpu, trajs, oracle_lib):
         assert abs(mi[0] - v) <= 1e-12
 
 
-def test_batch_explicit_paths(gpu, trajs):
-    """Every compiled batch path that covers a trajectory gives the same MI."""
-    rows, psi, k = trajs[1]                      # N = 900
+@pytest.mark.parametrize("t", [1, 2])
+def test_batch_explicit_paths(gpu, trajs, t):
+    """Every compiled batch path that covers a trajectory gives the same MI
+    (N = 900: kinds 0 and 2; N = 2500: kinds 1 and 3)."""
+    rows, psi, k = trajs[t]
     ref = None
     for kind, s4 in batch.fitting_paths(rows.shape[1], k):
         b = batch.KsgBatch()
