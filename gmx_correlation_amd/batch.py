@@ -22,8 +22,9 @@ from .ksg import KsgPlan
 # kind 0 = one wave per frame row streaming both distance rows (N <= 256*s4),
 # kind 1 = four waves per row (N <= 1024*s4), kind 2 = one wave per row over
 # the 512 nearest frames of the sparser atom (N <= 2048, k + 1 <= 256), kind 3
-# = the same over 1024-frame prefix stores (2048 < N <= 8192, k + 1 <= 256).
-BATCH_PATHS = [(2, 1), (3, 1), (0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (1, 4), (1, 6), (1, 8)]
+# = the same over the first 256*s4 frames of 1024-frame prefix stores
+# (2048 < N <= 8192, k + 1 <= 256; s4 = 2 or 4).
+BATCH_PATHS = [(2, 1), (3, 2), (3, 4), (0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (1, 4), (1, 6), (1, 8)]
 PREFIX_MAX_FRAMES = 2048
 LONG_MAX_FRAMES = 8192
 PREFIX_MAX_T = 256

@@ -99,7 +99,10 @@ int gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const doubl
         static const int coop[] = {3, 4, 6, 8};
         kind = -1;
         if (gmxcorr::batch_variant_ok(2, 1, nframes, k)) { kind = 2; s4 = 1; }
-        if (kind < 0 && gmxcorr::batch_variant_ok(3, 1, nframes, k)) { kind = 3; s4 = 1; }
+        if (kind < 0 && gmxcorr::batch_variant_ok(3, 2, nframes, k)) {
+            kind = 3;
+            s4 = gmxcorr::long_prefix_chunks(nframes, k);
+        }
         if (kind < 0)
             for (int v : single)
                 if (nframes <= 256 * v) { kind = 0; s4 = v; break; }

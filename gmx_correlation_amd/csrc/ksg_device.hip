@@ -941,6 +941,7 @@ __device__ __forceinline__ LongRows long_rows(const float* D, const float* LV, c
     return lr;
 }
 
+template <int C>
 __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, int i0, int i1,
                                             int step, long long item, uint32_t* hist, int lane,
                                             const RowOut& o)
@@ -965,10 +966,10 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         return ((uint32_t)readlane((int)sidebits, t & (kWave - 1)) >> (t >> 6)) & 1u;
     };
     // the next row: P prefix values and frames, SP[T-1], Q summary
-    uint4 nv[4];
-    uint2 ni[4];
+    uint4 nv[C];
+    uint2 ni[C];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < C; ++c) {
         nv[c] = make_uint4(0u, 0u, 0u, 0u);
         ni[c] = make_uint2(0u, 0u);
     }
@@ -979,7 +980,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         const uint32_t* base = (sa ? lr.LVa : lr.LVb) + row * kLongPitch + 64;
         const uint16_t* ib = (sa ? lr.LIa : lr.LIb) + row * kLongPrefix;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < C; ++c) {
             nv[c] = ld_lane<uint4>(base, lane4 * 4u + c * 1024u);
             ni[c] = ld_lane<uint2>(ib, lane4 * 2u + c * 512u);
         }
@@ -998,7 +999,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
     for (int t = 0;; ++t) {
         const bool have = t < R;
         asm volatile("" : "+v"(lane4));
-        uint32_t v[16], q[16];
+        uint32_t v[4 * C], q[4 * C];
         uint32_t lsq = 0xffffffffu, L = 0u;
         bool sideA = false;
         long long row = 0;
@@ -1006,9 +1007,9 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         if (have) {
             row = i0 + (long long)t * step;
             sideA = side_of(t);
-            uint32_t idx[8];
+            uint32_t idx[2 * C];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
+            for (int c = 0; c < C; ++c) {
                 v[4 * c + 0] = nv[c].x; v[4 * c + 1] = nv[c].y;
                 v[4 * c + 2] = nv[c].z; v[4 * c + 3] = nv[c].w;
                 idx[2 * c] = ni[c].x;
@@ -1018,7 +1019,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
             L = (uint32_t)readlane((int)nL, 0);
             rq = (sideA ? lr.Db : lr.Da) + row * lr.P;
 #pragma unroll
-            for (int e = 0; e < 16; ++e)
+            for (int e = 0; e < 4 * C; ++e)
                 q[e] = ld_lane<uint32_t>(rq, ((idx[e >> 1] >> (16 * (e & 1))) & 0xffffu) * 4u);
             if (t + 1 < R) issue(t + 1);
         }
@@ -1048,7 +1049,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         }
         if (!have) break;
         // ---- histogram of the prefix's joint keys over [L, H) --------------
-        const uint32_t H = min((uint32_t)readlane((int)v[15], kWave - 1), kPadBits);
+        const uint32_t H = min((uint32_t)readlane((int)v[4 * C - 1], kWave - 1), kPadBits);
         const uint32_t span1 = H > L ? H - L - 1u : 0u;
         int sh = max(0, 24 - (int)__builtin_clz(span1 | 1u));
         if ((span1 >> sh) > (uint32_t)(kPBins - 3)) ++sh;
@@ -1056,7 +1057,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         reinterpret_cast<uint4*>(hist)[lane] = make_uint4(0u, 0u, 0u, 0u);
         lds_order();
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
+        for (int e = 0; e < 4 * C; ++e) {
             const uint32_t key = max(v[e], q[e]);
             const uint32_t bin = min(__builtin_elementwise_sub_sat(key, Lm) >> sh,
                                      (uint32_t)(kPBins - 1));
@@ -1075,7 +1076,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
             uint2* cand = reinterpret_cast<uint2*>(hist);
             int nstored = 0;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
+            for (int e = 0; e < 4 * C; ++e) {
                 const uint32_t key = max(v[e], q[e]);
                 const bool sure = key < lo;
                 ex = max(ex, sure ? v[e] : 0u);
@@ -1139,7 +1140,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         //      a streaming count over Q's row ---------------------------------
         int c = 0;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) c += (v[e] <= ex ? 1 : 0) + (v[e] == ex ? 0x10000 : 0);
+        for (int e = 0; e < 4 * C; ++e) c += (v[e] <= ex ? 1 : 0) + (v[e] == ex ? 0x10000 : 0);
         c = wave_sum(c);
         pnp = c & 0xffff;
         ptie = (c >> 16) > 1;
@@ -1443,7 +1444,9 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
 }
 
 // Long-trajectory prefix kernel (kind 3): block and order as
-// ksg_pair_prefix_kernel; 1 KB of LDS per wave.
+// ksg_pair_prefix_kernel; 1 KB of LDS per wave.  C: 256-frame chunks of the
+// prefix read per row (2: 512 frames, 4: the whole 1024-frame store).
+template <int C>
 __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
 {
     __shared__ __attribute__((aligned(4 * kPBins))) uint32_t lds[kPairWaves][kPBins];
@@ -1465,7 +1468,7 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
     const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
     const LongRows lr = long_rows(args.D, args.LV, args.LI, args.LS, args.P, N, args.a1[p],
                                   args.a2[p]);
-    const double acc = rows_long(lr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
+    const double acc = rows_long<C>(lr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
     }
 }
@@ -1585,8 +1588,11 @@ __device__ __forceinline__ double batch_long(const TrajDesc& d, int a1, int a2, 
     const LongRows lr = long_rows(d.D, d.LV, d.LI, d.LS, d.P, d.N, a1, a2);
     double acc = 0.0;
     for (int c0 = 0; c0 < d.N; c0 += kPairWaves * kPrefix)
-        acc += rows_long(lr, d.N, d.K + 1, c0 + w, min(d.N, c0 + kPairWaves * kPrefix),
-                         kPairWaves, item, u.single[w], lane, o);
+        acc += d.s4 == 2
+                   ? rows_long<2>(lr, d.N, d.K + 1, c0 + w, min(d.N, c0 + kPairWaves * kPrefix),
+                                  kPairWaves, item, u.single[w], lane, o)
+                   : rows_long<4>(lr, d.N, d.K + 1, c0 + w, min(d.N, c0 + kPairWaves * kPrefix),
+                                  kPairWaves, item, u.single[w], lane, o);
     if (lane == 0) wsum[w] = acc;
     __syncthreads();
     return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
@@ -1889,7 +1895,9 @@ hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
 bool variant_fits(int kind, int s4, int N, int K)
 {
     if (kind == 2) return N <= kSortedMaxFrames && K + 1 <= kPrefixMaxT;
-    if (kind == 3) return N > kSortedMaxFrames && N <= kMaxFrames && K + 1 <= kPrefixMaxT;
+    if (kind == 3)
+        return (s4 == 2 || s4 == 4) && N > kSortedMaxFrames && N <= kMaxFrames &&
+               K + 1 <= kPrefixMaxT;
     const long long cap = (kind == 0 ? 256LL : 1024LL) * s4;
     const bool inst = kind == 0 ? (s4 == 1 || s4 == 2 || s4 == 3 || s4 == 4 || s4 == 5 ||
                                    s4 == 6 || s4 == 8)
@@ -1900,7 +1908,7 @@ bool variant_fits(int kind, int s4, int N, int K)
 void default_variant(int N, int K, int& kind, int& s4)
 {
     if (variant_fits(2, 1, N, K)) { kind = 2; s4 = 1; return; }
-    if (variant_fits(3, 1, N, K)) { kind = 3; s4 = 1; return; }
+    if (variant_fits(3, 2, N, K)) { kind = 3; s4 = long_prefix_chunks(N, K); return; }
     static const int single[] = {1, 2, 3, 4, 5, 6, 8};
     for (int v : single)
         if (N <= 256 * v) { kind = 0; s4 = v; return; }
@@ -1929,7 +1937,10 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4)
 #define GMX_SINGLE(V) case V: hipLaunchKernelGGL(ksg_pair_kernel<V>, grid, block, 0, s, a); break;
 #define GMX_COOP(V) case V: hipLaunchKernelGGL(ksg_pair_coop_kernel<V>, grid, block, 0, s, a); break;
     if (kind == 3) {
-        hipLaunchKernelGGL(ksg_pair_long_kernel, grid, block, 0, s, a);
+        if (s4 == 2)
+            hipLaunchKernelGGL(ksg_pair_long_kernel<2>, grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL(ksg_pair_long_kernel<4>, grid, block, 0, s, a);
     } else if (kind == 2) {
         hipLaunchKernelGGL(ksg_pair_prefix_kernel, grid, block, 0, s, a);
     } else if (kind == 0) {

@@ -369,12 +369,12 @@ int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** o
         const long long want = (long long)std::min<long double>(
             std::max<long double>(65536, run_rows / 100), (long double)(64LL << 20));
         int rc2 = ensure_fix(pl, (unsigned)want);
-        if (!rc2 && gmxcorr::variant_fits(3, 1, nframes, k)) rc2 = ensure_long(pl, (unsigned)want);
+        if (!rc2 && gmxcorr::variant_fits(3, 4, nframes, k)) rc2 = ensure_long(pl, (unsigned)want);
         if (rc2) return bail(rc2);
     }
     // prefix stores for the long-trajectory path when they fit with 4 GB to
     // spare; otherwise the plan runs the cooperative path
-    if (gmxcorr::variant_fits(3, 1, nframes, k)) {
+    if (gmxcorr::variant_fits(3, 4, nframes, k)) {
         const size_t rows = (size_t)natoms * nframes;
         const size_t lbytes = rows * (gmxcorr::kLongPitch * sizeof(float) +
                                       gmxcorr::kLongPrefix * sizeof(uint16_t) +

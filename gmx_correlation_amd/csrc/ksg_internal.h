@@ -181,6 +181,16 @@ bool batch_variant_ok(int kind, int s4, int N, int K);
 // the prefix stores (LV, LI, LS).
 inline bool variant_sorted(int kind) { return kind == 0 || kind == 2; }
 inline bool variant_prefix_store(int kind) { return kind == 3; }
+// Prefix chunks (256 frames) the long-trajectory path reads by default.  For
+// two independent atoms the (k+1)-th joint neighbour lies about
+// sqrt((k+1) N) frames deep in either atom's sorted row; correlated atoms
+// need fewer.  512 frames while that depth stays below 512, else the whole
+// 1024-frame store: rows beyond the prefix run the streaming path, about 4x
+// the cost of a prefix row.  Measured on 120-atom trajectories of 20 modes
+// with switch and rotor couplings: 512 frames is 1.1-1.4x faster at
+// (N, k) = (2500, 100), (3000, 50), (8000, 10); the two tie at (6000, 50);
+// at (5000, 100) a third of the rows overflow 512 and 1024 is 2.6x faster.
+inline int long_prefix_chunks(int N, int K) { return (long long)(K + 1) * N <= (1LL << 18) ? 2 : 4; }
 hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s);
 hipError_t launch_selftest(int* out, hipStream_t s);
 

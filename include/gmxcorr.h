@@ -86,8 +86,9 @@ int  gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* plan, gmxcorr_ksg_stats* st)
  * 1,2,3,4,5,6,8); kind 1 = four wavefronts per row holding 1024*s4 frames
  * (s4 in 2..8); kind 2 = one wavefront per row over the 512 nearest frames
  * of the sparser atom (nframes <= 2048, k + 1 <= 256; s4 ignored); kind 3 =
- * the same over 1024-frame prefix stores (2048 < nframes <= 8192,
- * k + 1 <= 256; s4 ignored; built when they fit in HBM); kind -1 =
+ * the same over the first 256*s4 frames of 1024-frame prefix stores
+ * (2048 < nframes <= 8192, k + 1 <= 256, s4 = 2 or 4; built when they fit
+ * in HBM); kind -1 =
  * the default for the plan's frame count and k.  variant_fits: 1 if
  * (kind, s4) is compiled and covers nframes at this k. */
 int  gmxcorr_ksg_plan_set_variant(gmxcorr_ksg_plan* plan, int kind, int s4);

@@ -203,19 +203,20 @@ def test_prefix_overflow_rows_take_streaming_path(gpu, oracle_lib):
         assert abs(mi[p] - v) <= 1e-12
 
 
-@pytest.mark.parametrize("kind", [-1, 1])
+@pytest.mark.parametrize("variant", [None, (3, 2), (3, 4), (1, 0)])
 @pytest.mark.parametrize("N,k,npairs", [(3000, 50, 2), (5000, 100, 1), (8000, 10, 1)])
-def test_long_trajectories(gpu, oracle_lib, N, k, npairs, kind):
-    """N > 2048: the default prefix-store path (kind 3) and the 4-waves-per-
-    row streaming kernel (kind 1); counts and MI against the FP64
-    brute-force recount."""
+def test_long_trajectories(gpu, oracle_lib, N, k, npairs, variant):
+    """N > 2048: the default path, the prefix-store path (kind 3) over 512
+    and 1024 prefix frames, and the 4-waves-per-row streaming kernel
+    (kind 1); counts and MI against the FP64 brute-force recount."""
     c = synth.generate(synth.TrajSpec(natoms=5, nframes=N, nmodes=4, seed=N))
     rows, psi = ksg.kraskov_prepare(ksg.center(c))
     a1 = np.array([1, 4])[:npairs]; a2 = np.array([0, 2])[:npairs]
     with ksg.KsgPlan(5, N, k) as plan:
         plan.load(rows, psi)
-        if kind == 1:
-            plan.set_variant(1, (N + 1023) // 1024)
+        if variant is not None:
+            kind, s4 = variant
+            plan.set_variant(kind, s4 if kind == 3 else (N + 1023) // 1024)
         mi, d = plan.run_pairs(a1, a2, detail=True)
     for p in range(npairs):
         v, ref = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), k, method="brute",
@@ -289,22 +290,25 @@ def test_rejects_bad_pair_indices(gpu, small_ws):
             plan.run_pairs([64], [0])
 
 
-def test_long_prefix_overflow_rows_take_cooperative_path(gpu, oracle_lib):
-    """Independent atoms at 6000 frames, k = 250: the (k+1)-th joint
-    neighbour lies beyond the 1024-frame prefix store of either atom in most
-    rows (about sqrt(251*6000) = 1227 frames are needed), so those rows run
-    the cooperative streaming row; counts equal the FP64 recount."""
+@pytest.mark.parametrize("k,s4", [(250, 4), (100, 2)])
+def test_long_prefix_overflow_rows_take_cooperative_path(gpu, oracle_lib, k, s4):
+    """Independent atoms at 6000 frames: the (k+1)-th joint neighbour lies
+    about sqrt((k+1)*6000) frames deep in either atom's sorted row (1227 at
+    k = 250, 778 at k = 100), beyond the prefix read (1024 or 512 frames) in
+    most rows, so those rows run the cooperative streaming row; counts equal
+    the FP64 recount."""
     rng = np.random.default_rng(12)
     coords = rng.standard_normal((6000, 4, 3)).astype(np.float32)
     rows, psi = ksg.kraskov_prepare(ksg.center(coords))
     a1 = np.array([1, 3]); a2 = np.array([0, 2])
-    with ksg.KsgPlan(4, 6000, 250) as plan:
+    with ksg.KsgPlan(4, 6000, k) as plan:
         plan.load(rows, psi)
+        plan.set_variant(3, s4)
         mi, d = plan.run_pairs(a1, a2, detail=True)
         st = plan.stats()
     assert st["long_rows"] > 1000
     for p in range(len(a1)):
-        v, ref = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), 250, method="brute",
+        v, ref = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), k, method="brute",
                                      detail=True)
         assert np.array_equal(d["nx"][p], ref["nx"]), p
         assert np.array_equal(d["ny"][p], ref["ny"]), p
