@@ -71,6 +71,7 @@ class BatchStats(ctypes.Structure):
         ("fixup_rows", _ll),
         ("ties", _ll),
         ("nwaves", _i),
+        ("long_rows", _ll),
     ]
 
     def as_dict(self) -> dict:

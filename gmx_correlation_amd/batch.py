@@ -28,6 +28,10 @@ BATCH_PATHS = [(2, 1), (3, 2), (3, 4), (0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (
 PREFIX_MAX_FRAMES = 2048
 LONG_MAX_FRAMES = 8192
 PREFIX_MAX_T = 256
+# Largest share of a trajectory's rows that may fall beyond the 512-frame
+# prefix of kind 3 before the 1024-frame prefix is faster: a long row costs
+# about 4x a prefix row and the longer prefix about 1.4x (break-even near 6%).
+LONG_SHARE_MAX = 0.04
 
 
 def fitting_paths(nframes: int, k: int) -> list[tuple[int, int]]:
@@ -40,13 +44,21 @@ def fitting_paths(nframes: int, k: int) -> list[tuple[int, int]]:
     return [(kd, s) for kd, s in BATCH_PATHS if fits(kd, s)]
 
 
-def tune_variant(rows: np.ndarray, psi: np.ndarray, k: int, natoms_probe: int = 6,
+def tune_variant(rows: np.ndarray, psi: np.ndarray, k: int, natoms_probe: int = 10,
                  reps: int = 2, paths=None) -> tuple[tuple[int, int], dict]:
     """Time every batch code path that covers the trajectory's frame count on
-    the pairs of its first `natoms_probe` atoms (standalone launches, HIP
-    events) and return the fastest (kind, s4) with all timings in ms."""
+    the pairs of `natoms_probe` atoms spread evenly over the trajectory
+    (standalone launches, HIP events; pair kernel plus long-row pass and FP64
+    recount) and return the fastest (kind, s4) with all timings in ms.
+    Spread atoms stand for the all-pairs mix: most pairs of a chain are far
+    apart and weakly coupled, and their (k+1)-th neighbour lies deepest in
+    the sorted rows."""
     N = rows.shape[1]
-    m = min(natoms_probe, rows.shape[0] // 3)
+    natoms = rows.shape[0] // 3
+    m = min(natoms_probe, natoms)
+    probe = np.unique(np.round(np.linspace(0, natoms - 1, m)).astype(np.int64))
+    m = len(probe)
+    sub = rows.reshape(natoms, 3, N)[probe].reshape(3 * m, N)
     cands = paths if paths is not None else fitting_paths(N, k)
     # keep only the two smallest of each kind: larger register tiles only add work
     by_kind: dict[int, list[int]] = {}
@@ -55,18 +67,25 @@ def tune_variant(rows: np.ndarray, psi: np.ndarray, k: int, natoms_probe: int = 
     cands = [(kd, s) for kd, ss in by_kind.items() for s in ss[:2]]
     a1, a2 = np.tril_indices(m, -1)
     timings = {}
+    long_share: dict[tuple[int, int], float] = {}
     with KsgPlan(m, N, k) as plan:
-        plan.load(rows[:3 * m], psi)
+        plan.load(np.ascontiguousarray(sub), psi)
         for kind, s4 in cands:
             plan.set_variant(kind, s4)
             plan.run_pairs(a1, a2)                      # warm-up
             t = []
             for _ in range(reps):
                 plan.run_pairs(a1, a2)
-                t.append(plan.stats()["ms_pairs"])
+                st = plan.stats()
+                t.append(st["ms_pairs"] + st["ms_fixup"])
+            long_share[(kind, s4)] = st["long_rows"] / float(len(a1) * N)
             timings[f"{kind}:{s4}"] = float(np.min(t))
     best = min(timings, key=timings.get)
     kind, s4 = (int(v) for v in best.split(":"))
+    if (kind, s4) == (3, 2) and long_share.get((3, 2), 0.0) > LONG_SHARE_MAX:
+        # a probe of a few pairs leaves the GPU idle and hides the cost of
+        # the long rows, which a full launch pays at about 4x a prefix row
+        kind, s4 = 3, 4
     return (kind, s4), timings
 
 

@@ -191,11 +191,11 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
     std::fflush(stderr);
     hipStream_t s;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipEvent_t ev[5];
+    hipEvent_t ev[6];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(ev[0], s));
     double ms_total = 0.0, ms_pairs_total = 0.0, ms_build_total = 0.0, ms_fix_total = 0.0;
-    long long fix_rows = 0, ties = 0, items_total = 0;
+    long long fix_rows = 0, ties = 0, long_rows = 0, items_total = 0;
     int rc = GMXCORR_OK;
     for (size_t wv = 0; wv < waves.size() && rc == GMXCORR_OK; ++wv) {
         const std::vector<int>& ts = waves[wv];
@@ -204,13 +204,13 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
         gmxcorr::TrajDesc* dDesc = nullptr;
         int *dItemTraj = nullptr, *dA1 = nullptr, *dA2 = nullptr;
         double *dPartial = nullptr, *dFixsum = nullptr, *dMI = nullptr;
-        gmxcorr::FixEntry* dFix = nullptr;
+        gmxcorr::FixEntry *dFix = nullptr, *dLong = nullptr;
         unsigned* dCnt = nullptr;
         unsigned long long* dQueue = nullptr;
         auto cleanup = [&]() {
             (void)hipStreamSynchronize(s);
             dfree(dDesc); dfree(dItemTraj); dfree(dA1); dfree(dA2); dfree(dPartial);
-            dfree(dFixsum); dfree(dMI); dfree(dFix); dfree(dCnt); dfree(dQueue);
+            dfree(dFixsum); dfree(dMI); dfree(dFix); dfree(dLong); dfree(dCnt); dfree(dQueue);
         };
         auto body = [&]() -> int {
             std::fprintf(stderr, "gmxcorr batch: wave %zu/%zu: building rows of %zu trajectories\n",
@@ -284,20 +284,25 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             HIPCHK(hipMalloc(&dMI, nitems * sizeof(double)));
             HIPCHK(hipMalloc(&dCnt, 4 * sizeof(unsigned)));
             HIPCHK(hipMalloc(&dQueue, sizeof(unsigned long long)));
-            // recount queue for 1% of the wave's rows (0.04-0.2% measured)
+            // recount queue for 1% of the wave's rows (0.04-0.2% measured),
+            // long-row queue for 6% (the tuner keeps the long-row share of a
+            // trajectory below 4%); an overflow re-runs the wave
             long long rows_w = 0;
             for (size_t q = 0; q < ts.size(); ++q)
                 rows_w += b->traj[ts[q]].npairs() * (long long)b->traj[ts[q]].N;
             unsigned fixcap = (unsigned)std::min<long long>(std::max<long long>(1 << 18, rows_w / 100),
                                                             64LL << 20);
+            unsigned longcap = (unsigned)std::min<long long>(
+                std::max<long long>(1 << 18, rows_w / 16), 256LL << 20);
             HIPCHK(hipMalloc(&dFix, (size_t)fixcap * sizeof(gmxcorr::FixEntry)));
+            HIPCHK(hipMalloc(&dLong, (size_t)longcap * sizeof(gmxcorr::FixEntry)));
             HIPCHK(hipMemcpyAsync(dDesc, desc.data(), desc.size() * sizeof(gmxcorr::TrajDesc),
                                   hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(dItemTraj, it.data(), nitems * sizeof(int), hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(dA1, i1.data(), nitems * sizeof(int), hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(dA2, i2.data(), nitems * sizeof(int), hipMemcpyHostToDevice, s));
             gmxcorr::KsgBatchArgs ba{dDesc, dItemTraj, dA1, dA2, nitems, dPartial, dFix, dCnt, fixcap,
-                                     dQueue};
+                                     dQueue, dLong, longcap};
             gmxcorr::FixupArgs fa{};
             fa.traj = dDesc;
             fa.item_traj = dItemTraj;
@@ -305,10 +310,13 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             fa.a2 = dA2;
             fa.fixsum = dFixsum;
             fa.counters = dCnt;
-            unsigned counters[2] = {0, 0};
-            for (int attempt = 0; attempt < 2; ++attempt) {
+            HIPCHK(hipEventRecord(ev[5], s));
+            unsigned counters[3] = {0, 0, 0};
+            for (int attempt = 0; attempt < 3; ++attempt) {
                 ba.fixq = dFix;
                 ba.fixcap = fixcap;
+                ba.longq = dLong;
+                ba.longcap = longcap;
                 fa.fixq = dFix;
                 HIPCHK(gmxcorr::launch_ksg_clear(dFixsum, nitems, dCnt, dQueue, s));
                 HIPCHK(hipEventRecord(ev[2], s));
@@ -316,13 +324,26 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                 HIPCHK(hipEventRecord(ev[3], s));
                 HIPCHK(hipMemcpyAsync(counters, dCnt, sizeof(counters), hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
-                if (counters[0] <= fixcap) break;
-                std::fprintf(stderr, "gmxcorr batch: recount queue overflow (%u rows), re-running the wave\n",
-                             counters[0]);
-                dfree(dFix);
-                fixcap = counters[0] + counters[0] / 4 + 1024;
-                HIPCHK(hipMalloc(&dFix, (size_t)fixcap * sizeof(gmxcorr::FixEntry)));
-                if (attempt == 1) return gmxcorr::set_error(GMXCORR_EHIP, "recount queue overflow");
+                if (counters[2] <= longcap) {
+                    // the long rows may queue tie rows to the recount
+                    HIPCHK(gmxcorr::launch_ksg_batch_long_rows(ba, dFixsum, counters[2], s));
+                    HIPCHK(hipMemcpyAsync(counters, dCnt, sizeof(counters), hipMemcpyDeviceToHost, s));
+                    HIPCHK(hipStreamSynchronize(s));
+                    if (counters[0] <= fixcap) break;
+                }
+                if (attempt == 2) return gmxcorr::set_error(GMXCORR_EHIP, "recount queue overflow");
+                std::fprintf(stderr, "gmxcorr batch: queue overflow (recount %u, long %u rows), "
+                             "re-running the wave\n", counters[0], counters[2]);
+                if (counters[0] > fixcap) {
+                    dfree(dFix);
+                    fixcap = counters[0] + counters[0] / 4 + 1024;
+                    HIPCHK(hipMalloc(&dFix, (size_t)fixcap * sizeof(gmxcorr::FixEntry)));
+                }
+                if (counters[2] > longcap) {
+                    dfree(dLong);
+                    longcap = counters[2] + counters[2] / 4 + 1024;
+                    HIPCHK(hipMalloc(&dLong, (size_t)longcap * sizeof(gmxcorr::FixEntry)));
+                }
             }
             HIPCHK(gmxcorr::launch_ksg_fixup(fa, counters[0], s));
             HIPCHK(gmxcorr::launch_ksg_finalize(fa, nitems, dPartial, 1, dMI, s));
@@ -332,7 +353,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             HIPCHK(hipMemcpyAsync(mi.data(), dMI, nitems * sizeof(double), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             float tb = 0.f, tp = 0.f, tf = 0.f, tt = 0.f;
-            (void)hipEventElapsedTime(&tb, ev[1], ev[2]);
+            (void)hipEventElapsedTime(&tb, ev[1], ev[5]);
             (void)hipEventElapsedTime(&tp, ev[2], ev[3]);
             (void)hipEventElapsedTime(&tf, ev[3], ev[4]);
             (void)hipEventElapsedTime(&tt, ev[0], ev[4]);
@@ -342,10 +363,12 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             ms_total = tt;
             fix_rows += counters[0];
             ties += counters[1];
+            long_rows += counters[2];
             std::fprintf(stderr,
                          "gmxcorr batch: wave %zu/%zu: %zu trajectories, %lld pairs, rows %.1f ms, "
-                         "pairs %.1f ms, recount %.1f ms\n",
-                         wv + 1, waves.size(), ts.size(), nitems, tb, tp, tf);
+                         "pairs %.1f ms, long rows + recount %.1f ms (%u + %u rows)\n",
+                         wv + 1, waves.size(), ts.size(), nitems, tb, tp, tf, counters[2],
+                         counters[0]);
             std::fflush(stderr);
             for (size_t q = 0; q < ts.size(); ++q) {
                 HostTraj& h = b->traj[ts[q]];
@@ -371,6 +394,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
     b->st.npairs = items_total;
     b->st.fixup_rows = fix_rows;
     b->st.ties = ties;
+    b->st.long_rows = long_rows;
     return GMXCORR_OK;
 }
 

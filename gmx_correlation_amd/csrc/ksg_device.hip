@@ -394,8 +394,11 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
 
 // ---------------------------------------------------------------------------
 // Prefix stores of the long-trajectory path from the distance rows: one block
-// per row (a, i), the row's (value, frame) pairs bitonic-sorted in dynamic
-// LDS (M = power of two >= N; ties in frame order), then
+// of 512 threads per row (a, i), thread t holding frames 16t .. 16t+15
+// (P <= 8192).  A 4-pass 8-bit radix select finds the value v* of the 1024th
+// smallest frame; the frames below v*, and the first frames at v* in frame
+// order, are the prefix (the 1024 smallest in (value, frame) order).  They
+// are bitonic-sorted by (value, frame) in LDS, then
 //   LV[a][i][0..64) = 0, LV[a][i][64 + t] = t-th smallest value, t < kLongPrefix
 //   LI[a][i][t]     = its frame
 //   LS[a][i][m]     = LV value 64m + 63
@@ -406,18 +409,95 @@ __global__ __launch_bounds__(512) void ksg_build_prefix_kernel(const float* __re
                                                                uint16_t* __restrict__ LI,
                                                                float* __restrict__ LS)
 {
-    extern __shared__ uint64_t sk[];
+    __shared__ uint64_t sk[kLongPrefix];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t wtot[8];
+    __shared__ uint32_t sel[3];   // chosen bin, rank left within it, output slot counter
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    const int f0 = 16 * tid;
     // grid-stride over rows: a launch's work-item count must stay below 2^32
     for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {
     const uint32_t* d = reinterpret_cast<const uint32_t*>(D) + row * P;
-    int M = 64;
-    while (M < N) M <<= 1;
-    for (int t = threadIdx.x; t < M; t += blockDim.x)
-        sk[t] = t < N ? ((uint64_t)d[t] << 32) | (uint32_t)t : ((uint64_t)kPadBits << 32) | (uint32_t)t;
+    uint32_t v[16];
+    if (f0 < P) {   // P is a multiple of 256: all 16 frames or none
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint4 w = reinterpret_cast<const uint4*>(d + f0)[c];
+            v[4 * c] = w.x; v[4 * c + 1] = w.y; v[4 * c + 2] = w.z; v[4 * c + 3] = w.w;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = 0xffffffffu;   // above every value and the padding
+    }
+    // ---- radix select of the value of rank kLongPrefix - 1 (N > kLongPrefix)
+    uint32_t pre = 0u, msk = 0u;
+    uint32_t rank = kLongPrefix - 1;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        if (tid < 256) hist[tid] = 0u;
+        if (tid == 0) sel[2] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+            if ((v[e] & msk) == pre) atomicAdd(&hist[(v[e] >> shift) & 255u], 1u);
+        __syncthreads();
+        if (tid < kWave) {
+            const uint4 h = reinterpret_cast<const uint4*>(hist)[lane];
+            const uint32_t own = h.x + h.y + h.z + h.w;
+            uint32_t incl = own;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t up = (uint32_t)__shfl_up((int)incl, o, kWave);
+                if (lane >= o) incl += up;
+            }
+            const uint32_t excl = incl - own;
+            if (excl <= rank && rank < incl) {
+                uint32_t r = rank - excl, bin = 4u * lane;
+                if (r >= h.x) { r -= h.x; ++bin;
+                    if (r >= h.y) { r -= h.y; ++bin;
+                        if (r >= h.z) { r -= h.z; ++bin; } } }
+                sel[0] = bin;
+                sel[1] = r;
+            }
+        }
+        __syncthreads();
+        pre |= sel[0] << shift;
+        msk |= 255u << shift;
+        rank = sel[1];
+        __syncthreads();   // sel and hist are rewritten by the next pass
+    }
+    // pre = v*; the prefix holds the frames below v* and the first rank + 1
+    // frames at v* in frame order: their order among ties comes from an
+    // exclusive scan of the per-thread tie counts
+    int eq = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) eq += v[e] == pre ? 1 : 0;
+    int incl = eq;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int up = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += up;
+    }
+    if (lane == kWave - 1) wtot[wv] = (uint32_t)incl;
     __syncthreads();
-    for (int k = 2; k <= M; k <<= 1)
+    int tie = incl - eq;
+    for (int u = 0; u < wv; ++u) tie += (int)wtot[u];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        bool take = v[e] < pre;
+        if (v[e] == pre) {
+            take = tie <= (int)rank;
+            ++tie;
+        }
+        if (take) {
+            const uint32_t slot = atomicAdd(&sel[2], 1u);
+            if (slot < kLongPrefix) sk[slot] = ((uint64_t)v[e] << 32) | (uint32_t)(f0 + e);
+        }
+    }
+    __syncthreads();
+    // ---- bitonic sort of the kLongPrefix selected (value, frame) keys
+    for (int k = 2; k <= kLongPrefix; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
+            for (int t = tid; t < kLongPrefix / 2; t += blockDim.x) {
                 const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
                 const uint64_t x = sk[lo], y = sk[hi];
                 if ((x > y) == ((lo & k) == 0)) {
@@ -428,13 +508,12 @@ __global__ __launch_bounds__(512) void ksg_build_prefix_kernel(const float* __re
             __syncthreads();
         }
     uint32_t* lv = reinterpret_cast<uint32_t*>(LV) + row * kLongPitch;
-    for (int t = threadIdx.x; t < kLongPitch; t += blockDim.x)
+    for (int t = tid; t < kLongPitch; t += blockDim.x)
         lv[t] = t < 64 ? 0u : (uint32_t)(sk[t - 64] >> 32);
-    for (int t = threadIdx.x; t < kLongPrefix; t += blockDim.x)
+    for (int t = tid; t < kLongPrefix; t += blockDim.x)
         LI[row * kLongPrefix + t] = (uint16_t)(sk[t] & 0xffffu);
-    if (threadIdx.x < kLongSum)
-        reinterpret_cast<uint32_t*>(LS)[row * kLongSum + threadIdx.x] =
-            (uint32_t)(sk[64 * threadIdx.x + 63] >> 32);
+    if (tid < kLongSum)
+        reinterpret_cast<uint32_t*>(LS)[row * kLongSum + tid] = (uint32_t)(sk[64 * tid + 63] >> 32);
     __syncthreads();   // sk is rewritten by the next row
     }
 }
@@ -1620,8 +1699,8 @@ __global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
     o.counters = b.counters;
     o.fixq = b.fixq;
     o.fixcap = b.fixcap;
-    o.longq = nullptr;
-    o.longcap = 0u;
+    o.longq = b.longq;
+    o.longcap = b.longcap;
     o.rc_nx = nullptr;
     o.rc_ny = nullptr;
     o.rc_flag = nullptr;
@@ -1657,6 +1736,45 @@ __global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
         }
         if (threadIdx.x == 0) b.partial[item] = acc;
         __syncthreads();
+    }
+}
+
+// Long rows of the batch (kinds 2 and 3): one block per entry runs the
+// cooperative streaming row of its trajectory, the psi sum added to
+// fixsum[item].  The register tile S4 covers the trajectory's frame count.
+__global__ __launch_bounds__(256) void ksg_batch_long_rows_kernel(KsgBatchArgs b, double* fixsum,
+                                                                  unsigned nent)
+{
+    __shared__ CoopLds sh;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    RowOut o;
+    o.counters = b.counters;
+    o.fixq = b.fixq;
+    o.fixcap = b.fixcap;
+    o.longq = nullptr;
+    o.longcap = 0u;
+    o.rc_nx = nullptr;
+    o.rc_ny = nullptr;
+    o.rc_flag = nullptr;
+    for (unsigned e = blockIdx.x; e < nent; e += gridDim.x) {   // uniform over the block
+    const FixEntry fe = b.longq[e];
+    const long long item = fe.pair;
+    const TrajDesc d = b.traj[b.item_traj[item]];
+    o.psi = d.psi;
+    const float* Da = d.D + (size_t)b.a1[item] * d.N * d.P;
+    const float* Db = d.D + (size_t)b.a2[item] * d.N * d.P;
+    const int T = d.K + 1;
+    double acc;
+    if (d.N <= 3 * 1024)
+        acc = row_coop<3>(Da, Db, d.P, d.N, T, fe.row, item, sh, lane, w, o);
+    else if (d.N <= 4 * 1024)
+        acc = row_coop<4>(Da, Db, d.P, d.N, T, fe.row, item, sh, lane, w, o);
+    else if (d.N <= 6 * 1024)
+        acc = row_coop<6>(Da, Db, d.P, d.N, T, fe.row, item, sh, lane, w, o);
+    else
+        acc = row_coop<8>(Da, Db, d.P, d.N, T, fe.row, item, sh, lane, w, o);
+    if (threadIdx.x == 0) atomicAdd(&fixsum[item], acc);
     }
 }
 
@@ -1869,15 +1987,10 @@ hipError_t launch_build_prefix(const float* D, int natoms, int N, long long P, f
 {
     if (!D || !LV || !LI || !LS || N <= kSortedMaxFrames || N > kMaxFrames)
         return hipErrorInvalidValue;
-    int M = 64;
-    while (M < N) M <<= 1;
-    const size_t bytes = (size_t)M * sizeof(uint64_t);
-    hipError_t e = hipFuncSetAttribute((const void*)ksg_build_prefix_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
+    if (P % 256 != 0 || P < N || P > 16 * 512) return hipErrorInvalidValue;
     const long long rows = (long long)natoms * N;
-    hipLaunchKernelGGL(ksg_build_prefix_kernel, dim3(capped_grid(rows)), dim3(512), bytes, s, D, N,
-                       P, rows, LV, LI, LS);
+    hipLaunchKernelGGL(ksg_build_prefix_kernel, dim3(capped_grid(rows)), dim3(512), 0, s, D, N, P,
+                       rows, LV, LI, LS);
     return hipGetLastError();
 }
 
@@ -1998,6 +2111,16 @@ bool batch_variant_ok(int kind, int s4, int N, int K)
     if (kind == 0) return (s4 == 2 || s4 == 4 || s4 == 6 || s4 == 8) && N <= 256 * s4;
     if (kind == 1) return (s4 == 3 || s4 == 4 || s4 == 6 || s4 == 8) && N <= 1024 * s4;
     return false;
+}
+
+hipError_t launch_ksg_batch_long_rows(const KsgBatchArgs& b, double* fixsum, unsigned nlong,
+                                      hipStream_t s)
+{
+    if (nlong == 0) return hipSuccess;
+    if (!b.longq || !fixsum) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ksg_batch_long_rows_kernel, dim3(capped_grid(nlong)),
+                       dim3(kWave * kPairWaves), 0, s, b, fixsum, nlong);
+    return hipGetLastError();
 }
 
 hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s)

@@ -102,6 +102,8 @@ struct KsgBatchArgs {
     unsigned* counters;
     unsigned fixcap;
     unsigned long long* queue;
+    FixEntry* longq;   // rows whose (k+1)-th neighbour lies beyond the prefix
+    unsigned longcap;
 };
 
 // Largest N handled by the pair kernels (cooperative kernel, 4 x 2048 frames).
@@ -192,6 +194,10 @@ inline bool variant_prefix_store(int kind) { return kind == 3; }
 // at (5000, 100) a third of the rows overflow 512 and 1024 is 2.6x faster.
 inline int long_prefix_chunks(int N, int K) { return (long long)(K + 1) * N <= (1LL << 18) ? 2 : 4; }
 hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s);
+// The batch's long rows (b.longq, nlong entries): cooperative streaming row,
+// psi sum added to fixsum[item]; tie rows go to b.fixq.
+hipError_t launch_ksg_batch_long_rows(const KsgBatchArgs& b, double* fixsum, unsigned nlong,
+                                      hipStream_t s);
 hipError_t launch_selftest(int* out, hipStream_t s);
 
 }  // namespace gmxcorr

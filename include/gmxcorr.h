@@ -144,11 +144,12 @@ typedef struct {
     double ms_total;      /* first upload to last result, all waves */
     double ms_build;      /* uploads + row builds */
     double ms_pairs;      /* persistent pair launches */
-    double ms_fixup;      /* FP64 recount + finalise */
+    double ms_fixup;      /* long rows + FP64 recount + finalise */
     long long npairs;
     long long fixup_rows;
     long long ties;
     int nwaves;
+    long long long_rows;  /* rows beyond the sorted prefix (streaming pass) */
 } gmxcorr_batch_stats;
 int  gmxcorr_ksg_batch_create(gmxcorr_ksg_batch** out);
 /* Returns the trajectory's index (>= 0) or a negative status. */

@@ -68,6 +68,26 @@ def test_batch_explicit_paths(gpu, trajs, t):
         assert np.max(np.abs(mi - ref)) <= 1e-12, (kind, s4)
 
 
+@pytest.mark.parametrize("N,k,path", [(2000, 200, (2, 1)), (6000, 100, (3, 2))])
+def test_batch_long_rows(gpu, oracle_lib, N, k, path):
+    """Independent atoms, whose (k+1)-th joint neighbour lies beyond the
+    prefix (512 frames) in most rows: the batch kernel queues those rows and
+    the cooperative streaming pass resolves them; counts stay exact, so MI
+    equals the FP64 brute recount."""
+    rng = np.random.default_rng(N + k)
+    coords = rng.standard_normal((N, 4, 3)).astype(np.float32)
+    rows, psi = ksg.kraskov_prepare(ksg.center(coords))
+    b = batch.KsgBatch()
+    b.add(rows, psi, k, *path)
+    st = b.run(nblocks=64)
+    assert st["long_rows"] > 1000
+    mi, _, _ = b.result(0)
+    a1, a2 = np.tril_indices(4, -1)
+    for p in range(len(a1)):
+        v = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), k, method="brute")
+        assert abs(mi[p] - v) <= 1e-12, p
+
+
 def test_tuner_returns_a_fitting_path(gpu, trajs):
     rows, psi, k = trajs[3]                      # N = 5000
     (kind, s4), timings = batch.tune_variant(rows, psi, k, natoms_probe=4, reps=1)

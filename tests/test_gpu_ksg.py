@@ -313,3 +313,27 @@ def test_long_prefix_overflow_rows_take_cooperative_path(gpu, oracle_lib, k, s4)
         assert np.array_equal(d["nx"][p], ref["nx"]), p
         assert np.array_equal(d["ny"][p], ref["ny"]), p
         assert abs(mi[p] - v) <= 1e-12
+
+
+@pytest.mark.parametrize("s4", [2, 4])
+def test_long_trajectories_with_ties(gpu, oracle_lib, s4):
+    """Coordinates on a quarter-unit grid at 3000 frames: distance rows hold
+    long runs of equal values, at the prefix boundary too (the prefix keeps
+    the first tied frames in frame order), and the joint and marginal radii
+    tie; counts equal the FP64 recount."""
+    rng = np.random.default_rng(13)
+    coords = np.round(rng.standard_normal((3000, 4, 3)) * 4.0) / 4.0
+    rows, psi = ksg.kraskov_prepare(ksg.center(coords.astype(np.float32)))
+    a1 = np.array([1, 3, 2]); a2 = np.array([0, 2, 0])
+    with ksg.KsgPlan(4, 3000, 20) as plan:
+        plan.load(rows, psi)
+        plan.set_variant(3, s4)
+        mi, d = plan.run_pairs(a1, a2, detail=True)
+        st = plan.stats()
+    assert st["ties"] > 0
+    for p in range(len(a1)):
+        v, ref = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), 20, method="brute",
+                                     detail=True)
+        assert np.array_equal(d["nx"][p], ref["nx"]), p
+        assert np.array_equal(d["ny"][p], ref["ny"]), p
+        assert abs(mi[p] - v) <= 1e-12
