@@ -217,6 +217,8 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                          wv + 1, waves.size(), ts.size());
             std::fflush(stderr);
             std::vector<gmxcorr::TrajDesc> desc(ts.size());
+            int kinds = 0;   // code paths the wave's trajectories take
+            for (size_t q = 0; q < ts.size(); ++q) kinds |= 1 << b->traj[ts[q]].kind;
             HIPCHK(hipEventRecord(ev[1], s));
             long long doff = 0, xoff = 0;
             for (size_t q = 0; q < ts.size(); ++q) {
@@ -320,7 +322,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                 fa.fixq = dFix;
                 HIPCHK(gmxcorr::launch_ksg_clear(dFixsum, nitems, dCnt, dQueue, s));
                 HIPCHK(hipEventRecord(ev[2], s));
-                HIPCHK(gmxcorr::launch_ksg_batch(ba, nblocks, s));
+                HIPCHK(gmxcorr::launch_ksg_batch(ba, nblocks, kinds, s));
                 HIPCHK(hipEventRecord(ev[3], s));
                 HIPCHK(hipMemcpyAsync(counters, dCnt, sizeof(counters), hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));

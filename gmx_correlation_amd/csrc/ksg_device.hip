@@ -83,6 +83,21 @@ __device__ __forceinline__ int readlane(int v, int lane)
     return __builtin_amdgcn_readlane(v, lane);
 }
 
+// Copy of a wave-uniform value that the compiler knows to be uniform (held
+// in scalar registers): each 32-bit word through readfirstlane.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T& v)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    int w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
 // Sum over the wave, returned uniform: quad swaps and row mirrors reduce each
 // row of 16 lanes, four readlanes combine the rows.
 __device__ __forceinline__ int wave_sum(int v)
@@ -1422,8 +1437,8 @@ __device__ __forceinline__ double row_coop(const float* Da, const float* Db, lon
     return contrib;
 }
 
-// Work item of virtual block vb (the kernels loop vb = blockIdx.x, +gridDim.x,
-// .. below roundup(n, 8), the grid a multiple of 8): each XCD (vb mod 8)
+// Work item of virtual block vb = vb0 + blockIdx.x (vb below roundup(n, 8);
+// vb0 and every grid but the last a multiple of 8): each XCD (vb mod 8)
 // receives a contiguous range of items, so neighbouring items keep their
 // shared atoms' rows in that XCD's L2.
 __device__ __forceinline__ long long xcd_item(long long vb, long long n)
@@ -1448,15 +1463,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // pairs share their first atom).
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
     const long long nitems = nquads * kRowSplit;
-    // grid-stride over virtual blocks: a launch's work-item count must stay
-    // below 2^32
-    for (long long vb = blockIdx.x; vb < (nitems + 7) / 8 * 8; vb += gridDim.x) {
+    // virtual block: the launcher splits a grid above 2^22 blocks into
+    // launches at offsets vb0 (a multiple of 8)
+    const long long vb = args.vb0 + blockIdx.x;
     const long long item = xcd_item(vb, nitems);
-    if (item >= nitems) continue;
+    if (item >= nitems) return;
     const long long q = item / kRowSplit;
     const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
     const int part = (int)(item % kRowSplit);
-    if (p < 0 || p >= args.npairs) continue;   // no block barriers below
+    if (p < 0 || p >= args.npairs) return;   // no block barriers below
     const int N = args.N, T = args.K + 1;
     const int i0 = (int)((long long)N * part / kRowSplit);
     const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
@@ -1471,7 +1486,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     sr.SMb = args.SM + (size_t)args.a2[p] * N * kSumPitch;
     const double acc = rows_single<S4>(Da, Db, sr, args.P, N, T, i0, i1, 1, p, lds[w], lane, o);
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
-    }
 }
 
 __device__ __forceinline__ PrefixRows prefix_rows(const float* D, const float* SD, const float* SM,
@@ -1501,17 +1515,17 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
     const long long nitems = nquads * kRowSplit;
-    // grid-stride over virtual blocks: a launch's work-item count must stay
-    // below 2^32
-    for (long long vb = blockIdx.x; vb < (nitems + 7) / 8 * 8; vb += gridDim.x) {
+    // virtual block: the launcher splits a grid above 2^22 blocks into
+    // launches at offsets vb0 (a multiple of 8)
+    const long long vb = args.vb0 + blockIdx.x;
     const long long item = xcd_item(vb, nitems);
-    if (item >= nitems) continue;
+    if (item >= nitems) return;
     // row-range major: the blocks in flight on one XCD are consecutive tiles
     // (sharing their first two atoms) over the same row range
     const long long q = item % nquads;
     const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
     const int part = (int)(item / nquads);
-    if (p < 0 || p >= args.npairs) continue;   // no block barriers below
+    if (p < 0 || p >= args.npairs) return;   // no block barriers below
     const int N = args.N, T = args.K + 1;
     const int i0 = (int)((long long)N * part / kRowSplit);
     const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
@@ -1519,7 +1533,6 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
                                       args.a2[p]);
     const double acc = rows_prefix(pr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
-    }
 }
 
 // Long-trajectory prefix kernel (kind 3): block and order as
@@ -1533,15 +1546,15 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
     const long long nitems = nquads * kRowSplit;
-    // grid-stride over virtual blocks: a launch's work-item count must stay
-    // below 2^32
-    for (long long vb = blockIdx.x; vb < (nitems + 7) / 8 * 8; vb += gridDim.x) {
+    // virtual block: the launcher splits a grid above 2^22 blocks into
+    // launches at offsets vb0 (a multiple of 8)
+    const long long vb = args.vb0 + blockIdx.x;
     const long long item = xcd_item(vb, nitems);
-    if (item >= nitems) continue;
+    if (item >= nitems) return;
     const long long q = item % nquads;
     const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
     const int part = (int)(item / nquads);
-    if (p < 0 || p >= args.npairs) continue;   // no block barriers below
+    if (p < 0 || p >= args.npairs) return;   // no block barriers below
     const int N = args.N, T = args.K + 1;
     const int i0 = (int)((long long)N * part / kRowSplit);
     const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
@@ -1549,7 +1562,6 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
                                   args.a2[p]);
     const double acc = rows_long<C>(lr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
-    }
 }
 
 // Rows the long-trajectory kernel queued to longq (the (k+1)-th neighbour
@@ -1604,9 +1616,9 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
     __shared__ CoopLds sh;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    for (long long vb = blockIdx.x; vb < (args.npairs + 7) / 8 * 8; vb += gridDim.x) {
+    const long long vb = args.vb0 + blockIdx.x;
     const long long p = xcd_item(vb, args.npairs);
-    if (p >= args.npairs) continue;   // uniform over the block
+    if (p >= args.npairs) return;   // uniform over the block
     const int N = args.N, T = args.K + 1;
     const float* Da = args.D + (size_t)args.a1[p] * N * args.P;
     const float* Db = args.D + (size_t)args.a2[p] * N * args.P;
@@ -1614,7 +1626,6 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
     double acc = 0.0;
     for (int i = 0; i < N; ++i) acc += row_coop<S4>(Da, Db, args.P, N, T, i, p, sh, lane, w, o);
     if (threadIdx.x < kRowSplit) args.partial[p * kRowSplit + threadIdx.x] = threadIdx.x == 0 ? acc : 0.0;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1648,11 +1659,11 @@ __device__ __forceinline__ double batch_single(const TrajDesc& d, const float* D
 }
 
 __device__ __forceinline__ double batch_prefix(const TrajDesc& d, int a1, int a2, long long item,
-                                               BatchLds& u, double* wsum, int lane, int w,
+                                               PrefixLds& lds, double* wsum, int lane, int w,
                                                const RowOut& o)
 {
     const PrefixRows pr = prefix_rows(d.D, d.SD, d.SM, d.SI, d.P, d.N, a1, a2);
-    const double acc = rows_prefix(pr, d.N, d.K + 1, w, d.N, kPairWaves, item, u.prefix[w], lane, o);
+    const double acc = rows_prefix(pr, d.N, d.K + 1, w, d.N, kPairWaves, item, lds, lane, o);
     if (lane == 0) wsum[w] = acc;
     __syncthreads();
     return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
@@ -1661,17 +1672,22 @@ __device__ __forceinline__ double batch_prefix(const TrajDesc& d, int a1, int a2
 // Kind 3 in the batched kernel: wave w runs rows w, w+4, ... in chunks of at
 // most kPrefix rows.
 __device__ __forceinline__ double batch_long(const TrajDesc& d, int a1, int a2, long long item,
-                                             BatchLds& u, double* wsum, int lane, int w,
+                                             uint32_t* hist, double* wsum, int lane, int w,
                                              const RowOut& o)
 {
     const LongRows lr = long_rows(d.D, d.LV, d.LI, d.LS, d.P, d.N, a1, a2);
     double acc = 0.0;
-    for (int c0 = 0; c0 < d.N; c0 += kPairWaves * kPrefix)
+    for (int c0 = 0; c0 < d.N; c0 += kPairWaves * kPrefix) {
+        // opaque per chunk: lane-derived addresses are recomputed, not held
+        // across chunks (115 VGPRs instead of 153)
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
         acc += d.s4 == 2
                    ? rows_long<2>(lr, d.N, d.K + 1, c0 + w, min(d.N, c0 + kPairWaves * kPrefix),
-                                  kPairWaves, item, u.single[w], lane, o)
+                                  kPairWaves, item, hist, ln, o)
                    : rows_long<4>(lr, d.N, d.K + 1, c0 + w, min(d.N, c0 + kPairWaves * kPrefix),
-                                  kPairWaves, item, u.single[w], lane, o);
+                                  kPairWaves, item, hist, ln, o);
+    }
     if (lane == 0) wsum[w] = acc;
     __syncthreads();
     return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
@@ -1688,13 +1704,42 @@ __device__ __forceinline__ double batch_coop(const TrajDesc& d, const float* Da,
     return acc;
 }
 
+// LDS of the batch kernel for the code paths it compiles (bit k = kind k):
+// the streaming paths' union, the prefix path's staged rows, or the
+// long-trajectory path's histograms alone.
+template <int Paths>
+struct BatchShared {
+    BatchLds u;
+    __device__ PrefixLds& pre(int w) { return u.prefix[w]; }
+    __device__ uint32_t* hist(int w) { return u.single[w]; }
+};
+template <>
+struct BatchShared<4> {
+    PrefixLds prefix[kPairWaves];
+    __device__ PrefixLds& pre(int w) { return prefix[w]; }
+};
+template <>
+struct BatchShared<12> {
+    PrefixLds prefix[kPairWaves];
+    __device__ PrefixLds& pre(int w) { return prefix[w]; }
+    __device__ uint32_t* hist(int w) { return prefix[w].hist; }
+};
+template <>
+struct BatchShared<8> {
+    __attribute__((aligned(4 * kPBins))) uint32_t h[kPairWaves][kPBins];
+    __device__ uint32_t* hist(int w) { return h[w]; }
+};
+
+// Paths: the code paths compiled in (bit k = kind k).  A kernel's register
+// and LDS allocation is that of its largest path, so the host launches the
+// smallest instantiation that covers the kinds of a launch's trajectories.
+template <int Paths>
 __global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
 {
-    __shared__ BatchLds u;
+    __shared__ __attribute__((aligned(4 * kPBins))) BatchShared<Paths> sh;
     __shared__ double wsum[kPairWaves];
     __shared__ long long next;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int w = threadIdx.x / kWave;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     RowOut o;
     o.counters = b.counters;
     o.fixq = b.fixq;
@@ -1707,31 +1752,47 @@ __global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
     for (;;) {
         if (threadIdx.x == 0) next = (long long)atomicAdd(b.queue, 1ULL);
         __syncthreads();
-        const long long item = next;
+        const long long item = uniform_copy(next);
         __syncthreads();
         if (item >= b.nitems) break;
-        const TrajDesc d = b.traj[b.item_traj[item]];
+        // item, trajectory and atoms are uniform over the block
+        // opaque per item: keeps lane-derived addresses from being hoisted
+        // out of the item loop and held in registers across items
+        int lane = (int)(threadIdx.x & (kWave - 1));
+        asm volatile("" : "+v"(lane));
+        const TrajDesc d = uniform_copy(b.traj[uniform_copy(b.item_traj[item])]);
+        const int ia1 = uniform_copy(b.a1[item]), ia2 = uniform_copy(b.a2[item]);
         o.psi = d.psi;
-        const float* Da = d.D + (size_t)b.a1[item] * d.N * d.P;
-        const float* Db = d.D + (size_t)b.a2[item] * d.N * d.P;
+        const float* Da = d.D + (size_t)ia1 * d.N * d.P;
+        const float* Db = d.D + (size_t)ia2 * d.N * d.P;
         double acc = 0.0;
-        if (d.kind == 2) {
-            acc = batch_prefix(d, b.a1[item], b.a2[item], item, u, wsum, lane, w, o);
-        } else if (d.kind == 3) {
-            acc = batch_long(d, b.a1[item], b.a2[item], item, u, wsum, lane, w, o);
-        } else if (d.kind == 0) {
-            switch (d.s4) {
-                case 2: acc = batch_single<2>(d, Da, Db, item, u, wsum, lane, w, o); break;
-                case 4: acc = batch_single<4>(d, Da, Db, item, u, wsum, lane, w, o); break;
-                case 6: acc = batch_single<6>(d, Da, Db, item, u, wsum, lane, w, o); break;
-                default: acc = batch_single<8>(d, Da, Db, item, u, wsum, lane, w, o); break;
+        // the branches are uniform over the block (one item per block)
+        if constexpr ((Paths & 4) != 0) {
+            if (d.kind == 2) acc = batch_prefix(d, ia1, ia2, item, sh.pre(w),
+                                                wsum, lane, w, o);
+        }
+        if constexpr ((Paths & 8) != 0) {
+            if (d.kind == 3) acc = batch_long(d, ia1, ia2, item, sh.hist(w), wsum,
+                                              lane, w, o);
+        }
+        if constexpr ((Paths & 1) != 0) {
+            if (d.kind == 0) {
+                switch (d.s4) {
+                    case 2: acc = batch_single<2>(d, Da, Db, item, sh.u, wsum, lane, w, o); break;
+                    case 4: acc = batch_single<4>(d, Da, Db, item, sh.u, wsum, lane, w, o); break;
+                    case 6: acc = batch_single<6>(d, Da, Db, item, sh.u, wsum, lane, w, o); break;
+                    default: acc = batch_single<8>(d, Da, Db, item, sh.u, wsum, lane, w, o); break;
+                }
             }
-        } else {
-            switch (d.s4) {
-                case 3: acc = batch_coop<3>(d, Da, Db, item, u, lane, w, o); break;
-                case 4: acc = batch_coop<4>(d, Da, Db, item, u, lane, w, o); break;
-                case 6: acc = batch_coop<6>(d, Da, Db, item, u, lane, w, o); break;
-                default: acc = batch_coop<8>(d, Da, Db, item, u, lane, w, o); break;
+        }
+        if constexpr ((Paths & 2) != 0) {
+            if (d.kind == 1) {
+                switch (d.s4) {
+                    case 3: acc = batch_coop<3>(d, Da, Db, item, sh.u, lane, w, o); break;
+                    case 4: acc = batch_coop<4>(d, Da, Db, item, sh.u, lane, w, o); break;
+                    case 6: acc = batch_coop<6>(d, Da, Db, item, sh.u, lane, w, o); break;
+                    default: acc = batch_coop<8>(d, Da, Db, item, sh.u, lane, w, o); break;
+                }
             }
         }
         if (threadIdx.x == 0) b.partial[item] = acc;
@@ -2046,16 +2107,24 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4)
     const long long nitems =
         kind != 1 ? (a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves) * kRowSplit
                   : a.npairs;
-    const dim3 grid(capped_grid(nitems)), block(kWave * kPairWaves);
-#define GMX_SINGLE(V) case V: hipLaunchKernelGGL(ksg_pair_kernel<V>, grid, block, 0, s, a); break;
-#define GMX_COOP(V) case V: hipLaunchKernelGGL(ksg_pair_coop_kernel<V>, grid, block, 0, s, a); break;
+    // one block per virtual block; a dispatch carries fewer than 2^32
+    // work-items, so grids above 2^22 blocks go out in several launches
+    const long long nvb = (nitems + 7) / 8 * 8;
+    const long long cap = 1LL << 22;
+    const dim3 block(kWave * kPairWaves);
+    for (long long vb0 = 0; vb0 < nvb; vb0 += cap) {
+    KsgArgs c = a;
+    c.vb0 = vb0;
+    const dim3 grid((unsigned)(nvb - vb0 < cap ? nvb - vb0 : cap));
+#define GMX_SINGLE(V) case V: hipLaunchKernelGGL(ksg_pair_kernel<V>, grid, block, 0, s, c); break;
+#define GMX_COOP(V) case V: hipLaunchKernelGGL(ksg_pair_coop_kernel<V>, grid, block, 0, s, c); break;
     if (kind == 3) {
         if (s4 == 2)
-            hipLaunchKernelGGL(ksg_pair_long_kernel<2>, grid, block, 0, s, a);
+            hipLaunchKernelGGL(ksg_pair_long_kernel<2>, grid, block, 0, s, c);
         else
-            hipLaunchKernelGGL(ksg_pair_long_kernel<4>, grid, block, 0, s, a);
+            hipLaunchKernelGGL(ksg_pair_long_kernel<4>, grid, block, 0, s, c);
     } else if (kind == 2) {
-        hipLaunchKernelGGL(ksg_pair_prefix_kernel, grid, block, 0, s, a);
+        hipLaunchKernelGGL(ksg_pair_prefix_kernel, grid, block, 0, s, c);
     } else if (kind == 0) {
         switch (s4) {
             GMX_SINGLE(1) GMX_SINGLE(2) GMX_SINGLE(3) GMX_SINGLE(4) GMX_SINGLE(5) GMX_SINGLE(6)
@@ -2070,7 +2139,10 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4)
     }
 #undef GMX_SINGLE
 #undef GMX_COOP
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s)
@@ -2123,9 +2195,18 @@ hipError_t launch_ksg_batch_long_rows(const KsgBatchArgs& b, double* fixsum, uns
     return hipGetLastError();
 }
 
-hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s)
+hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, int kinds, hipStream_t s)
 {
-    hipLaunchKernelGGL(ksg_batch_kernel, dim3(nblocks), dim3(kWave * kPairWaves), 0, s, b);
+    if (kinds & ~0xf) return hipErrorInvalidValue;
+    const dim3 grid(nblocks), block(kWave * kPairWaves);
+    if (kinds == 4)
+        hipLaunchKernelGGL(ksg_batch_kernel<4>, grid, block, 0, s, b);
+    else if (kinds == 8)
+        hipLaunchKernelGGL(ksg_batch_kernel<8>, grid, block, 0, s, b);
+    else if ((kinds & 3) == 0)
+        hipLaunchKernelGGL(ksg_batch_kernel<12>, grid, block, 0, s, b);
+    else
+        hipLaunchKernelGGL(ksg_batch_kernel<15>, grid, block, 0, s, b);
     return hipGetLastError();
 }
 

@@ -53,6 +53,7 @@ struct KsgArgs {
     int           N;
     int           K;
     double        psiN_phiK; // psi[N] + (psi[K] - 1/K)
+    long long     vb0;       // first virtual block of this launch (set by the launcher)
 };
 
 // One trajectory's device data and shape.  kind/s4 select the pair-kernel
@@ -193,7 +194,8 @@ inline bool variant_prefix_store(int kind) { return kind == 3; }
 // (N, k) = (2500, 100), (3000, 50), (8000, 10); the two tie at (6000, 50);
 // at (5000, 100) a third of the rows overflow 512 and 1024 is 2.6x faster.
 inline int long_prefix_chunks(int N, int K) { return (long long)(K + 1) * N <= (1LL << 18) ? 2 : 4; }
-hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, hipStream_t s);
+// kinds: bit k set when some trajectory of the launch takes kind k.
+hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, int kinds, hipStream_t s);
 // The batch's long rows (b.longq, nlong entries): cooperative streaming row,
 // psi sum added to fixsum[item]; tie rows go to b.fixq.
 hipError_t launch_ksg_batch_long_rows(const KsgBatchArgs& b, double* fixsum, unsigned nlong,
