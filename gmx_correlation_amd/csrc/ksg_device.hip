@@ -1535,9 +1535,14 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
 }
 
-// Long-trajectory prefix kernel (kind 3): block and order as
-// ksg_pair_prefix_kernel; 1 KB of LDS per wave.  C: 256-frame chunks of the
-// prefix read per row (2: 512 frames, 4: the whole 1024-frame store).
+// Long-trajectory prefix kernel (kind 3): four pairs per block (a 2x2 atom
+// tile, as ksg_pair_prefix_kernel), 1 KB of LDS per wave.  C: 256-frame
+// chunks of the prefix read per row (2: 512 frames, 4: the whole 1024-frame
+// store).  A block runs one chunk of kLongChunk frame rows, and blocks are
+// ordered chunk-major: the blocks in flight on one XCD run consecutive tiles,
+// which share their first two atoms, over the same kLongChunk rows, so those
+// atoms' distance rows and prefixes are read from that XCD's L2.  Each
+// wave's sum over its rows is added to fixsum[pair].
 template <int C>
 __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
 {
@@ -1545,7 +1550,8 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
-    const long long nitems = nquads * kRowSplit;
+    const int nchunk = (args.N + kLongChunk - 1) / kLongChunk;
+    const long long nitems = nquads * nchunk;
     // virtual block: the launcher splits a grid above 2^22 blocks into
     // launches at offsets vb0 (a multiple of 8)
     const long long vb = args.vb0 + blockIdx.x;
@@ -1553,15 +1559,15 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
     if (item >= nitems) return;
     const long long q = item % nquads;
     const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
-    const int part = (int)(item / nquads);
+    const int chunk = (int)(item / nquads);
     if (p < 0 || p >= args.npairs) return;   // no block barriers below
     const int N = args.N, T = args.K + 1;
-    const int i0 = (int)((long long)N * part / kRowSplit);
-    const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
+    const int i0 = chunk * kLongChunk;
+    const int i1 = min(N, i0 + kLongChunk);
     const LongRows lr = long_rows(args.D, args.LV, args.LI, args.LS, args.P, N, args.a1[p],
                                   args.a2[p]);
     const double acc = rows_long<C>(lr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
-    if (lane == 0) args.partial[p * kRowSplit + part] = acc;
+    if (lane == 0) atomicAdd(&args.fixsum[p], acc);
 }
 
 // Rows the long-trajectory kernel queued to longq (the (k+1)-th neighbour
@@ -2091,7 +2097,7 @@ void default_variant(int N, int K, int& kind, int& s4)
     if (s4 < 2) s4 = 2;
 }
 
-hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4)
+hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4, int* nsplit)
 {
     if (kind < 0) {
         default_variant(a.N, a.K, kind, s4);
@@ -2104,9 +2110,12 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4)
     if (!variant_fits(kind, s4, a.N, a.K)) return hipErrorInvalidValue;
     if (variant_sorted(kind) && !(a.SD && a.SM && a.SI)) return hipErrorInvalidValue;
     if (variant_prefix_store(kind) && !(a.LV && a.LI && a.LS)) return hipErrorInvalidValue;
-    const long long nitems =
-        kind != 1 ? (a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves) * kRowSplit
-                  : a.npairs;
+    const long long nquads = a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves;
+    const long long nitems = kind == 1   ? a.npairs
+                             : kind == 3 ? nquads * ((a.N + kLongChunk - 1) / kLongChunk)
+                                         : nquads * kRowSplit;
+    // partial sums per pair: kRowSplit slots, or none (kind 3 adds into fixsum)
+    if (nsplit) *nsplit = kind == 3 ? 0 : kRowSplit;
     // one block per virtual block; a dispatch carries fewer than 2^32
     // work-items, so grids above 2^22 blocks go out in several launches
     const long long nvb = (nitems + 7) / 8 * 8;

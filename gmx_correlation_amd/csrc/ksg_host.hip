@@ -237,7 +237,8 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
         f.fixq = pl->dFix;
         HIPCHK(gmxcorr::launch_ksg_clear(pl->dFixsum, npairs, pl->dCounters, nullptr, pl->stream));
         HIPCHK(hipEventRecord(pl->ev[2], pl->stream));
-        HIPCHK(gmxcorr::launch_ksg_pairs(a, pl->stream, pl->var_kind, pl->var_s4));
+        int nsplit = gmxcorr::kRowSplit;
+        HIPCHK(gmxcorr::launch_ksg_pairs(a, pl->stream, pl->var_kind, pl->var_s4, &nsplit));
         HIPCHK(hipEventRecord(pl->ev[3], pl->stream));
         unsigned counters[3] = {0, 0, 0};
         HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,
@@ -262,7 +263,7 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
             continue;
         }
         HIPCHK(gmxcorr::launch_ksg_fixup(f, counters[0], pl->stream));
-        HIPCHK(gmxcorr::launch_ksg_finalize(f, npairs, pl->dPartial, gmxcorr::kRowSplit,
+        HIPCHK(gmxcorr::launch_ksg_finalize(f, npairs, pl->dPartial, nsplit,
                                                   pl->dMI, pl->stream));
         HIPCHK(hipEventRecord(pl->ev[4], pl->stream));
         HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,

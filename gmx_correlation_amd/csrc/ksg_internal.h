@@ -110,8 +110,9 @@ struct KsgBatchArgs {
 // Largest N handled by the pair kernels (cooperative kernel, 4 x 2048 frames).
 constexpr int kMaxFrames = 8192;
 
-// The single-trajectory pair kernels write kRowSplit partial sums per pair
-// (frame rows split into that many ranges); partial holds npairs*kRowSplit.
+// The single-trajectory pair kernels of kinds 0-2 write kRowSplit partial
+// sums per pair (frame rows split into that many ranges); partial holds
+// npairs*kRowSplit.  Kind 3 adds its row sums into fixsum instead.
 constexpr int kRowSplit = 16;
 
 // Distance-row pitch in floats: a multiple of 256 frames, so the single-wave
@@ -139,6 +140,8 @@ __host__ __device__ inline long long sorted_pitch(int N)
 // zeros (pitch kLongPitch floats), their frames (u16), and kLongSum summary
 // values (value 64m + 63).  6.4 KB per row instead of a full sorted row.
 constexpr int kLongPrefix = 1024;
+// Frame rows per block of the long-trajectory pair kernel.
+constexpr int kLongChunk = 32;
 constexpr int kLongPitch = 64 + kLongPrefix;
 constexpr int kLongSum = kLongPrefix / 64;
 
@@ -171,7 +174,10 @@ hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
 // kind < 0: default.
 bool variant_fits(int kind, int s4, int N, int K);
 void default_variant(int N, int K, int& kind, int& s4);
-hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind = -1, int s4 = 0);
+// nsplit (optional): partial-sum slots per pair the launch wrote, for the
+// finalise step (0: all of it went to fixsum).
+hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind = -1, int s4 = 0,
+                            int* nsplit = nullptr);
 hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s);
 // Rows that the sorted-prefix path queued to a.longq (k-th neighbour beyond
 // the prefix): the streaming one-wave path per row, psi sums into a.fixsum.
