@@ -825,9 +825,11 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         if (lane == 0) nL = ld_lane<uint32_t>(base, (uint32_t)(T - 1) * 4u);
         nsm = ld_lane<uint32_t>((sa ? pr.SMb : pr.SMa) + row * kSumPitch, lane4);
         const uint32_t* rq = (sa ? pr.Db : pr.Da) + row * pr.P;
+        // all 8 chunks: past the row's last chunk the last one again (a
+        // row of P < 2048 floats), staged to LDS slots no gather reads
 #pragma unroll
         for (int m = 0; m < 8; ++m)
-            if (m < nblk) nq[m] = ld_lane<uint4>(rq, lane4 * 4u + m * 1024u);
+            nq[m] = ld_lane<uint4>(rq, lane4 * 4u + (uint32_t)min(m, nblk - 1) * 1024u);
     };
     double acc = 0.0, px = 0.0, py = 0.0;
     // the previous row, waiting for its Q block
@@ -854,8 +856,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             L = (uint32_t)readlane((int)nL, 0);
             // stage the Q row (frames 256m + 4l + c at lane l, chunk m)
 #pragma unroll
-            for (int m = 0; m < 8; ++m)
-                if (m < nblk) reinterpret_cast<uint4*>(lds.qrow)[m * kWave + lane] = nq[m];
+            for (int m = 0; m < 8; ++m) reinterpret_cast<uint4*>(lds.qrow)[m * kWave + lane] = nq[m];
             lds_order();
 #pragma unroll
             for (int e = 0; e < 8; ++e) q[e] = lds.qrow[(idx[e >> 1] >> (16 * (e & 1))) & 0xffffu];
@@ -1508,6 +1509,7 @@ __device__ __forceinline__ PrefixRows prefix_rows(const float* D, const float* S
 
 // Sorted-prefix pair kernel (kind 2): block = (4 pairs, one range of frame
 // rows) as ksg_pair_kernel, each wave one pair.
+template <bool Detail>
 __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
 {
     __shared__ __attribute__((aligned(4 * kPBins))) PrefixLds lds[kPairWaves];
@@ -1531,7 +1533,13 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
     const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
     const PrefixRows pr = prefix_rows(args.D, args.SD, args.SM, args.SI, args.P, N, args.a1[p],
                                       args.a2[p]);
-    const double acc = rows_prefix(pr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
+    RowOut o = row_out(args);
+    if (!Detail) {   // per-row count output off: its pointers leave the SGPR file
+        o.rc_nx = nullptr;
+        o.rc_ny = nullptr;
+        o.rc_flag = nullptr;
+    }
+    const double acc = rows_prefix(pr, N, T, i0, i1, 1, p, lds[w], lane, o);
     if (lane == 0) args.partial[p * kRowSplit + part] = acc;
 }
 
@@ -1543,7 +1551,7 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
 // which share their first two atoms, over the same kLongChunk rows, so those
 // atoms' distance rows and prefixes are read from that XCD's L2.  Each
 // wave's sum over its rows is added to fixsum[pair].
-template <int C>
+template <int C, bool Detail>
 __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
 {
     __shared__ __attribute__((aligned(4 * kPBins))) uint32_t lds[kPairWaves][kPBins];
@@ -1566,7 +1574,13 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
     const int i1 = min(N, i0 + kLongChunk);
     const LongRows lr = long_rows(args.D, args.LV, args.LI, args.LS, args.P, N, args.a1[p],
                                   args.a2[p]);
-    const double acc = rows_long<C>(lr, N, T, i0, i1, 1, p, lds[w], lane, row_out(args));
+    RowOut o = row_out(args);
+    if (!Detail) {   // per-row count output off: its pointers leave the SGPR file
+        o.rc_nx = nullptr;
+        o.rc_ny = nullptr;
+        o.rc_flag = nullptr;
+    }
+    const double acc = rows_long<C>(lr, N, T, i0, i1, 1, p, lds[w], lane, o);
     if (lane == 0) atomicAdd(&args.fixsum[p], acc);
 }
 
@@ -2128,12 +2142,19 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4, i
 #define GMX_SINGLE(V) case V: hipLaunchKernelGGL(ksg_pair_kernel<V>, grid, block, 0, s, c); break;
 #define GMX_COOP(V) case V: hipLaunchKernelGGL(ksg_pair_coop_kernel<V>, grid, block, 0, s, c); break;
     if (kind == 3) {
-        if (s4 == 2)
-            hipLaunchKernelGGL(ksg_pair_long_kernel<2>, grid, block, 0, s, c);
+        if (s4 == 2 && c.rc_nx)
+            hipLaunchKernelGGL((ksg_pair_long_kernel<2, true>), grid, block, 0, s, c);
+        else if (s4 == 2)
+            hipLaunchKernelGGL((ksg_pair_long_kernel<2, false>), grid, block, 0, s, c);
+        else if (c.rc_nx)
+            hipLaunchKernelGGL((ksg_pair_long_kernel<4, true>), grid, block, 0, s, c);
         else
-            hipLaunchKernelGGL(ksg_pair_long_kernel<4>, grid, block, 0, s, c);
+            hipLaunchKernelGGL((ksg_pair_long_kernel<4, false>), grid, block, 0, s, c);
     } else if (kind == 2) {
-        hipLaunchKernelGGL(ksg_pair_prefix_kernel, grid, block, 0, s, c);
+        if (c.rc_nx)
+            hipLaunchKernelGGL(ksg_pair_prefix_kernel<true>, grid, block, 0, s, c);
+        else
+            hipLaunchKernelGGL(ksg_pair_prefix_kernel<false>, grid, block, 0, s, c);
     } else if (kind == 0) {
         switch (s4) {
             GMX_SINGLE(1) GMX_SINGLE(2) GMX_SINGLE(3) GMX_SINGLE(4) GMX_SINGLE(5) GMX_SINGLE(6)
