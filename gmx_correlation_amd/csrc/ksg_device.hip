@@ -224,6 +224,28 @@ __device__ __forceinline__ void find_bin(const uint32_t* hist, int lane, int t, 
 {
     static_assert(BPL % 4 == 0, "16-byte reads");
     const uint4* mine = reinterpret_cast<const uint4*>(hist + ((lane + ROT) & (kWave - 1)) * BPL);
+    if constexpr (BPL == 4 && ROT == 0) {
+        // four bins per lane: the owner's bins are in its registers, and the
+        // bin within them follows from three scalar comparisons
+        const uint4 v = mine[0];
+        const int s = (int)((v.x + v.y) + (v.z + v.w));
+        const int incl = wave_incl_scan(s);
+        const int excl = incl - s;
+        const uint64_t bal = __ballot(excl < t && t <= incl);
+        const int owner = bal ? __ffsll((unsigned long long)bal) - 1 : 0;
+        const int r = t - readlane(excl, owner);
+        const int h0 = readlane((int)v.x, owner), h1 = readlane((int)v.y, owner);
+        const int h2 = readlane((int)v.z, owner), h3 = readlane((int)v.w, owner);
+        const int c1 = h0 + h1, c2 = c1 + h2, c3 = c2 + h3;
+        int sub = 0, below = 0, p = h0;
+        if (r > h0 && r <= c1) { sub = 1; below = h0; p = h1; }
+        else if (r > c1 && r <= c2) { sub = 2; below = c1; p = h2; }
+        else if (r > c2 && r <= c3) { sub = 3; below = c2; p = h3; }
+        bin = owner * BPL + sub;
+        rank = r - below;
+        pop = p;
+        return;
+    }
     int s = 0;
     // 16-byte reads, issued together; the rotated order spreads the lanes
     // over the LDS banks
@@ -982,12 +1004,15 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             continue;
         }
         // ---- nP from the prefix in registers (epsP < H); Q block load -------
-        int c = 0;
+        // one compare per value into a lane mask, counted on the scalar unit
+        int c = 0, ce = 0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) c += (v[e] <= ex ? 1 : 0) + (v[e] == ex ? 0x10000 : 0);
-        c = wave_sum(c);
-        pnp = c & 0xffff;
-        ptie = (c >> 16) > 1;
+        for (int e = 0; e < 8; ++e) {
+            c += __popcll(__ballot(v[e] <= ex));
+            ce += __popcll(__ballot(v[e] == ex));
+        }
+        pnp = c;
+        ptie = ce > 1;
         pc1 = __popcll(__ballot(sm <= ey));
         const uint32_t* sq = (sideA ? pr.SDb : pr.SDa) + row * pr.PS + 64 * pc1;
         pblk = ld_lane<uint2>(sq, lane4 * 2u);
@@ -1233,12 +1258,14 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         }
         // ---- nP from the prefix in registers; nQ from Q's prefix store or
         //      a streaming count over Q's row ---------------------------------
-        int c = 0;
+        int c = 0, ce = 0;
 #pragma unroll
-        for (int e = 0; e < 4 * C; ++e) c += (v[e] <= ex ? 1 : 0) + (v[e] == ex ? 0x10000 : 0);
-        c = wave_sum(c);
-        pnp = c & 0xffff;
-        ptie = (c >> 16) > 1;
+        for (int e = 0; e < 4 * C; ++e) {
+            c += __popcll(__ballot(v[e] <= ex));
+            ce += __popcll(__ballot(v[e] == ex));
+        }
+        pnp = c;
+        ptie = ce > 1;
         const uint32_t qtop = (uint32_t)readlane((int)lsq, kLongSum - 1);
         if (ey < qtop) {
             pready = false;
