@@ -198,11 +198,14 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
     long long fix_rows = 0, ties = 0, long_rows = 0, items_total = 0;
     int rc = GMXCORR_OK;
     for (size_t wv = 0; wv < waves.size() && rc == GMXCORR_OK; ++wv) {
-        const std::vector<int>& ts = waves[wv];
+        // trajectories by decreasing frame count: the work order
+        std::vector<int> ts = waves[wv];
+        std::stable_sort(ts.begin(), ts.end(),
+                         [&](int x, int y) { return b->traj[x].N > b->traj[y].N; });
         std::vector<double*> dX(ts.size(), nullptr), dPsi(ts.size(), nullptr);
         std::vector<float*> dD(ts.size(), nullptr);
         gmxcorr::TrajDesc* dDesc = nullptr;
-        int *dItemTraj = nullptr, *dA1 = nullptr, *dA2 = nullptr;
+        int *dItemTraj = nullptr, *dA1 = nullptr, *dA2 = nullptr, *dQuads = nullptr;
         double *dPartial = nullptr, *dFixsum = nullptr, *dMI = nullptr;
         gmxcorr::FixEntry *dFix = nullptr, *dLong = nullptr;
         unsigned* dCnt = nullptr;
@@ -211,6 +214,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             (void)hipStreamSynchronize(s);
             dfree(dDesc); dfree(dItemTraj); dfree(dA1); dfree(dA2); dfree(dPartial);
             dfree(dFixsum); dfree(dMI); dfree(dFix); dfree(dLong); dfree(dCnt); dfree(dQueue);
+            dfree(dQuads);
         };
         auto body = [&]() -> int {
             std::fprintf(stderr, "gmxcorr batch: wave %zu/%zu: building rows of %zu trajectories\n",
@@ -258,11 +262,9 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                 desc[q] = gmxcorr::TrajDesc{dX[q], dD[q], dSD, dSM, dSI, dLV, dLI, dLS, dPsi[q],
                                             h.P, h.N, h.K, h.psiN_phiK, h.kind, h.s4};
             }
-            // items: trajectories by decreasing frame count, pairs a1 > a2 within each
+            // pairs a1 > a2 of each trajectory, in the work order
             std::vector<int> tq(ts.size());
             std::iota(tq.begin(), tq.end(), 0);
-            std::sort(tq.begin(), tq.end(),
-                      [&](int x, int y) { return b->traj[ts[x]].N > b->traj[ts[y]].N; });
             std::vector<int> it, i1, i2;
             std::vector<long long> first(ts.size());
             for (int q : tq) {
@@ -275,8 +277,34 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                         i2.push_back(a2);
                     }
             }
-            const long long nitems = (long long)it.size();
+            const long long nitems = (long long)it.size();   // pairs of the wave
             items_total += nitems;
+            // work items of each trajectory, and the tile tables of the kind-3 ones
+            std::vector<int> hq;
+            std::vector<long long> qoff(ts.size(), -1);
+            for (size_t q = 0; q < ts.size(); ++q) {
+                const HostTraj& h = b->traj[ts[q]];
+                desc[q].pair0 = first[q];
+                desc[q].quads = nullptr;
+                if (h.kind == 3) {
+                    const std::vector<int> t4 = gmxcorr::pair_quads(h.natoms);
+                    qoff[q] = (long long)hq.size();
+                    hq.insert(hq.end(), t4.begin(), t4.end());
+                    desc[q].nquads = (long long)t4.size() / 4;
+                    desc[q].nitems = desc[q].nquads *
+                                     ((h.N + gmxcorr::kLongChunk - 1) / gmxcorr::kLongChunk);
+                } else {
+                    desc[q].nquads = 0;
+                    desc[q].nitems = h.npairs();
+                }
+            }
+            if (!hq.empty()) {
+                HIPCHK(hipMalloc(&dQuads, hq.size() * sizeof(int)));
+                HIPCHK(hipMemcpyAsync(dQuads, hq.data(), hq.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, s));
+                for (size_t q = 0; q < ts.size(); ++q)
+                    if (qoff[q] >= 0) desc[q].quads = dQuads + qoff[q];
+            }
             HIPCHK(hipMalloc(&dDesc, desc.size() * sizeof(gmxcorr::TrajDesc)));
             HIPCHK(hipMalloc(&dItemTraj, nitems * sizeof(int)));
             HIPCHK(hipMalloc(&dA1, nitems * sizeof(int)));
@@ -285,7 +313,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             HIPCHK(hipMalloc(&dFixsum, nitems * sizeof(double)));
             HIPCHK(hipMalloc(&dMI, nitems * sizeof(double)));
             HIPCHK(hipMalloc(&dCnt, 4 * sizeof(unsigned)));
-            HIPCHK(hipMalloc(&dQueue, sizeof(unsigned long long)));
+            HIPCHK(hipMalloc(&dQueue, 8 * sizeof(unsigned long long)));
             // recount queue for 1% of the wave's rows (0.04-0.2% measured),
             // long-row queue for 6% (the tuner keeps the long-row share of a
             // trajectory below 4%); an overflow re-runs the wave
@@ -304,7 +332,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             HIPCHK(hipMemcpyAsync(dA1, i1.data(), nitems * sizeof(int), hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(dA2, i2.data(), nitems * sizeof(int), hipMemcpyHostToDevice, s));
             gmxcorr::KsgBatchArgs ba{dDesc, dItemTraj, dA1, dA2, nitems, dPartial, dFix, dCnt, fixcap,
-                                     dQueue, dLong, longcap};
+                                     dQueue, dLong, longcap, dFixsum, (int)ts.size()};
             gmxcorr::FixupArgs fa{};
             fa.traj = dDesc;
             fa.item_traj = dItemTraj;
@@ -320,7 +348,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                 ba.longq = dLong;
                 ba.longcap = longcap;
                 fa.fixq = dFix;
-                HIPCHK(gmxcorr::launch_ksg_clear(dFixsum, nitems, dCnt, dQueue, s));
+                HIPCHK(gmxcorr::launch_ksg_clear(dFixsum, nitems, dCnt, dQueue, s, dPartial, 8));
                 HIPCHK(hipEventRecord(ev[2], s));
                 HIPCHK(gmxcorr::launch_ksg_batch(ba, nblocks, kinds, s));
                 HIPCHK(hipEventRecord(ev[3], s));

@@ -1716,28 +1716,26 @@ __device__ __forceinline__ double batch_prefix(const TrajDesc& d, int a1, int a2
     return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
 }
 
-// Kind 3 in the batched kernel: wave w runs rows w, w+4, ... in chunks of at
-// most kPrefix rows.
-__device__ __forceinline__ double batch_long(const TrajDesc& d, int a1, int a2, long long item,
-                                             uint32_t* hist, double* wsum, int lane, int w,
-                                             const RowOut& o)
+// Kind 3 in the batched kernel: one work item is a 2x2 atom tile of the
+// trajectory times one chunk of kLongChunk frame rows (chunk-major, as
+// ksg_pair_long_kernel); wave w runs the tile's pair w over the chunk and
+// adds its row sum to fixsum[pair].
+__device__ __forceinline__ void batch_long(const TrajDesc& d, const KsgBatchArgs& b,
+                                           long long local, uint32_t* hist, int lane, int w,
+                                           const RowOut& o)
 {
+    const long long chunk = local / d.nquads, qd = local - chunk * d.nquads;
+    const int p = uniform_copy(d.quads[qd * kPairWaves + w]);
+    if (p < 0) return;   // no block barriers in this path
+    const long long gp = d.pair0 + p;
+    const int a1 = uniform_copy(b.a1[gp]), a2 = uniform_copy(b.a2[gp]);
+    const int i0 = (int)chunk * kLongChunk;
+    const int i1 = min(d.N, i0 + kLongChunk);
     const LongRows lr = long_rows(d.D, d.LV, d.LI, d.LS, d.P, d.N, a1, a2);
-    double acc = 0.0;
-    for (int c0 = 0; c0 < d.N; c0 += kPairWaves * kPrefix) {
-        // opaque per chunk: lane-derived addresses are recomputed, not held
-        // across chunks (115 VGPRs instead of 153)
-        int ln = lane;
-        asm volatile("" : "+v"(ln));
-        acc += d.s4 == 2
-                   ? rows_long<2>(lr, d.N, d.K + 1, c0 + w, min(d.N, c0 + kPairWaves * kPrefix),
-                                  kPairWaves, item, hist, ln, o)
-                   : rows_long<4>(lr, d.N, d.K + 1, c0 + w, min(d.N, c0 + kPairWaves * kPrefix),
-                                  kPairWaves, item, hist, ln, o);
-    }
-    if (lane == 0) wsum[w] = acc;
-    __syncthreads();
-    return ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+    const double acc = d.s4 == 2
+                           ? rows_long<2>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o)
+                           : rows_long<4>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o);
+    if (lane == 0) atomicAdd(&b.fixsum[gp], acc);
 }
 
 template <int S4>
@@ -1796,54 +1794,76 @@ __global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
     o.rc_nx = nullptr;
     o.rc_ny = nullptr;
     o.rc_flag = nullptr;
-    for (;;) {
-        if (threadIdx.x == 0) next = (long long)atomicAdd(b.queue, 1ULL);
-        __syncthreads();
-        const long long item = uniform_copy(next);
-        __syncthreads();
-        if (item >= b.nitems) break;
-        // item, trajectory and atoms are uniform over the block
-        // opaque per item: keeps lane-derived addresses from being hoisted
-        // out of the item loop and held in registers across items
-        int lane = (int)(threadIdx.x & (kWave - 1));
-        asm volatile("" : "+v"(lane));
-        const TrajDesc d = uniform_copy(b.traj[uniform_copy(b.item_traj[item])]);
-        const int ia1 = uniform_copy(b.a1[item]), ia2 = uniform_copy(b.a2[item]);
-        o.psi = d.psi;
-        const float* Da = d.D + (size_t)ia1 * d.N * d.P;
-        const float* Db = d.D + (size_t)ia2 * d.N * d.P;
-        double acc = 0.0;
-        // the branches are uniform over the block (one item per block)
-        if constexpr ((Paths & 4) != 0) {
-            if (d.kind == 2) acc = batch_prefix(d, ia1, ia2, item, sh.pre(w),
-                                                wsum, lane, w, o);
-        }
-        if constexpr ((Paths & 8) != 0) {
-            if (d.kind == 3) acc = batch_long(d, ia1, ia2, item, sh.hist(w), wsum,
-                                              lane, w, o);
-        }
-        if constexpr ((Paths & 1) != 0) {
-            if (d.kind == 0) {
-                switch (d.s4) {
-                    case 2: acc = batch_single<2>(d, Da, Db, item, sh.u, wsum, lane, w, o); break;
-                    case 4: acc = batch_single<4>(d, Da, Db, item, sh.u, wsum, lane, w, o); break;
-                    case 6: acc = batch_single<6>(d, Da, Db, item, sh.u, wsum, lane, w, o); break;
-                    default: acc = batch_single<8>(d, Da, Db, item, sh.u, wsum, lane, w, o); break;
+    // Work queues: XCD x (block id mod 8) takes the x-th eighth of every
+    // trajectory's items, trajectories in order, so the blocks in flight on
+    // one XCD run neighbouring items and share their atoms' rows in its L2;
+    // a block whose queue is empty takes items from the other queues.
+    const int xcd = (int)(blockIdx.x & 7);
+    for (int k = 0; k < 8; ++k) {
+        const int xq = (xcd + k) & 7;
+        for (;;) {
+            if (threadIdx.x == 0) next = (long long)atomicAdd(&b.queue[xq], 1ULL);
+            __syncthreads();
+            long long v = uniform_copy(next);
+            __syncthreads();
+            int tq = -1;
+            long long local = 0;
+            for (int q = 0; q < b.ntraj; ++q) {
+                const long long n = uniform_copy(b.traj[q].nitems);
+                const long long lo = n * xq / 8, hi = n * (xq + 1) / 8;
+                if (v < hi - lo) {
+                    tq = q;
+                    local = lo + v;
+                    break;
+                }
+                v -= hi - lo;
+            }
+            if (tq < 0) break;   // queue xq is empty (uniform over the block)
+            // opaque per item: keeps lane-derived addresses from being hoisted
+            // out of the item loop and held in registers across items
+            int lane = (int)(threadIdx.x & (kWave - 1));
+            asm volatile("" : "+v"(lane));
+            const TrajDesc d = uniform_copy(b.traj[tq]);
+            o.psi = d.psi;
+            // the branches are uniform over the block (one item per block)
+            if constexpr ((Paths & 8) != 0) {
+                if (d.kind == 3) {
+                    batch_long(d, b, local, sh.hist(w), lane, w, o);
+                    __syncthreads();
+                    continue;
                 }
             }
-        }
-        if constexpr ((Paths & 2) != 0) {
-            if (d.kind == 1) {
-                switch (d.s4) {
-                    case 3: acc = batch_coop<3>(d, Da, Db, item, sh.u, lane, w, o); break;
-                    case 4: acc = batch_coop<4>(d, Da, Db, item, sh.u, lane, w, o); break;
-                    case 6: acc = batch_coop<6>(d, Da, Db, item, sh.u, lane, w, o); break;
-                    default: acc = batch_coop<8>(d, Da, Db, item, sh.u, lane, w, o); break;
+            const long long gp = d.pair0 + local;   // one pair per item
+            const int ia1 = uniform_copy(b.a1[gp]), ia2 = uniform_copy(b.a2[gp]);
+            const float* Da = d.D + (size_t)ia1 * d.N * d.P;
+            const float* Db = d.D + (size_t)ia2 * d.N * d.P;
+            double acc = 0.0;
+            if constexpr ((Paths & 4) != 0) {
+                if (d.kind == 2) acc = batch_prefix(d, ia1, ia2, gp, sh.pre(w), wsum, lane, w, o);
+            }
+            if constexpr ((Paths & 1) != 0) {
+                if (d.kind == 0) {
+                    switch (d.s4) {
+                        case 2: acc = batch_single<2>(d, Da, Db, gp, sh.u, wsum, lane, w, o); break;
+                        case 4: acc = batch_single<4>(d, Da, Db, gp, sh.u, wsum, lane, w, o); break;
+                        case 6: acc = batch_single<6>(d, Da, Db, gp, sh.u, wsum, lane, w, o); break;
+                        default: acc = batch_single<8>(d, Da, Db, gp, sh.u, wsum, lane, w, o); break;
+                    }
                 }
             }
+            if constexpr ((Paths & 2) != 0) {
+                if (d.kind == 1) {
+                    switch (d.s4) {
+                        case 3: acc = batch_coop<3>(d, Da, Db, gp, sh.u, lane, w, o); break;
+                        case 4: acc = batch_coop<4>(d, Da, Db, gp, sh.u, lane, w, o); break;
+                        case 6: acc = batch_coop<6>(d, Da, Db, gp, sh.u, lane, w, o); break;
+                        default: acc = batch_coop<8>(d, Da, Db, gp, sh.u, lane, w, o); break;
+                    }
+                }
+            }
+            if (threadIdx.x == 0) b.partial[gp] = acc;
+            __syncthreads();
         }
-        if (threadIdx.x == 0) b.partial[item] = acc;
-        __syncthreads();
     }
 }
 
@@ -2028,12 +2048,13 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned
 }
 
 __global__ void ksg_clear_kernel(double* fixsum, long long npairs, unsigned* counters,
-                                 unsigned long long* queue)
+                                 unsigned long long* queue, double* partial, int nqueue)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < npairs) fixsum[t] = 0.0;
+    if (partial && t < npairs) partial[t] = 0.0;
     if (t < 3) counters[t] = 0u;
-    if (t == 0 && queue) *queue = 0ull;
+    if (queue && t < nqueue) queue[t] = 0ull;
 }
 
 // MI = psi[N] + phi[K] - sum/N as lib/kraskov.cpp:1318.
@@ -2103,11 +2124,12 @@ hipError_t launch_build_prefix(const float* D, int natoms, int N, long long P, f
 }
 
 hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
-                            unsigned long long* queue, hipStream_t s)
+                            unsigned long long* queue, hipStream_t s, double* partial, int nqueue)
 {
-    const long long m = n > 3 ? n : 3;
+    if (nqueue < 0 || nqueue > 256) return hipErrorInvalidValue;
+    const long long m = n > 256 ? n : 256;
     hipLaunchKernelGGL(ksg_clear_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
-                       fixsum, n, counters, queue);
+                       fixsum, n, counters, queue, partial, nqueue);
     return hipGetLastError();
 }
 

@@ -60,6 +60,30 @@ int check_rows(const double* xrows, long long nrows, int N, const char* who)
     return GMXCORR_OK;
 }
 
+// Blocks of four pairs for the all-pairs run (pair index of (a1, a2), a2 <
+// a1, is a1*(a1-1)/2 + a2): 2x2 atom tiles {2t, 2t+1} x {2s, 2s+1}, s < t,
+// ordered by t then s, so two waves share each A row and two each B row;
+// the pairs outside full tiles follow in runs of four that share a1.
+std::vector<int> pair_quads(int n)
+{
+    auto pid = [](int a1, int a2) { return (int)((long long)a1 * (a1 - 1) / 2 + a2); };
+    std::vector<int> q, rest;
+    for (int t = 0; 2 * t + 1 < n; ++t) {
+        for (int s = 0; s < t; ++s) {
+            q.push_back(pid(2 * t, 2 * s));
+            q.push_back(pid(2 * t, 2 * s + 1));
+            q.push_back(pid(2 * t + 1, 2 * s));
+            q.push_back(pid(2 * t + 1, 2 * s + 1));
+        }
+        rest.push_back(pid(2 * t + 1, 2 * t));
+    }
+    if (n % 2)
+        for (int a2 = 0; a2 < n - 1; ++a2) rest.push_back(pid(n - 1, a2));
+    for (size_t r = 0; r < rest.size(); r += 4)
+        for (size_t c = 0; c < 4; ++c) q.push_back(r + c < rest.size() ? rest[r + c] : -1);
+    return q;
+}
+
 }  // namespace gmxcorr
 
 namespace {
@@ -178,29 +202,6 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
-// Blocks of four pairs for the all-pairs run (pair index of (a1, a2), a2 <
-// a1, is a1*(a1-1)/2 + a2): 2x2 atom tiles {2t, 2t+1} x {2s, 2s+1}, s < t,
-// ordered by t then s, so two waves share each A row and two each B row;
-// the pairs outside full tiles follow in runs of four that share a1.
-std::vector<int> pair_quads(int n)
-{
-    auto pid = [](int a1, int a2) { return (int)((long long)a1 * (a1 - 1) / 2 + a2); };
-    std::vector<int> q, rest;
-    for (int t = 0; 2 * t + 1 < n; ++t) {
-        for (int s = 0; s < t; ++s) {
-            q.push_back(pid(2 * t, 2 * s));
-            q.push_back(pid(2 * t, 2 * s + 1));
-            q.push_back(pid(2 * t + 1, 2 * s));
-            q.push_back(pid(2 * t + 1, 2 * s + 1));
-        }
-        rest.push_back(pid(2 * t + 1, 2 * t));
-    }
-    if (n % 2)
-        for (int a2 = 0; a2 < n - 1; ++a2) rest.push_back(pid(n - 1, a2));
-    for (size_t r = 0; r < rest.size(); r += 4)
-        for (size_t c = 0; c < 4; ++c) q.push_back(r + c < rest.size() ? rest[r + c] : -1);
-    return q;
-}
 
 // Pair kernel + FP64 recount + finalise on the plan stream.  If the recount
 // queue overflowed, the run is repeated with a queue large enough.
@@ -455,7 +456,7 @@ int gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* pl)
         HIPCHK(hipMemcpy(pl->dA1, h1.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(pl->dA2, h2.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
         if (!pl->dQuads && npairs < (1LL << 31)) {
-            const std::vector<int> q = pair_quads((int)n);
+            const std::vector<int> q = gmxcorr::pair_quads((int)n);
             HIPCHK(hipMalloc(&pl->dQuads, q.size() * sizeof(int)));
             HIPCHK(hipMemcpy(pl->dQuads, q.data(), q.size() * sizeof(int), hipMemcpyHostToDevice));
             pl->nquads = (long long)q.size() / 4;

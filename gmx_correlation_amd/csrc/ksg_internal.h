@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <vector>
 
 namespace gmxcorr {
 
@@ -74,6 +75,13 @@ struct TrajDesc {
     double psiN_phiK;
     int kind;
     int s4;
+    // batched launch: this trajectory's work items (kind 3: 2x2 atom tiles
+    // times kLongChunk-row chunks, chunk-major; otherwise one per pair), the
+    // global index of its first pair, and its tile table
+    long long nitems;
+    long long pair0;
+    long long nquads;
+    const int* quads;    // [nquads][4] local pair indices, -1 for none (kind 3)
 };
 
 // Arguments of the FP64 recount and finalise kernels.  item_traj == nullptr
@@ -102,9 +110,11 @@ struct KsgBatchArgs {
     FixEntry* fixq;
     unsigned* counters;
     unsigned fixcap;
-    unsigned long long* queue;
+    unsigned long long* queue;   // [8] work counters, one per XCD
     FixEntry* longq;   // rows whose (k+1)-th neighbour lies beyond the prefix
     unsigned longcap;
+    double* fixsum;    // [pairs] kind 3 adds its row sums here
+    int ntraj;         // traj[] in work order (decreasing frame count)
 };
 
 // Largest N handled by the pair kernels (cooperative kernel, 4 x 2048 frames).
@@ -161,8 +171,15 @@ hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, fl
 // LV/LI/LS from the distance rows D (N in (kSortedMaxFrames, kMaxFrames]).
 hipError_t launch_build_prefix(const float* D, int natoms, int N, long long P, float* LV,
                                uint16_t* LI, float* LS, hipStream_t s);
+// 2x2 atom tiles of the all-pairs list of n atoms (pair index of (a1, a2),
+// a2 < a1, is a1*(a1-1)/2 + a2), four pair indices per tile, -1 for none.
+std::vector<int> pair_quads(int n);
+
+// Zeroes fixsum[0..n), partial[0..n) (if given), the counters and nqueue
+// work counters.
 hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
-                            unsigned long long* queue, hipStream_t s);
+                            unsigned long long* queue, hipStream_t s, double* partial = nullptr,
+                            int nqueue = 1);
 // kind 0: one wave per frame row streaming both distance rows, N <= 256*s4
 //         (s4 in 1,2,3,4,5,6,8);
 // kind 1: four waves per row, N <= 1024*s4 (s4 in 2..8);
