@@ -916,15 +916,24 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         int sh = max(0, 24 - (int)__builtin_clz(span1 | 1u));
         if ((span1 >> sh) > (uint32_t)(kPBins - 3)) ++sh;
         const uint32_t Lm = L - min(L, 1u << sh);
-        reinterpret_cast<uint4*>(hist)[lane] = make_uint4(0u, 0u, 0u, 0u);
+        // Bin 0 (keys below B1) and the keys at or above H are counted apart:
+        // most keys fall there, and atomics on one address serialise.  Bin 0
+        // takes the count of the keys below B1; bin 255 a count that reaches
+        // T, so that a target beyond H is reported as bin 255.
+        const uint32_t B1 = Lm + (1u << sh);
+        int cb = 0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) cb += __popcll(__ballot(max(v[e], q[e]) < B1));
+        reinterpret_cast<uint4*>(hist)[lane] =
+            make_uint4(lane == 0 ? (uint32_t)cb : 0u, 0u, 0u, lane == kWave - 1 ? 4096u : 0u);
         lds_order();
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const uint32_t key = max(v[e], q[e]);
-            const uint32_t bin = min(__builtin_elementwise_sub_sat(key, Lm) >> sh,
-                                     (uint32_t)(kPBins - 1));
-            __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            const uint32_t bin = (key - Lm) >> sh;   // in 1 .. 254 for B1 <= key < H
+            if (key >= B1 && key < H)
+                __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
         lds_order();
         int b1, r0, pop;
@@ -945,7 +954,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
                 const bool sure = key < lo;
                 ex = max(ex, sure ? v[e] : 0u);
                 ey = max(ey, sure ? q[e] : 0u);
-                const bool inbin = key - lo < width;
+                const bool inbin = key - lo < width && key < H;   // as counted
                 const uint64_t cm = __ballot(inbin);
                 if (cm) {
                     if (inbin) cand[nstored + mbcnt(cm)] = make_uint2(v[e], q[e]);
@@ -1174,15 +1183,36 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         int sh = max(0, 24 - (int)__builtin_clz(span1 | 1u));
         if ((span1 >> sh) > (uint32_t)(kPBins - 3)) ++sh;
         const uint32_t Lm = L - min(L, 1u << sh);
-        reinterpret_cast<uint4*>(hist)[lane] = make_uint4(0u, 0u, 0u, 0u);
-        lds_order();
+        const uint32_t B1 = Lm + (1u << sh);
+        if constexpr (C == 2) {
+            // as rows_prefix: bin 0 and the keys at or above H counted apart
+            int cb = 0;
 #pragma unroll
-        for (int e = 0; e < 4 * C; ++e) {
-            const uint32_t key = max(v[e], q[e]);
-            const uint32_t bin = min(__builtin_elementwise_sub_sat(key, Lm) >> sh,
-                                     (uint32_t)(kPBins - 1));
-            __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            for (int e = 0; e < 4 * C; ++e) cb += __popcll(__ballot(max(v[e], q[e]) < B1));
+            reinterpret_cast<uint4*>(hist)[lane] =
+                make_uint4(lane == 0 ? (uint32_t)cb : 0u, 0u, 0u, lane == kWave - 1 ? 4096u : 0u);
+            lds_order();
+#pragma unroll
+            for (int e = 0; e < 4 * C; ++e) {
+                const uint32_t key = max(v[e], q[e]);
+                const uint32_t bin = (key - Lm) >> sh;   // in 1 .. 254 for B1 <= key < H
+                if (key >= B1 && key < H)
+                    __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
+        } else {
+            // 1024 frames: few keys reach H, and every key is counted (the
+            // split above measured 28% slower here)
+            reinterpret_cast<uint4*>(hist)[lane] = make_uint4(0u, 0u, 0u, 0u);
+            lds_order();
+#pragma unroll
+            for (int e = 0; e < 4 * C; ++e) {
+                const uint32_t key = max(v[e], q[e]);
+                const uint32_t bin = min(__builtin_elementwise_sub_sat(key, Lm) >> sh,
+                                         (uint32_t)(kPBins - 1));
+                __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
         }
         lds_order();
         int b1, r0, pop;
@@ -1201,7 +1231,8 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
                 const bool sure = key < lo;
                 ex = max(ex, sure ? v[e] : 0u);
                 ey = max(ey, sure ? q[e] : 0u);
-                const bool inbin = key - lo < width;
+                // as counted (keys at or above H are not, with C == 2)
+                const bool inbin = key - lo < width && (C != 2 || key < H);
                 const uint64_t cm = __ballot(inbin);
                 if (cm) {
                     if (inbin) cand[nstored + mbcnt(cm)] = make_uint2(v[e], q[e]);
