@@ -856,7 +856,8 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
     double acc = 0.0, px = 0.0, py = 0.0;
     // the previous row, waiting for its Q block
     long long prow = -1;
-    bool pside = false, ptie = false;
+    bool pside = false;
+    uint32_t ptv = 0u, pexp = 0u;   // value before the previous row's radius, and the radius
     int pnp = 0, pc1 = 0;
     uint32_t pexq = 0u, peps = 0u;
     uint2 pblk = make_uint2(0u, 0u);
@@ -891,6 +892,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             bool tq;
             const int nq_ = sorted_count(pblk, pc1, pexq, lane, tq) - 1;
             const int np = pnp - 1;
+            const bool ptie = pnp >= 2 && (uint32_t)readlane((int)ptv, 0) == pexp;
             const bool flag = tq || ptie;
             const int nx = pside ? np : nq_, ny = pside ? nq_ : np;
             if (!flag && lane == 0) {   // the sum is kept on lane 0
@@ -927,12 +929,14 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         reinterpret_cast<uint4*>(hist)[lane] =
             make_uint4(lane == 0 ? (uint32_t)cb : 0u, 0u, 0u, lane == kWave - 1 ? 4096u : 0u);
         lds_order();
+        // one unsigned compare: keys below B1 wrap above the span
+        const uint32_t span = H > B1 ? H - B1 : 0u;
+        const uint32_t hbin1 = hbase + 4u;   // address of bin 1
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const uint32_t key = max(v[e], q[e]);
-            const uint32_t bin = (key - Lm) >> sh;   // in 1 .. 254 for B1 <= key < H
-            if (key >= B1 && key < H)
-                __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
+            const uint32_t dk = max(v[e], q[e]) - B1;
+            if (dk < span)   // bin (dk >> sh) + 1, in 1 .. 254
+                __hip_atomic_fetch_add((lds_u32*)(uintptr_t)(((dk >> sh) << 2) + hbin1), 1u,
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
         lds_order();
@@ -1014,14 +1018,15 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         }
         // ---- nP from the prefix in registers (epsP < H); Q block load -------
         // one compare per value into a lane mask, counted on the scalar unit
-        int c = 0, ce = 0;
+        int c = 0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            c += __popcll(__ballot(v[e] <= ex));
-            ce += __popcll(__ballot(v[e] == ex));
-        }
+        for (int e = 0; e < 8; ++e) c += __popcll(__ballot(v[e] <= ex));
         pnp = c;
-        ptie = ce > 1;
+        // ex is the sorted prefix's value at position c - 1; a tie at it
+        // shows as an equal value at position c - 2 (compared one row later)
+        ptv = ld_lane<uint32_t>((sideA ? pr.SDa : pr.SDb) + row * pr.PS + 64,
+                                (uint32_t)max(c - 2, 0) * 4u);
+        pexp = ex;
         pc1 = __popcll(__ballot(sm <= ey));
         const uint32_t* sq = (sideA ? pr.SDb : pr.SDa) + row * pr.PS + 64 * pc1;
         pblk = ld_lane<uint2>(sq, lane4 * 2u);
@@ -1120,7 +1125,8 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
     // the previous row: its P count, and either its Q block (pending) or its
     // Q count from the streaming path (ready)
     long long prow = -1;
-    bool pside = false, ptie = false, pready = false, pqtie = false;
+    bool pside = false, pready = false, pqtie = false;
+    uint32_t ptv = 0u, pexp = 0u;   // value before the previous row's radius, and the radius
     int pnp = 0, pc1 = 0, pnq = 0;
     uint32_t pexq = 0u, peps = 0u;
     uint2 pblk = make_uint2(0u, 0u);
@@ -1159,6 +1165,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
             bool tq = pqtie;
             const int nq_ = (pready ? pnq : sorted_count(pblk, pc1, pexq, lane, tq)) - 1;
             const int np = pnp - 1;
+            const bool ptie = pnp >= 2 && (uint32_t)readlane((int)ptv, 0) == pexp;
             const bool flag = tq || ptie;
             const int nx = pside ? np : nq_, ny = pside ? nq_ : np;
             if (!flag && lane == 0) {
@@ -1192,12 +1199,13 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
             reinterpret_cast<uint4*>(hist)[lane] =
                 make_uint4(lane == 0 ? (uint32_t)cb : 0u, 0u, 0u, lane == kWave - 1 ? 4096u : 0u);
             lds_order();
+            const uint32_t span = H > B1 ? H - B1 : 0u;
+            const uint32_t hbin1 = hbase + 4u;   // address of bin 1
 #pragma unroll
             for (int e = 0; e < 4 * C; ++e) {
-                const uint32_t key = max(v[e], q[e]);
-                const uint32_t bin = (key - Lm) >> sh;   // in 1 .. 254 for B1 <= key < H
-                if (key >= B1 && key < H)
-                    __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((bin << 2) | hbase), 1u,
+                const uint32_t dk = max(v[e], q[e]) - B1;
+                if (dk < span)   // bin (dk >> sh) + 1, in 1 .. 254
+                    __hip_atomic_fetch_add((lds_u32*)(uintptr_t)(((dk >> sh) << 2) + hbin1), 1u,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             }
         } else {
@@ -1289,14 +1297,14 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         }
         // ---- nP from the prefix in registers; nQ from Q's prefix store or
         //      a streaming count over Q's row ---------------------------------
-        int c = 0, ce = 0;
+        int c = 0;
 #pragma unroll
-        for (int e = 0; e < 4 * C; ++e) {
-            c += __popcll(__ballot(v[e] <= ex));
-            ce += __popcll(__ballot(v[e] == ex));
-        }
+        for (int e = 0; e < 4 * C; ++e) c += __popcll(__ballot(v[e] <= ex));
         pnp = c;
-        ptie = ce > 1;
+        // a tie at ex shows as an equal value at prefix position c - 2
+        ptv = ld_lane<uint32_t>((sideA ? lr.LVa : lr.LVb) + row * kLongPitch + 64,
+                                (uint32_t)max(c - 2, 0) * 4u);
+        pexp = ex;
         const uint32_t qtop = (uint32_t)readlane((int)lsq, kLongSum - 1);
         if (ey < qtop) {
             pready = false;
