@@ -432,10 +432,14 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
 // ---------------------------------------------------------------------------
 // Prefix stores of the long-trajectory path from the distance rows: one block
 // of 512 threads per row (a, i), thread t holding frames 16t .. 16t+15
-// (P <= 8192).  A 4-pass 8-bit radix select finds the value v* of the 1024th
-// smallest frame; the frames below v*, and the first frames at v* in frame
-// order, are the prefix (the 1024 smallest in (value, frame) order).  They
-// are bitonic-sorted by (value, frame) in LDS, then
+// (P <= 8192).  A radix select over v - lo, lo and hi the row's smallest and
+// largest value, 8 bits per pass from the top set bit of hi - lo, finds the
+// value v* of the 1024th smallest frame.  (Binning the raw value bits would
+// put a row's values, which share a few exponents, into a few bins of the
+// first pass, and their LDS atomics would serialise.)  The frames below v*,
+// and the first frames at v* in frame order, are the prefix (the 1024
+// smallest in (value, frame) order).  They are bitonic-sorted by (value,
+// frame) in LDS, then
 //   LV[a][i][0..64) = 0, LV[a][i][64 + t] = t-th smallest value, t < kLongPrefix
 //   LI[a][i][t]     = its frame
 //   LS[a][i][m]     = LV value 64m + 63
@@ -450,6 +454,7 @@ __global__ __launch_bounds__(512) void ksg_build_prefix_kernel(const float* __re
     __shared__ uint32_t hist[256];
     __shared__ uint32_t wtot[8];
     __shared__ uint32_t sel[3];   // chosen bin, rank left within it, output slot counter
+    __shared__ uint32_t wlo[8], whi[8];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
     const int f0 = 16 * tid;
     // grid-stride over rows: a launch's work-item count must stay below 2^32
@@ -466,16 +471,43 @@ __global__ __launch_bounds__(512) void ksg_build_prefix_kernel(const float* __re
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = 0xffffffffu;   // above every value and the padding
     }
-    // ---- radix select of the value of rank kLongPrefix - 1 (N > kLongPrefix)
-    uint32_t pre = 0u, msk = 0u;
+    // ---- value range of the real frames (padding and frames past P excluded)
+    uint32_t mn = 0xffffffffu, mx = 0u;
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+        if (f0 + e < N) {
+            mn = min(mn, v[e]);
+            mx = max(mx, v[e]);
+        }
+    mn = ~wave_max_u(~mn);
+    mx = wave_max_u(mx);
+    if (lane == 0) {
+        wlo[wv] = mn;
+        whi[wv] = mx;
+    }
+    __syncthreads();
+    uint32_t lo = wlo[0], hi = whi[0];
+#pragma unroll
+    for (int u = 1; u < 8; ++u) {
+        lo = min(lo, wlo[u]);
+        hi = max(hi, whi[u]);
+    }
+    // ---- radix select of rank kLongPrefix - 1 (N > kLongPrefix) over
+    //      v - lo: before the pass with top bit T, pre = (v - lo) >> T of
+    //      the frames still in the running
+    const int nb = hi > lo ? 32 - __builtin_clz(hi - lo) : 0;
+    uint32_t pre = 0u;
     uint32_t rank = kLongPrefix - 1;
-    for (int shift = 24; shift >= 0; shift -= 8) {
+    for (int top = nb; top > 0;) {
+        const int shift = max(top - 8, 0);
         if (tid < 256) hist[tid] = 0u;
         if (tid == 0) sel[2] = 0u;
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < 16; ++e)
-            if ((v[e] & msk) == pre) atomicAdd(&hist[(v[e] >> shift) & 255u], 1u);
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t dv = v[e] - lo;
+            if (v[e] <= hi && (dv >> top) == pre) atomicAdd(&hist[(dv >> shift) & 255u], 1u);
+        }
         __syncthreads();
         if (tid < kWave) {
             const uint4 h = reinterpret_cast<const uint4*>(hist)[lane];
@@ -497,11 +529,12 @@ __global__ __launch_bounds__(512) void ksg_build_prefix_kernel(const float* __re
             }
         }
         __syncthreads();
-        pre |= sel[0] << shift;
-        msk |= 255u << shift;
+        pre = (pre << (top - shift)) | sel[0];
         rank = sel[1];
+        top = shift;
         __syncthreads();   // sel and hist are rewritten by the next pass
     }
+    pre += lo;   // v*
     // pre = v*; the prefix holds the frames below v* and the first rank + 1
     // frames at v* in frame order: their order among ties comes from an
     // exclusive scan of the per-thread tie counts
@@ -531,19 +564,27 @@ __global__ __launch_bounds__(512) void ksg_build_prefix_kernel(const float* __re
         }
     }
     __syncthreads();
-    // ---- bitonic sort of the kLongPrefix selected (value, frame) keys
+    // ---- bitonic sort of the kLongPrefix selected (value, frame) keys,
+    //      thread t exchanging keys lo = 2t - (t & (j - 1)) and lo + j.  For
+    //      j <= 64 both lie among the 128 keys of t's wave, so those passes
+    //      need the wave's own (in-order) LDS accesses, not a block barrier.
+    static_assert(kLongPrefix == 2 * 512, "one exchange per thread of the 512-thread block");
+    int pj = 0;
     for (int k = 2; k <= kLongPrefix; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = tid; t < kLongPrefix / 2; t += blockDim.x) {
-                const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
-                const uint64_t x = sk[lo], y = sk[hi];
-                if ((x > y) == ((lo & k) == 0)) {
-                    sk[lo] = y;
-                    sk[hi] = x;
-                }
+            if (j >= 2 * kWave || pj >= 2 * kWave)
+                __syncthreads();
+            else
+                lds_order();
+            const int lo = 2 * tid - (tid & (j - 1)), hi = lo + j;
+            const uint64_t x = sk[lo], y = sk[hi];
+            if ((x > y) == ((lo & k) == 0)) {
+                sk[lo] = y;
+                sk[hi] = x;
             }
-            __syncthreads();
+            pj = j;
         }
+    __syncthreads();
     uint32_t* lv = reinterpret_cast<uint32_t*>(LV) + row * kLongPitch;
     for (int t = tid; t < kLongPitch; t += blockDim.x)
         lv[t] = t < 64 ? 0u : (uint32_t)(sk[t - 64] >> 32);
