@@ -397,8 +397,20 @@ __global__ __launch_bounds__(1024) void dense_copula_sort_kernel(const double* _
     while (M < N) M <<= 1;
     for (int f = threadIdx.x; f < M; f += blockDim.x) keys[f] = f < N ? order_key(row[f]) : ~0ull;
     __syncthreads();
+    // Exchange t pairs keys lo = 2t - (t & (j - 1)) and lo + j; for j <= 64
+    // both lie among 128 keys owned by t's wave (blockDim a multiple of 64),
+    // so those passes need the wave's own in-order LDS accesses and no
+    // block barrier.
+    int pj = 0;
     for (int k = 2; k <= M; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 128 || pj >= 128) {
+                __syncthreads();
+            } else {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_wave_barrier();
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            }
             for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
                 const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
                 const uint64_t x = keys[lo], y = keys[hi];
@@ -407,8 +419,9 @@ __global__ __launch_bounds__(1024) void dense_copula_sort_kernel(const double* _
                     keys[hi] = x;
                 }
             }
-            __syncthreads();
+            pj = j;
         }
+    __syncthreads();
     for (int f = threadIdx.x; f < N; f += blockDim.x) {
         const uint64_t key = order_key(row[f]);
         int lo = 0, hi = N;

@@ -400,8 +400,20 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
         sv[t] = ((uint64_t)kPadBits << 32) | (uint32_t)t;
     __syncthreads();
     // bitonic sort of sv[0..M), ascending (non-negative float bits, then frame)
+    // Exchange t pairs keys lo = 2t - (t & (j - 1)) and lo + j; for j <= 64
+    // both lie among 128 keys owned by t's wave (blockDim a multiple of 64),
+    // so those passes need the wave's own in-order LDS accesses and no
+    // block barrier.
+    int pj = 0;
     for (int k = 2; k <= M; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 128 || pj >= 128) {
+                __syncthreads();
+            } else {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_wave_barrier();
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            }
             for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
                 const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
                 const uint64_t x = sv[lo], y = sv[hi];
@@ -410,8 +422,9 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
                     sv[hi] = x;
                 }
             }
-            __syncthreads();
+            pj = j;
         }
+    __syncthreads();
     const long long ps = sorted_pitch(N);
     uint32_t* srow = reinterpret_cast<uint32_t*>(SD) + row * ps;
     for (int t = threadIdx.x; t < ps; t += blockDim.x) {
