@@ -873,9 +873,9 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
     for (int u = 0; u < kPrefix / kWave; ++u) {
         const int t = lane + kWave * u;
         if (t < R) {
-            const long long row = i0 + (long long)t * step;
-            const uint32_t sa = pr.SMa[row * kSumPitch + 3];
-            const uint32_t sb = pr.SMb[row * kSumPitch + 3];
+            const int row = i0 + t * step;   // < N: 32-bit offsets below
+            const uint32_t sa = pr.SMa[(uint32_t)row * (uint32_t)kSumPitch + 3];
+            const uint32_t sb = pr.SMb[(uint32_t)row * (uint32_t)kSumPitch + 3];
             sidebits |= (sa >= sb ? 1u : 0u) << u;
         }
     }
@@ -889,18 +889,18 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
     for (int m = 0; m < 8; ++m) nq[m] = nv0;
     uint32_t nsm = 0u, nL = 0u;
     auto issue = [&](int t) {
-        const long long row = i0 + (long long)t * step;
+        const int row = i0 + t * step;   // < N: 32-bit offsets below
         const bool sa = side_of(t);
-        const uint32_t* base = (sa ? pr.SDa : pr.SDb) + row * pr.PS + 64;
+        const uint32_t* base = (sa ? pr.SDa : pr.SDb) + (uint32_t)row * (uint32_t)pr.PS + 64;
         nv0 = ld_lane<uint4>(base, lane4 * 4u);
         nv1 = ld_lane<uint4>(base, lane4 * 4u + 1024u);
-        const uint16_t* ib = (sa ? pr.SIa : pr.SIb) + row * kPrefix;
+        const uint16_t* ib = (sa ? pr.SIa : pr.SIb) + (uint32_t)row * (uint32_t)kPrefix;
         const uint2 i01 = ld_lane<uint2>(ib, lane4 * 2u);
         const uint2 i23 = ld_lane<uint2>(ib, lane4 * 2u + 512u);
         ni = make_uint4(i01.x, i01.y, i23.x, i23.y);
         if (lane == 0) nL = ld_lane<uint32_t>(base, (uint32_t)(T - 1) * 4u);
-        nsm = ld_lane<uint32_t>((sa ? pr.SMb : pr.SMa) + row * kSumPitch, lane4);
-        const uint32_t* rq = (sa ? pr.Db : pr.Da) + row * pr.P;
+        nsm = ld_lane<uint32_t>((sa ? pr.SMb : pr.SMa) + (uint32_t)row * (uint32_t)kSumPitch, lane4);
+        const uint32_t* rq = (sa ? pr.Db : pr.Da) + (uint32_t)row * (uint32_t)pr.P;
         // all 8 chunks: past the row's last chunk the last one again (a
         // row of P < 2048 floats), staged to LDS slots no gather reads
 #pragma unroll
@@ -909,7 +909,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
     };
     double acc = 0.0, px = 0.0, py = 0.0;
     // the previous row, waiting for its Q block
-    long long prow = -1;
+    int prow = -1;
     bool pside = false;
     uint32_t ptv = 0u, pexp = 0u;   // value before the previous row's radius, and the radius
     int pnp = 0, pc1 = 0;
@@ -922,9 +922,9 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         uint32_t v[8], q[8];
         uint32_t sm = 0u, L = 0u;
         bool sideA = false;
-        long long row = 0;
+        int row = 0;
         if (have) {
-            row = i0 + (long long)t * step;
+            row = i0 + t * step;
             sideA = side_of(t);
             v[0] = nv0.x; v[1] = nv0.y; v[2] = nv0.z; v[3] = nv0.w;
             v[4] = nv1.x; v[5] = nv1.y; v[6] = nv1.z; v[7] = nv1.w;
@@ -1078,11 +1078,11 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         pnp = c;
         // ex is the sorted prefix's value at position c - 1; a tie at it
         // shows as an equal value at position c - 2 (compared one row later)
-        ptv = ld_lane<uint32_t>((sideA ? pr.SDa : pr.SDb) + row * pr.PS + 64,
+        ptv = ld_lane<uint32_t>((sideA ? pr.SDa : pr.SDb) + (uint32_t)row * (uint32_t)pr.PS + 64,
                                 (uint32_t)max(c - 2, 0) * 4u);
         pexp = ex;
         pc1 = __popcll(__ballot(sm <= ey));
-        const uint32_t* sq = (sideA ? pr.SDb : pr.SDa) + row * pr.PS + 64 * pc1;
+        const uint32_t* sq = (sideA ? pr.SDb : pr.SDa) + (uint32_t)row * (uint32_t)pr.PS + 64 * pc1;
         pblk = ld_lane<uint2>(sq, lane4 * 2u);
         prow = row;
         pside = sideA;
@@ -1144,9 +1144,9 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
     for (int u = 0; u < kPrefix / kWave; ++u) {
         const int t = lane + kWave * u;
         if (t < R) {
-            const long long row = i0 + (long long)t * step;
-            const uint32_t sa = lr.LVa[row * kLongPitch + 64 + 255];
-            const uint32_t sb = lr.LVb[row * kLongPitch + 64 + 255];
+            const int row = i0 + t * step;   // < N: 32-bit offsets below
+            const uint32_t sa = lr.LVa[(uint32_t)row * (uint32_t)kLongPitch + 64 + 255];
+            const uint32_t sb = lr.LVb[(uint32_t)row * (uint32_t)kLongPitch + 64 + 255];
             sidebits |= (sa >= sb ? 1u : 0u) << u;
         }
     }
@@ -1163,22 +1163,24 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
     }
     uint32_t nL = 0u, nls = 0xffffffffu;
     auto issue = [&](int t) {
-        const long long row = i0 + (long long)t * step;
+        const int row = i0 + t * step;   // < N: 32-bit offsets below
         const bool sa = side_of(t);
-        const uint32_t* base = (sa ? lr.LVa : lr.LVb) + row * kLongPitch + 64;
-        const uint16_t* ib = (sa ? lr.LIa : lr.LIb) + row * kLongPrefix;
+        const uint32_t* base = (sa ? lr.LVa : lr.LVb) + (uint32_t)row * (uint32_t)kLongPitch + 64;
+        const uint16_t* ib = (sa ? lr.LIa : lr.LIb) + (uint32_t)row * (uint32_t)kLongPrefix;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             nv[c] = ld_lane<uint4>(base, lane4 * 4u + c * 1024u);
             ni[c] = ld_lane<uint2>(ib, lane4 * 2u + c * 512u);
         }
         if (lane == 0) nL = ld_lane<uint32_t>(base, (uint32_t)(T - 1) * 4u);
-        if (lane < kLongSum) nls = ld_lane<uint32_t>((sa ? lr.LSb : lr.LSa) + row * kLongSum, lane4);
+        if (lane < kLongSum)
+            nls = ld_lane<uint32_t>((sa ? lr.LSb : lr.LSa) + (uint32_t)row * (uint32_t)kLongSum,
+                                    lane4);
     };
     double acc = 0.0, px = 0.0, py = 0.0;
     // the previous row: its P count, and either its Q block (pending) or its
     // Q count from the streaming path (ready)
-    long long prow = -1;
+    int prow = -1;
     bool pside = false, pready = false, pqtie = false;
     uint32_t ptv = 0u, pexp = 0u;   // value before the previous row's radius, and the radius
     int pnp = 0, pc1 = 0, pnq = 0;
@@ -1191,10 +1193,10 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         uint32_t v[4 * C], q[4 * C];
         uint32_t lsq = 0xffffffffu, L = 0u;
         bool sideA = false;
-        long long row = 0;
+        int row = 0;
         const uint32_t* rq = nullptr;
         if (have) {
-            row = i0 + (long long)t * step;
+            row = i0 + t * step;
             sideA = side_of(t);
             uint32_t idx[2 * C];
 #pragma unroll
@@ -1206,7 +1208,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
             }
             lsq = nls;
             L = (uint32_t)readlane((int)nL, 0);
-            rq = (sideA ? lr.Db : lr.Da) + row * lr.P;
+            rq = (sideA ? lr.Db : lr.Da) + (uint32_t)row * (uint32_t)lr.P;
 #pragma unroll
             for (int e = 0; e < 4 * C; ++e)
                 q[e] = ld_lane<uint32_t>(rq, ((idx[e >> 1] >> (16 * (e & 1))) & 0xffffu) * 4u);
@@ -1356,7 +1358,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         for (int e = 0; e < 4 * C; ++e) c += __popcll(__ballot(v[e] <= ex));
         pnp = c;
         // a tie at ex shows as an equal value at prefix position c - 2
-        ptv = ld_lane<uint32_t>((sideA ? lr.LVa : lr.LVb) + row * kLongPitch + 64,
+        ptv = ld_lane<uint32_t>((sideA ? lr.LVa : lr.LVb) + (uint32_t)row * (uint32_t)kLongPitch + 64,
                                 (uint32_t)max(c - 2, 0) * 4u);
         pexp = ex;
         const uint32_t qtop = (uint32_t)readlane((int)lsq, kLongSum - 1);
@@ -1364,7 +1366,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
             pready = false;
             pqtie = false;
             pc1 = __popcll(__ballot(lane < kLongSum && lsq <= ey));
-            const uint32_t* sq = (sideA ? lr.LVb : lr.LVa) + row * kLongPitch + 64 * pc1;
+            const uint32_t* sq = (sideA ? lr.LVb : lr.LVa) + (uint32_t)row * (uint32_t)kLongPitch + 64 * pc1;
             pblk = ld_lane<uint2>(sq, lane4 * 2u);
         } else {
             // frames [N, P) hold padding above every distance
