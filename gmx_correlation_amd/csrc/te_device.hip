@@ -12,9 +12,15 @@
 // scalar loads.  Distances are FP64, computed exactly as the reference does.
 //   pass 1  the k smallest FP32 keys fl32(dist) in a per-lane max-heap in LDS
 //           (rounding to FP32 is monotone, so the heap root is fl32(eps));
-//   pass 2  eps in FP64: the r-th smallest FP64 distance among the frames
-//           whose key equals the root, r = k - #{key < root};
-//   pass 3  the three strict counts in FP64.
+//   pass 2  eps and the counts together: a frame whose joint key is below
+//           the root lies inside eps with all its subspace distances, and a
+//           subspace distance whose FP32 key is below (above) the root is
+//           below (above) eps.  When exactly one frame has the root key it
+//           is the k-th (eps = its FP64 distance), and when no subspace key
+//           of another frame equals the root, the counts are exact;
+//   else    (FP32 ties at the root, rare) eps as the r-th smallest FP64
+//           distance among the frames whose key equals the root, r = k -
+//           #{key < root}, then the three strict counts in FP64.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -93,64 +99,84 @@ __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
     }
     const float root = heap[0][lane];
 
-    // ---- pass 2: eps in FP64 ----------------------------------------
-    int clt = 0, gcnt = 0;
+    // ---- pass 2: eps and the strict counts from FP32 keys ----------------
+    int clt = 0, gcnt = 0, amb = 0;
+    int nxfxp = 0, nypxp = 0, nxp = 0;
     double gmin = INFINITY, gmax = -INFINITY;
+    double g1 = 0.0, g2 = 0.0, g3 = 0.0;   // subspace distances of a root-key frame
     for (int j = 0; j < Nv; ++j) {
         double c[9];
         load_frame(X, Y, j, lag, c);
         double dxf, dxp, dyp;
         dist9(c, dxf, dxp, dyp);
+        if (j == q) continue;
         const double dd = fmax(fmax(dxf, dxp), dyp);
         const float key = (float)dd;
-        if (j == q) continue;
+        const double s1 = fmax(dxf, dxp), s2 = fmax(dyp, dxp), s3 = dxp;
         clt += key < root;
         if (key == root) {
             ++gcnt;
             gmin = fmin(gmin, dd);
             gmax = fmax(gmax, dd);
+            g1 = s1;
+            g2 = s2;
+            g3 = s3;
+        } else {
+            const float k1 = (float)s1, k2 = (float)s2, k3 = (float)s3;
+            nxfxp += k1 < root;
+            nypxp += k2 < root;
+            nxp += k3 < root;
+            amb += (k1 == root) + (k2 == root) + (k3 == root);
         }
     }
-    const int r = k - clt;   // 1 <= r <= gcnt
-    double eps = gmin;
-    if (gcnt > 1 && r > 1 && !(r == gcnt)) {
-        // walk the group's distinct FP64 values upwards (rare)
-        double prev = -1.0;
-        int cum = 0;
-        for (;;) {
-            double vmin = INFINITY;
-            int m = 0;
-            for (int j = 0; j < Nv; ++j) {
-                if (j == q) continue;
-                double c[9];
-                load_frame(X, Y, j, lag, c);
-                double dxf, dxp, dyp;
-                dist9(c, dxf, dxp, dyp);
-                const double dd = fmax(fmax(dxf, dxp), dyp);
-                if ((float)dd == root && dd > prev) {
-                    if (dd < vmin) { vmin = dd; m = 1; }
-                    else if (dd == vmin) ++m;
+    if (gcnt == 1 && amb == 0) {
+        // the one root-key frame is the k-th: eps = its distance, and its
+        // subspace distances (<= eps) count when below it
+        const double eps = gmin;
+        nxfxp += g1 < eps;
+        nypxp += g2 < eps;
+        nxp += g3 < eps;
+    } else {
+        // FP32 ties at the root: eps in FP64, then the counts in FP64
+        const int r = k - clt;   // 1 <= r <= gcnt
+        double eps = gmin;
+        if (gcnt > 1 && r > 1 && !(r == gcnt)) {
+            // walk the group's distinct FP64 values upwards
+            double prev = -1.0;
+            int cum = 0;
+            for (;;) {
+                double vmin = INFINITY;
+                int m = 0;
+                for (int j = 0; j < Nv; ++j) {
+                    if (j == q) continue;
+                    double c[9];
+                    load_frame(X, Y, j, lag, c);
+                    double dxf, dxp, dyp;
+                    dist9(c, dxf, dxp, dyp);
+                    const double dd = fmax(fmax(dxf, dxp), dyp);
+                    if ((float)dd == root && dd > prev) {
+                        if (dd < vmin) { vmin = dd; m = 1; }
+                        else if (dd == vmin) ++m;
+                    }
                 }
+                if (m == 0 || cum + m >= r) { eps = vmin; break; }
+                cum += m;
+                prev = vmin;
             }
-            if (m == 0 || cum + m >= r) { eps = vmin; break; }
-            cum += m;
-            prev = vmin;
+        } else if (r == gcnt) {
+            eps = gmax;
         }
-    } else if (r == gcnt) {
-        eps = gmax;
-    }
-
-    // ---- pass 3: strict counts ------------------------------------------
-    int nxfxp = 0, nypxp = 0, nxp = 0;
-    for (int j = 0; j < Nv; ++j) {
-        double c[9];
-        load_frame(X, Y, j, lag, c);
-        double dxf, dxp, dyp;
-        dist9(c, dxf, dxp, dyp);
-        if (j == q) continue;
-        nxfxp += fmax(dxf, dxp) < eps;
-        nypxp += fmax(dyp, dxp) < eps;
-        nxp += dxp < eps;
+        nxfxp = nypxp = nxp = 0;
+        for (int j = 0; j < Nv; ++j) {
+            double c[9];
+            load_frame(X, Y, j, lag, c);
+            double dxf, dxp, dyp;
+            dist9(c, dxf, dxp, dyp);
+            if (j == q) continue;
+            nxfxp += fmax(dxf, dxp) < eps;
+            nypxp += fmax(dyp, dxp) < eps;
+            nxp += dxp < eps;
+        }
     }
     double te = 0.0;
     if (active) {
