@@ -10,8 +10,10 @@
 // One lane per query frame, 64 queries per wave, one wave per block.  The
 // frame loop is uniform across the wave, so frame j's nine coordinates are
 // scalar loads.  Distances are FP64, computed exactly as the reference does.
-//   pass 1  the k smallest FP32 keys fl32(dist) in a per-lane max-heap in LDS
-//           (rounding to FP32 is monotone, so the heap root is fl32(eps));
+//   pass 1  the k smallest FP32 keys fl32(dist) in a per-lane 8-ary max-heap
+//           in LDS (rounding to FP32 is monotone, so the heap root is
+//           fl32(eps)); 8 children per node keep a sift-down at 3 levels for
+//           k = 100, each level's 8 reads independent;
 //   pass 2  eps and the counts together: a frame whose joint key is below
 //           the root lies inside eps with all its subspace distances, and a
 //           subspace distance whose FP32 key is below (above) the root is
@@ -52,6 +54,7 @@ __device__ __forceinline__ void load_frame(const double* __restrict__ X,
 template <int KMAX>
 __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
 {
+    constexpr int kD = 8;   // heap arity
     __shared__ float heap[KMAX][64];
     const int lane = threadIdx.x;
     const int nqb = args.nqblocks;
@@ -72,41 +75,60 @@ __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
 
     // ---- pass 1: k smallest keys (max-heap, root = largest kept) -------
     for (int t = 0; t < k; ++t) heap[t][lane] = INFINITY;
+    // the root lives in a register: a frame's test needs no LDS round trip,
+    // only an insertion reads the new root back
+    float top = INFINITY;
+    // frame j + 1's coordinates are loaded while frame j is processed
+    double cn[9];
+    load_frame(X, Y, 0, lag, cn);
     for (int j = 0; j < Nv; ++j) {
         double c[9];
-        load_frame(X, Y, j, lag, c);
+#pragma unroll
+        for (int d = 0; d < 9; ++d) c[d] = cn[d];
+        load_frame(X, Y, min(j + 1, Nv - 1), lag, cn);
         double dxf, dxp, dyp;
         dist9(c, dxf, dxp, dyp);
         const float key = (float)fmax(fmax(dxf, dxp), dyp);
-        if (active && j != q && key < heap[0][lane]) {
+        if (active && j != q && key < top) {
             // replace the root and sift down
             int pos = 0;
             for (;;) {
-                const int l = 2 * pos + 1, r = l + 1;
-                if (l >= k) break;
-                int big = l;
-                float vb = heap[l][lane];
-                if (r < k) {
-                    const float vr = heap[r][lane];
-                    if (vr > vb) { big = r; vb = vr; }
-                }
+                const int c0 = kD * pos + 1;
+                if (c0 >= k) break;
+                // largest child among c0 .. min(c0 + kD, k) - 1
+                float ch[kD];
+#pragma unroll
+                for (int c = 0; c < kD; ++c)   // slots past k are read (clamped) and ignored
+                    ch[c] = heap[min(c0 + c, KMAX - 1)][lane];
+                int big = c0;
+                float vb = ch[0];
+#pragma unroll
+                for (int c = 1; c < kD; ++c)
+                    if (c0 + c < k && ch[c] > vb) {
+                        vb = ch[c];
+                        big = c0 + c;
+                    }
                 if (vb <= key) break;
                 heap[pos][lane] = vb;
                 pos = big;
             }
             heap[pos][lane] = key;
+            top = heap[0][lane];
         }
     }
-    const float root = heap[0][lane];
+    const float root = top;
 
     // ---- pass 2: eps and the strict counts from FP32 keys ----------------
     int clt = 0, gcnt = 0, amb = 0;
     int nxfxp = 0, nypxp = 0, nxp = 0;
     double gmin = INFINITY, gmax = -INFINITY;
     double g1 = 0.0, g2 = 0.0, g3 = 0.0;   // subspace distances of a root-key frame
+    load_frame(X, Y, 0, lag, cn);
     for (int j = 0; j < Nv; ++j) {
         double c[9];
-        load_frame(X, Y, j, lag, c);
+#pragma unroll
+        for (int d = 0; d < 9; ++d) c[d] = cn[d];
+        load_frame(X, Y, min(j + 1, Nv - 1), lag, cn);
         double dxf, dxp, dyp;
         dist9(c, dxf, dxp, dyp);
         if (j == q) continue;
