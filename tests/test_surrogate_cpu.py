@@ -26,3 +26,14 @@ def test_oracle_loss_closed_form():
     X = np.zeros((3, 16))
     want = 0.5 * (-1.0 + np.mean((y - 0.3) ** 2) * np.exp(1.0))
     assert abs(M.loss(p, X, y) - want) < 1e-15
+
+
+def test_all_pairs_order_check():
+    """The hybrid hands a fallback of every pair, in run_all's order, to
+    run_all; any other list goes through run_pairs."""
+    from gmx_correlation_amd.surrogate import _all_pairs_in_order
+    a1, a2 = np.tril_indices(6, -1)
+    assert _all_pairs_in_order(a1.astype(np.int32), a2.astype(np.int32), 6)
+    assert not _all_pairs_in_order(a1[::-1], a2[::-1], 6)
+    assert not _all_pairs_in_order(a1[:-1], a2[:-1], 6)
+    assert not _all_pairs_in_order(a2, a1, 6)
