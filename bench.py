@@ -349,8 +349,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU pair sharding is a design in DESIGN.md, not a measured path.",
-              file=sys.stderr)
+              "sharding the pairs over GPUs (atom tiles, then an all-gather of the MI values) "
+              "is not built or measured.", file=sys.stderr)
         return 2
     if args.workload == "dense":
         return bench_dense(args)
