@@ -75,9 +75,51 @@ __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
 
     // ---- pass 1: k smallest keys (max-heap, root = largest kept) -------
     for (int t = 0; t < k; ++t) heap[t][lane] = INFINITY;
-    // the root lives in a register: a frame's test needs no LDS round trip,
-    // only an insertion reads the new root back
+    // the root lives in a register: a frame's test needs no LDS round trip
     float top = INFINITY;
+    // key below the root: replace the root and sift down, return the new root
+    auto insert = [&](float key) -> float {
+        int pos = 0;
+        for (;;) {
+            const int c0 = kD * pos + 1;
+            if (c0 >= k) break;
+            // largest child among c0 .. min(c0 + kD, k) - 1
+            float ch[kD];
+#pragma unroll
+            for (int c = 0; c < kD; ++c)   // slots past k are read (clamped) and ignored
+                ch[c] = heap[min(c0 + c, KMAX - 1)][lane];
+            int big = c0;
+            float vb = ch[0];
+#pragma unroll
+            for (int c = 1; c < kD; ++c)
+                if (c0 + c < k && ch[c] > vb) {
+                    vb = ch[c];
+                    big = c0 + c;
+                }
+            if (vb <= key) break;
+            heap[pos][lane] = vb;
+            pos = big;
+        }
+        heap[pos][lane] = key;
+        return heap[0][lane];
+    };
+    // Keys below the root wait in kW registers per lane; when some lane
+    // holds kW, every lane inserts its waiting keys together (each checked
+    // against the root again), so the wave runs the sift with most lanes
+    // active instead of for one lane at a time.
+    constexpr int kW = 8;
+    float wk[kW];
+#pragma unroll
+    for (int u = 0; u < kW; ++u) wk[u] = INFINITY;
+    int nw = 0;
+    auto flush = [&]() {
+#pragma unroll
+        for (int u = 0; u < kW; ++u) {
+            if (wk[u] < top) top = insert(wk[u]);
+            wk[u] = INFINITY;
+        }
+        nw = 0;
+    };
     // frame j + 1's coordinates are loaded while frame j is processed
     double cn[9];
     load_frame(X, Y, 0, lag, cn);
@@ -90,32 +132,13 @@ __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
         dist9(c, dxf, dxp, dyp);
         const float key = (float)fmax(fmax(dxf, dxp), dyp);
         if (active && j != q && key < top) {
-            // replace the root and sift down
-            int pos = 0;
-            for (;;) {
-                const int c0 = kD * pos + 1;
-                if (c0 >= k) break;
-                // largest child among c0 .. min(c0 + kD, k) - 1
-                float ch[kD];
 #pragma unroll
-                for (int c = 0; c < kD; ++c)   // slots past k are read (clamped) and ignored
-                    ch[c] = heap[min(c0 + c, KMAX - 1)][lane];
-                int big = c0;
-                float vb = ch[0];
-#pragma unroll
-                for (int c = 1; c < kD; ++c)
-                    if (c0 + c < k && ch[c] > vb) {
-                        vb = ch[c];
-                        big = c0 + c;
-                    }
-                if (vb <= key) break;
-                heap[pos][lane] = vb;
-                pos = big;
-            }
-            heap[pos][lane] = key;
-            top = heap[0][lane];
+            for (int u = 0; u < kW; ++u) wk[u] = nw == u ? key : wk[u];
+            ++nw;
         }
+        if (__ballot(nw == kW)) flush();   // uniform over the wave
     }
+    flush();
     const float root = top;
 
     // ---- pass 2: eps and the strict counts from FP32 keys ----------------
