@@ -337,3 +337,32 @@ def test_long_trajectories_with_ties(gpu, oracle_lib, s4):
         assert np.array_equal(d["nx"][p], ref["nx"]), p
         assert np.array_equal(d["ny"][p], ref["ny"]), p
         assert abs(mi[p] - v) <= 1e-12
+
+
+@pytest.mark.parametrize("natoms,N,k", [
+    (5, 2048, 255),   # sorted-prefix path at its largest N and k + 1 = 256
+    (5, 2049, 100),   # prefix-store path at its smallest N
+    (5, 8192, 5),     # the largest N
+    (7, 300, 10),     # rows shorter than the 512-frame prefix; odd atom count
+    (3, 513, 256),    # k + 1 = 257: beyond the prefix paths, streaming kernel
+    (2, 102, 100),    # N = k + 2, a single pair
+])
+def test_boundary_shapes_all_pairs(gpu, oracle_lib, natoms, N, k):
+    """All pairs through the tiled launch (2x2 atom tiles plus the ragged
+    remainder) at the edges of each path's range; every MI equals the
+    oracle's grid restatement to 1e-12 nats, and the first pair's per-frame
+    counts equal the FP64 brute-force recount."""
+    c = synth.generate(synth.TrajSpec(natoms=natoms, nframes=N, nmodes=3, seed=N + k))
+    rows, psi = ksg.kraskov_prepare(ksg.center(c))
+    a1, a2 = np.tril_indices(natoms, -1)
+    order = np.lexsort((a2, a1))   # the reference's order: a1 = 1.., a2 < a1
+    a1, a2 = a1[order], a2[order]
+    with ksg.KsgPlan(natoms, N, k) as plan:
+        plan.load(rows, psi)
+        mi = plan.run_all()
+        _, d = plan.run_pairs(a1[:1], a2[:1], detail=True)
+    ref = oracle_lib.ksg_pairs(rows, psi, k, a1, a2)
+    assert np.max(np.abs(mi - ref)) <= 1e-12
+    _, rd = oracle_lib.ksg_pair(rows, psi, int(a1[0]), int(a2[0]), k, method="brute", detail=True)
+    assert np.array_equal(d["nx"][0], rd["nx"])
+    assert np.array_equal(d["ny"][0], rd["ny"])
