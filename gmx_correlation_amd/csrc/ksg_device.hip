@@ -231,7 +231,9 @@ __device__ __forceinline__ void find_bin(const uint32_t* hist, int lane, int t, 
         const int s = (int)((v.x + v.y) + (v.z + v.w));
         const int incl = wave_incl_scan(s);
         const int excl = incl - s;
-        const uint64_t bal = __ballot(excl < t && t <= incl);
+        // excl < t <= incl as one unsigned compare (a compound predicate
+        // costs the ballot two more VALU instructions)
+        const uint64_t bal = __ballot((uint32_t)(t - 1 - excl) < (uint32_t)s);
         const int owner = bal ? __ffsll((unsigned long long)bal) - 1 : 0;
         const int r = t - readlane(excl, owner);
         const int h0 = readlane((int)v.x, owner), h1 = readlane((int)v.y, owner);
@@ -256,14 +258,15 @@ __device__ __forceinline__ void find_bin(const uint32_t* hist, int lane, int t, 
     }
     const int incl = wave_incl_scan(s);
     const int excl = incl - s;
-    const uint64_t bal = __ballot(excl < t && t <= incl);
+    const uint64_t bal = __ballot((uint32_t)(t - 1 - excl) < (uint32_t)s);
     const int owner = bal ? __ffsll((unsigned long long)bal) - 1 : 0;
     const int base = readlane(excl, owner);
     const uint32_t* own = hist + ((owner + ROT) & (kWave - 1)) * BPL;
     const int v = lane < BPL ? (int)own[lane] : 0;
     const int vi = wave_incl_scan(v);
     const int r = t - base;
-    const uint64_t b2 = __ballot(lane < BPL && vi - v < r && r <= vi);
+    // vi - v < r <= vi; lanes >= BPL hold v = 0 and never pass
+    const uint64_t b2 = __ballot((uint32_t)(r - 1 - (vi - v)) < (uint32_t)v);
     const int sub = b2 ? __ffsll((unsigned long long)b2) - 1 : 0;
     bin = owner * BPL + sub;
     rank = r - readlane(vi - v, sub);
@@ -637,8 +640,9 @@ struct SortedRows {
 // 0..31: block c1-1, lanes 32..63: block c1, two values per lane).
 __device__ __forceinline__ int sorted_count(uint2 v, int c1, uint32_t r, int lane, bool& tie)
 {
-    const bool hi = lane >= 32;
-    const int c2 = __popcll(__ballot(hi && v.x <= r)) + __popcll(__ballot(hi && v.y <= r));
+    // lanes 32..63 only, masked on the scalar unit
+    const uint64_t hi = 0xffffffff00000000ull;
+    const int c2 = __popcll(__ballot(v.x <= r) & hi) + __popcll(__ballot(v.y <= r) & hi);
     const int u = 64 * c1 + c2;   // #{values <= r}, self included
     const int q = c2 + 62;        // position of the value before the last <= r
     const uint32_t prev = (uint32_t)readlane((int)((q & 1) ? v.y : v.x), q >> 1);
@@ -765,7 +769,7 @@ __device__ __forceinline__ double rows_single(const float* Da, const float* Db, 
                 const uint32_t sel =
                     (uint32_t)readlane((int)low, __ffsll((unsigned long long)live) - 1);
                 eps_key = lo + sel;
-                const bool in = act && low <= sel;
+                const bool in = low <= sel;   // inactive lanes: low = 0xffffffff
                 queued = __popcll(__ballot(in)) != r0;
                 ex = wave_max_u(max(ex, in ? ca : 0u));
                 ey = wave_max_u(max(ey, in ? cb : 0u));
@@ -1004,6 +1008,9 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             //      frames appended, compacted, to LDS
             const uint32_t lo = Lm + ((uint32_t)b1 << sh);
             const uint32_t width = 1u << sh;
+            // the bin's keys as counted, [lo, min(lo + width, H)): one
+            // unsigned compare (keys below lo wrap)
+            const uint32_t lim = min(width, H > lo ? H - lo : 0u);
             uint2* cand = reinterpret_cast<uint2*>(hist);
             int nstored = 0;
 #pragma unroll
@@ -1012,7 +1019,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
                 const bool sure = key < lo;
                 ex = max(ex, sure ? v[e] : 0u);
                 ey = max(ey, sure ? q[e] : 0u);
-                const bool inbin = key - lo < width && key < H;   // as counted
+                const bool inbin = key - lo < lim;
                 const uint64_t cm = __ballot(inbin);
                 if (cm) {
                     if (inbin) cand[nstored + mbcnt(cm)] = make_uint2(v[e], q[e]);
@@ -1046,7 +1053,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             }
             const uint32_t sel = (uint32_t)readlane((int)low, __ffsll((unsigned long long)live) - 1);
             eps_key = lo + sel;
-            const bool in = act && low <= sel;
+            const bool in = low <= sel;   // inactive lanes: low = 0xffffffff
             queued = __popcll(__ballot(in)) != r0 || eps_key >= H;
             ex = wave_max_u(max(ex, in ? ca : 0u));
             ey = wave_max_u(max(ey, in ? cb : 0u));
@@ -1287,6 +1294,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         if (!queued) {
             const uint32_t lo = Lm + ((uint32_t)b1 << sh);
             const uint32_t width = 1u << sh;
+            const uint32_t lim = C == 2 ? min(width, H > lo ? H - lo : 0u) : width;
             uint2* cand = reinterpret_cast<uint2*>(hist);
             int nstored = 0;
 #pragma unroll
@@ -1296,7 +1304,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
                 ex = max(ex, sure ? v[e] : 0u);
                 ey = max(ey, sure ? q[e] : 0u);
                 // as counted (keys at or above H are not, with C == 2)
-                const bool inbin = key - lo < width && (C != 2 || key < H);
+                const bool inbin = key - lo < lim;
                 const uint64_t cm = __ballot(inbin);
                 if (cm) {
                     if (inbin) cand[nstored + mbcnt(cm)] = make_uint2(v[e], q[e]);
@@ -1329,7 +1337,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
             }
             const uint32_t sel = (uint32_t)readlane((int)low, __ffsll((unsigned long long)live) - 1);
             eps_key = lo + sel;
-            const bool in = act && low <= sel;
+            const bool in = low <= sel;   // inactive lanes: low = 0xffffffff
             queued = __popcll(__ballot(in)) != r0 || eps_key >= H;
             ex = wave_max_u(max(ex, in ? ca : 0u));
             ey = wave_max_u(max(ey, in ? cb : 0u));
@@ -1365,7 +1373,7 @@ __device__ __forceinline__ double rows_long(const LongRows& lr, int N, int T, in
         if (ey < qtop) {
             pready = false;
             pqtie = false;
-            pc1 = __popcll(__ballot(lane < kLongSum && lsq <= ey));
+            pc1 = __popcll(__ballot(lsq <= ey) & ((1ull << kLongSum) - 1));
             const uint32_t* sq = (sideA ? lr.LVb : lr.LVa) + (uint32_t)row * (uint32_t)kLongPitch + 64 * pc1;
             pblk = ld_lane<uint2>(sq, lane4 * 2u);
         } else {
@@ -1512,7 +1520,7 @@ __device__ __forceinline__ double row_coop(const float* Da, const float* Db, lon
                 }
             }
             eps_key = lo + sel;
-            const bool in = act && low <= sel;
+            const bool in = low <= sel;   // inactive lanes: low = 0xffffffff
             sel_tie = __popcll(__ballot(in)) != r0;
             rex = fmaxf(rex, wave_max(in ? ca : 0.0f));
             rey = fmaxf(rey, wave_max(in ? cb : 0.0f));
