@@ -7,14 +7,17 @@
 // subspaces; TE = mean over q of psi[k] + psi[n_xp+1] - psi[n_xfxp+1]
 // - psi[n_ypxp+1] (counts clamped to Nv).
 //
-// One lane per query frame, 64 queries per wave, one wave per block.  The
-// frame loop is uniform across the wave, so frame j's nine coordinates are
-// scalar loads.  Distances are FP64, computed exactly as the reference does.
-//   pass 1  the k smallest FP32 keys fl32(dist) in a per-lane 8-ary max-heap
-//           in LDS (rounding to FP32 is monotone, so the heap root is
+// One lane per query frame, 64 queries per wave.  The frame loop is uniform
+// across the wave, so frame j's nine coordinates are scalar loads.
+// Distances are FP64, computed exactly as the reference does.
+//   pass 1  (te_heap_kernel, one wave per block) the k smallest FP32 keys
+//           fl32(dist) in a per-lane 8-ary max-heap in LDS, k * 256 bytes
+//           per wave (rounding to FP32 is monotone, so the heap root is
 //           fl32(eps)); 8 children per node keep a sift-down at 3 levels for
-//           k = 100, each level's 8 reads independent;
-//   pass 2  eps and the counts together: a frame whose joint key is below
+//           k = 100, each level's 8 reads independent.  The root goes to HBM;
+//   pass 2  (te_count_kernel, four waves per block, no LDS, so occupancy
+//           is bounded by registers, not by the heap) eps and the counts
+//           together: a frame whose joint key is below
 //           the root lies inside eps with all its subspace distances, and a
 //           subspace distance whose FP32 key is below (above) the root is
 //           below (above) eps.  When exactly one frame has the root key it
@@ -51,11 +54,13 @@ __device__ __forceinline__ void load_frame(const double* __restrict__ X,
 
 }  // namespace
 
-template <int KMAX>
-__global__ __launch_bounds__(64) void te_kernel(TeArgs args)
+// Pass 1.  Block b = one wave = query block b % nqblocks of pair b /
+// nqblocks (pairs of this launch); root[64 b + lane] = fl32(eps) of its query.
+__global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
 {
     constexpr int kD = 8;   // heap arity
-    __shared__ float heap[KMAX][64];
+    extern __shared__ float heap_lds[];   // [k][64]
+    auto heap = [&](int t) -> float& { return heap_lds[t * 64 + threadIdx.x]; };
     const int lane = threadIdx.x;
     const int nqb = args.nqblocks;
     const long long p = blockIdx.x / nqb;
@@ -74,7 +79,7 @@ __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
     };
 
     // ---- pass 1: k smallest keys (max-heap, root = largest kept) -------
-    for (int t = 0; t < k; ++t) heap[t][lane] = INFINITY;
+    for (int t = 0; t < k; ++t) heap(t) = INFINITY;
     // the root lives in a register: a frame's test needs no LDS round trip
     float top = INFINITY;
     // key below the root: replace the root and sift down, return the new root
@@ -87,7 +92,7 @@ __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
             float ch[kD];
 #pragma unroll
             for (int c = 0; c < kD; ++c)   // slots past k are read (clamped) and ignored
-                ch[c] = heap[min(c0 + c, KMAX - 1)][lane];
+                ch[c] = heap(min(c0 + c, k - 1));
             int big = c0;
             float vb = ch[0];
 #pragma unroll
@@ -97,11 +102,11 @@ __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
                     big = c0 + c;
                 }
             if (vb <= key) break;
-            heap[pos][lane] = vb;
+            heap(pos) = vb;
             pos = big;
         }
-        heap[pos][lane] = key;
-        return heap[0][lane];
+        heap(pos) = key;
+        return heap(0);
     };
     // Keys below the root wait in kW registers per lane; when some lane
     // holds kW, every lane inserts its waiting keys together (each checked
@@ -139,7 +144,35 @@ __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
         if (__ballot(nw == kW)) flush();   // uniform over the wave
     }
     flush();
-    const float root = top;
+    args.root[(size_t)blockIdx.x * 64 + lane] = top;
+}
+
+// Pass 2 and the TE sum.  Wave w of block b runs item 4b + w = query block
+// item % nqblocks of pair item / nqblocks; its partial sum goes to
+// partial[item].
+__global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
+{
+    const int lane = threadIdx.x & 63;
+    const long long item = (long long)blockIdx.x * 4 +
+                           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nqb = args.nqblocks;
+    if (item >= args.npairs * nqb) return;
+    const long long p = item / nqb;
+    const int qb = (int)(item - p * nqb);
+    const int N = args.N, lag = args.lag, k = args.k, Nv = N - lag;
+    const double* X = args.traj + (size_t)args.a1[p] * N * 3;
+    const double* Y = args.traj + (size_t)args.a2[p] * N * 3;
+    const int q = qb * 64 + lane;
+    const bool active = q < Nv;
+    double qc[9];
+    load_frame(X, Y, active ? q : 0, lag, qc);
+    auto dist9 = [&](const double* c, double& dxf, double& dxp, double& dyp) {
+        dxf = fmax(fmax(fabs(qc[0] - c[0]), fabs(qc[1] - c[1])), fabs(qc[2] - c[2]));
+        dxp = fmax(fmax(fabs(qc[3] - c[3]), fabs(qc[4] - c[4])), fabs(qc[5] - c[5]));
+        dyp = fmax(fmax(fabs(qc[6] - c[6]), fabs(qc[7] - c[7])), fabs(qc[8] - c[8]));
+    };
+    const float root = args.root[(size_t)item * 64 + lane];
+    double cn[9];
 
     // ---- pass 2: eps and the strict counts from FP32 keys ----------------
     int clt = 0, gcnt = 0, amb = 0;
@@ -230,7 +263,7 @@ __global__ __launch_bounds__(64) void te_kernel(TeArgs args)
     }
     // wave sum, fixed order
     for (int o = 32; o >= 1; o >>= 1) te += __shfl_xor(te, o, 64);
-    if (lane == 0) args.partial[blockIdx.x] = te;
+    if (lane == 0) args.partial[item] = te;
 }
 
 __global__ void te_finalize_kernel(TeArgs args, double* mat, int natoms)
@@ -242,15 +275,27 @@ __global__ void te_finalize_kernel(TeArgs args, double* mat, int natoms)
     mat[(size_t)args.a1[p] * natoms + args.a2[p]] = s / (args.N - args.lag);
 }
 
+long long te_chunk_pairs(int nqblocks)
+{
+    const long long c = kTeChunkQueries / ((long long)nqblocks * 64);
+    return c > 0 ? c : 1;
+}
+
 hipError_t launch_te(const TeArgs& a, hipStream_t s)
 {
-    const unsigned grid = (unsigned)(a.npairs * a.nqblocks);
-    if (a.k <= 32)
-        hipLaunchKernelGGL(te_kernel<32>, dim3(grid), dim3(64), 0, s, a);
-    else if (a.k <= 128)
-        hipLaunchKernelGGL(te_kernel<128>, dim3(grid), dim3(64), 0, s, a);
-    else
-        return hipErrorInvalidValue;
+    if (a.k < 1 || a.k > 128 || !a.root) return hipErrorInvalidValue;
+    const long long chunk = te_chunk_pairs(a.nqblocks);
+    const size_t lds = (size_t)a.k * 64 * sizeof(float);
+    for (long long p0 = 0; p0 < a.npairs; p0 += chunk) {
+        TeArgs c = a;
+        c.a1 = a.a1 + p0;
+        c.a2 = a.a2 + p0;
+        c.partial = a.partial + p0 * a.nqblocks;
+        c.npairs = a.npairs - p0 < chunk ? a.npairs - p0 : chunk;
+        const long long items = c.npairs * a.nqblocks;
+        hipLaunchKernelGGL(te_heap_kernel, dim3((unsigned)items), dim3(64), lds, s, c);
+        hipLaunchKernelGGL(te_count_kernel, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, s, c);
+    }
     return hipGetLastError();
 }
 

@@ -69,10 +69,13 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
     for (int i = 1; i < Nv; ++i) psi[i + 1] = psi[i] + 1.0 / i;
     const int nqb = (Nv + 63) / 64;
     DevBuf<double> dTraj, dPsi, dPart, dMat;
+    DevBuf<float> dRoot;
     DevBuf<int> d1, d2;
     HIPCHK(dTraj.alloc((size_t)natoms * nframes * 3));
     HIPCHK(dPsi.alloc(psi.size()));
     HIPCHK(dPart.alloc((size_t)npairs * nqb));
+    const long long chunk = gmxcorr::te_chunk_pairs(nqb);
+    HIPCHK(dRoot.alloc((size_t)(npairs < chunk ? npairs : chunk) * nqb * 64));
     HIPCHK(d1.alloc(npairs));
     HIPCHK(d2.alloc(npairs));
     HIPCHK(dMat.alloc(mat ? (size_t)natoms * natoms : (size_t)npairs));
@@ -86,7 +89,7 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
     HIPCHK(hipMemcpyAsync(dPsi.p, psi.data(), psi.size() * sizeof(double), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d1.p, a1, npairs * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d2.p, a2, npairs * sizeof(int), hipMemcpyHostToDevice, s));
-    gmxcorr::TeArgs ta{dTraj.p, nframes, lag, k, d1.p, d2.p, npairs, nqb, dPsi.p, dPart.p};
+    gmxcorr::TeArgs ta{dTraj.p, nframes, lag, k, d1.p, d2.p, npairs, nqb, dPsi.p, dPart.p, dRoot.p};
     HIPCHK(hipEventRecord(e0, s));
     HIPCHK(gmxcorr::launch_te(ta, s));
     HIPCHK(hipEventRecord(e1, s));

@@ -15,8 +15,15 @@ struct TeArgs {
     int nqblocks;         // ceil((N - lag) / 64)
     const double* psi;    // [N - lag + 1]
     double* partial;      // [npairs][nqblocks]
+    float* root;          // [te_chunk_pairs(nqblocks) * nqblocks * 64] per-query fl32(eps)
 };
 
+// Queries (64 per query block) per launch pair of the heap and count
+// kernels: each dispatch stays far below 2^32 work-items, and the root
+// buffer at 4 bytes per query below 2 GiB.
+constexpr long long kTeChunkQueries = 1LL << 29;
+// Pairs per launch pair for nqblocks query blocks per pair.
+long long te_chunk_pairs(int nqblocks);
 hipError_t launch_te(const TeArgs& a, hipStream_t s);
 hipError_t launch_te_finalize(const TeArgs& a, double* mat, int natoms, hipStream_t s);
 
