@@ -125,23 +125,35 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         }
         nw = 0;
     };
-    // frame j + 1's coordinates are loaded while frame j is processed
-    double cn[9];
-    load_frame(X, Y, 0, lag, cn);
-    for (int j = 0; j < Nv; ++j) {
-        double c[9];
+    // two frames per iteration (two independent FP64 chains); frames j + 2
+    // and j + 3 are loaded while j and j + 1 are processed.  A lane takes
+    // at most two keys per iteration, so the buffer is flushed once some
+    // lane holds kW - 1.
+    double cn0[9], cn1[9];
+    load_frame(X, Y, 0, lag, cn0);
+    load_frame(X, Y, min(1, Nv - 1), lag, cn1);
+    auto take = [&](float key) {
 #pragma unroll
-        for (int d = 0; d < 9; ++d) c[d] = cn[d];
-        load_frame(X, Y, min(j + 1, Nv - 1), lag, cn);
-        double dxf, dxp, dyp;
-        dist9(c, dxf, dxp, dyp);
-        const float key = (float)fmax(fmax(dxf, dxp), dyp);
-        if (active && j != q && key < top) {
+        for (int u = 0; u < kW; ++u) wk[u] = nw == u ? key : wk[u];
+        ++nw;
+    };
+    for (int j = 0; j < Nv; j += 2) {
+        double c0[9], c1[9];
 #pragma unroll
-            for (int u = 0; u < kW; ++u) wk[u] = nw == u ? key : wk[u];
-            ++nw;
+        for (int d = 0; d < 9; ++d) {
+            c0[d] = cn0[d];
+            c1[d] = cn1[d];
         }
-        if (__ballot(nw == kW)) flush();   // uniform over the wave
+        load_frame(X, Y, min(j + 2, Nv - 1), lag, cn0);
+        load_frame(X, Y, min(j + 3, Nv - 1), lag, cn1);
+        double a0, b0, e0, a1, b1, e1;
+        dist9(c0, a0, b0, e0);
+        dist9(c1, a1, b1, e1);
+        const float key0 = (float)fmax(fmax(a0, b0), e0);
+        const float key1 = (float)fmax(fmax(a1, b1), e1);
+        if (active && j != q && key0 < top) take(key0);
+        if (active && j + 1 < Nv && j + 1 != q && key1 < top) take(key1);
+        if (__ballot(nw >= kW - 1)) flush();   // uniform over the wave
     }
     flush();
     args.root[(size_t)blockIdx.x * 64 + lane] = top;
