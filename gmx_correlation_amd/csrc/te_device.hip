@@ -88,22 +88,29 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         for (;;) {
             const int c0 = kD * pos + 1;
             if (c0 >= k) break;
-            // largest child among c0 .. min(c0 + kD, k) - 1
+            // largest child among c0 .. min(c0 + kD, k) - 1: a tree of
+            // pairwise maxima (log2 kD dependent steps); slots past k are
+            // read clamped and count as -inf
             float ch[kD];
+            int ix[kD];
 #pragma unroll
-            for (int c = 0; c < kD; ++c)   // slots past k are read (clamped) and ignored
-                ch[c] = heap(min(c0 + c, k - 1));
-            int big = c0;
-            float vb = ch[0];
+            for (int c = 0; c < kD; ++c) {
+                const float h = heap(min(c0 + c, k - 1));
+                ch[c] = c0 + c < k ? h : -INFINITY;
+                ix[c] = c;
+            }
 #pragma unroll
-            for (int c = 1; c < kD; ++c)
-                if (c0 + c < k && ch[c] > vb) {
-                    vb = ch[c];
-                    big = c0 + c;
+            for (int w = kD / 2; w >= 1; w /= 2) {
+#pragma unroll
+                for (int c = 0; c < w; ++c) {
+                    const bool r = ch[c + w] > ch[c];
+                    ch[c] = r ? ch[c + w] : ch[c];
+                    ix[c] = r ? ix[c + w] : ix[c];
                 }
-            if (vb <= key) break;
-            heap(pos) = vb;
-            pos = big;
+            }
+            if (ch[0] <= key) break;
+            heap(pos) = ch[0];
+            pos = c0 + ix[0];
         }
         heap(pos) = key;
         return heap(0);
