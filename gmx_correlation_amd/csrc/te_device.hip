@@ -194,6 +194,11 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
     double cn[9];
 
     // ---- pass 2: eps and the strict counts from FP32 keys ----------------
+    // Every frame, the self frame included (removed after the loop), adds
+    // its strict FP32 comparisons; frames with the root key are gathered
+    // under a wave-uniform branch (rarely taken) and their own terms fixed
+    // after the loop.  fl32 is monotone, so the keys are maxima of the
+    // converted 3-D distances.
     int clt = 0, gcnt = 0, amb = 0;
     int nxfxp = 0, nypxp = 0, nxp = 0;
     double gmin = INFINITY, gmax = -INFINITY;
@@ -206,27 +211,49 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
         load_frame(X, Y, min(j + 1, Nv - 1), lag, cn);
         double dxf, dxp, dyp;
         dist9(c, dxf, dxp, dyp);
-        if (j == q) continue;
-        const double dd = fmax(fmax(dxf, dxp), dyp);
-        const float key = (float)dd;
-        const double s1 = fmax(dxf, dxp), s2 = fmax(dyp, dxp), s3 = dxp;
+        const float fxf = (float)dxf, fxp = (float)dxp, fyp = (float)dyp;
+        const float k1 = fmaxf(fxf, fxp), k2 = fmaxf(fyp, fxp);
+        const float key = fmaxf(k1, fyp);
         clt += key < root;
-        if (key == root) {
-            ++gcnt;
-            gmin = fmin(gmin, dd);
-            gmax = fmax(gmax, dd);
-            g1 = s1;
-            g2 = s2;
-            g3 = s3;
-        } else {
-            const float k1 = (float)s1, k2 = (float)s2, k3 = (float)s3;
-            nxfxp += k1 < root;
-            nypxp += k2 < root;
-            nxp += k3 < root;
-            amb += (k1 == root) + (k2 == root) + (k3 == root);
+        nxfxp += k1 < root;
+        nypxp += k2 < root;
+        nxp += fxp < root;
+        amb += (k1 == root) + (k2 == root) + (fxp == root);
+        if (__ballot(key == root)) {   // uniform
+            if (key == root && j != q) {
+                ++gcnt;
+                const double dd = fmax(fmax(dxf, dxp), dyp);
+                gmin = fmin(gmin, dd);
+                gmax = fmax(gmax, dd);
+                g1 = fmax(dxf, dxp);
+                g2 = fmax(dyp, dxp);
+                g3 = dxp;
+            }
         }
     }
-    if (gcnt == 1 && amb == 0) {
+    // the self frame (all distances 0): below a positive root it added one
+    // to each count; at a zero root it added three equalities
+    if (root > 0.0f) {
+        --clt;
+        --nxfxp;
+        --nypxp;
+        --nxp;
+    } else {
+        amb -= 3;
+    }
+    // a root-key frame added its subspace keys below the root to the
+    // counts and those equal to it to amb; the fast path below adds them
+    // back as FP64 comparisons with eps
+    if (gcnt == 1) {
+        const float r1 = (float)g1, r2 = (float)g2, r3 = (float)g3;
+        amb -= (r1 == root) + (r2 == root) + (r3 == root);
+        nxfxp -= r1 < root;
+        nypxp -= r2 < root;
+        nxp -= r3 < root;
+    }
+    if (!active) {
+        // no query: nothing to count
+    } else if (gcnt == 1 && amb == 0) {
         // the one root-key frame is the k-th: eps = its distance, and its
         // subspace distances (<= eps) count when below it
         const double eps = gmin;

@@ -44,3 +44,20 @@ def test_te_pairs_and_short_series(gpu, oracle_lib):
     ref = [oracle_lib.te_pair(traj, 0, 3, 10, 2), oracle_lib.te_pair(traj, 4, 1, 10, 2)]
     assert np.max(np.abs(te - ref)) <= 1e-12 and ms > 0
     assert not dense.te_matrix(traj[:, :12], k=10, lag=4).any()
+
+
+def test_te_duplicate_frames_vs_oracle(gpu, oracle_lib):
+    """Three atoms move on two levels of one axis, so many 9-D points
+    coincide exactly: at k = 5 the k-th distance is 0 for most queries (the
+    query's own frame then has the root key too), at k = 40 and 100 the
+    root is positive with many frames tied at it; the fourth atom is
+    continuous."""
+    rng = np.random.default_rng(21)
+    c = np.zeros((300, 4, 3), np.float32)
+    c[:, :, 0] = rng.integers(0, 2, (300, 4))
+    c[:, 3, :] += rng.standard_normal((300, 3)).astype(np.float32)
+    traj = ksg.center(c)
+    for k, lag in ((5, 1), (40, 2), (100, 1)):
+        mat = dense.te_matrix(traj, k=k, lag=lag)
+        ref = oracle_lib.te_matrix(traj, k, lag)
+        assert np.max(np.abs(mat - ref)) <= 1e-12, (k, lag)
