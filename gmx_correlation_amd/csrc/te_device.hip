@@ -11,8 +11,8 @@
 // across the wave, so frame j's nine coordinates are scalar loads.
 // Distances are FP64, computed exactly as the reference does.
 //   pass 1  (te_heap_kernel, one wave per block) the k smallest FP32 keys
-//           fl32(dist) in a per-lane 8-ary max-heap in LDS, k * 256 bytes
-//           per wave (rounding to FP32 is monotone, so the heap root is
+//           fl32(dist) in a per-lane 8-ary max-heap in LDS, (k + 1) * 256
+//           bytes per wave (rounding to FP32 is monotone, so the heap root is
 //           fl32(eps)); 8 children per node keep a sift-down at 3 levels for
 //           k = 100, each level's 8 reads independent.  The root goes to HBM;
 //   pass 2  (te_count_kernel, four waves per block, no LDS, so occupancy
@@ -59,7 +59,7 @@ __device__ __forceinline__ void load_frame(const double* __restrict__ X,
 __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
 {
     constexpr int kD = 8;   // heap arity
-    extern __shared__ float heap_lds[];   // [k][64]
+    extern __shared__ float heap_lds[];   // [k + 1][64]
     auto heap = [&](int t) -> float& { return heap_lds[t * 64 + threadIdx.x]; };
     const int lane = threadIdx.x;
     const int nqb = args.nqblocks;
@@ -79,7 +79,13 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
     };
 
     // ---- pass 1: k smallest keys (max-heap, root = largest kept) -------
-    for (int t = 0; t < k; ++t) heap(t) = INFINITY;
+    // The heap keeps kh = k + 1 keys and takes every frame, the query's own
+    // included: its key 0 is the smallest, so the (k+1)-th smallest key with
+    // it is the k-th smallest without it (also when other frames tie at 0),
+    // and the frame loop needs no per-frame self test.  Lanes without a
+    // query fill a heap whose root is not used.
+    const int kh = k + 1;
+    for (int t = 0; t < kh; ++t) heap(t) = INFINITY;
     // the root lives in a register: a frame's test needs no LDS round trip
     float top = INFINITY;
     // key below the root: replace the root and sift down, return the new root
@@ -87,16 +93,16 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         int pos = 0;
         for (;;) {
             const int c0 = kD * pos + 1;
-            if (c0 >= k) break;
-            // largest child among c0 .. min(c0 + kD, k) - 1: a tree of
-            // pairwise maxima (log2 kD dependent steps); slots past k are
+            if (c0 >= kh) break;
+            // largest child among c0 .. min(c0 + kD, kh) - 1: a tree of
+            // pairwise maxima (log2 kD dependent steps); slots past kh are
             // read clamped and count as -inf
             float ch[kD];
             int ix[kD];
 #pragma unroll
             for (int c = 0; c < kD; ++c) {
-                const float h = heap(min(c0 + c, k - 1));
-                ch[c] = c0 + c < k ? h : -INFINITY;
+                const float h = heap(min(c0 + c, kh - 1));
+                ch[c] = c0 + c < kh ? h : -INFINITY;
                 ix[c] = c;
             }
 #pragma unroll
@@ -158,8 +164,8 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         dist9(c1, a1, b1, e1);
         const float key0 = (float)fmax(fmax(a0, b0), e0);
         const float key1 = (float)fmax(fmax(a1, b1), e1);
-        if (active && j != q && key0 < top) take(key0);
-        if (active && j + 1 < Nv && j + 1 != q && key1 < top) take(key1);
+        if (key0 < top) take(key0);
+        if (j + 1 < Nv && key1 < top) take(key1);
         if (__ballot(nw >= kW - 1)) flush();   // uniform over the wave
     }
     flush();
@@ -331,7 +337,7 @@ hipError_t launch_te(const TeArgs& a, hipStream_t s)
 {
     if (a.k < 1 || a.k > 128 || !a.root) return hipErrorInvalidValue;
     const long long chunk = te_chunk_pairs(a.nqblocks);
-    const size_t lds = (size_t)a.k * 64 * sizeof(float);
+    const size_t lds = (size_t)(a.k + 1) * 64 * sizeof(float);
     for (long long p0 = 0; p0 < a.npairs; p0 += chunk) {
         TeArgs c = a;
         c.a1 = a.a1 + p0;
