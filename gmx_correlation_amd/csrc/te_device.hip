@@ -121,10 +121,10 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         heap(pos) = key;
         return heap(0);
     };
-    // Keys below the root wait in kW registers per lane; when some lane
-    // holds kW, every lane inserts its waiting keys together (each checked
-    // against the root again), so the wave runs the sift with most lanes
-    // active instead of for one lane at a time.
+    // Keys below the root wait in kW registers per lane; when some lane is
+    // about to run out of them, every lane inserts its waiting keys together
+    // (each checked against the root again), so the wave runs the sift with
+    // most lanes active instead of for one lane at a time.
     constexpr int kW = 8;
     float wk[kW];
 #pragma unroll
