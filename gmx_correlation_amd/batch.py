@@ -23,14 +23,16 @@ from .ksg import KsgPlan
 # kind 1 = four waves per row (N <= 1024*s4), kind 2 = one wave per row over
 # the 512 nearest frames of the sparser atom (N <= 2048, k + 1 <= 256), kind 3
 # = the same over the first 256*s4 frames of 1024-frame prefix stores
-# (2048 < N <= 8192, k + 1 <= 256; s4 = 2 or 4).
-BATCH_PATHS = [(2, 1), (3, 2), (3, 4), (0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (1, 4), (1, 6), (1, 8)]
+# (2048 < N <= 8192, k + 1 <= 256; s4 = 2, 3 or 4).
+BATCH_PATHS = [(2, 1), (3, 2), (3, 3), (3, 4), (0, 2), (0, 4), (0, 6), (0, 8), (1, 3), (1, 4),
+               (1, 6), (1, 8)]
 PREFIX_MAX_FRAMES = 2048
 LONG_MAX_FRAMES = 8192
 PREFIX_MAX_T = 256
-# Largest share of a trajectory's rows that may fall beyond the 512-frame
-# prefix of kind 3 before the 1024-frame prefix is faster: a long row costs
-# about 4x a prefix row and the longer prefix about 1.4x (break-even near 6%).
+# Largest share of a trajectory's rows that may fall beyond the prefix of
+# kind 3 before a longer prefix is faster: a long row costs about 4x a
+# prefix row and the 1024-frame prefix about 1.4x the 512-frame one
+# (break-even near 6%).
 LONG_SHARE_MAX = 0.04
 
 
@@ -60,11 +62,12 @@ def tune_variant(rows: np.ndarray, psi: np.ndarray, k: int, natoms_probe: int = 
     m = len(probe)
     sub = rows.reshape(natoms, 3, N)[probe].reshape(3 * m, N)
     cands = paths if paths is not None else fitting_paths(N, k)
-    # keep only the two smallest of each kind: larger register tiles only add work
+    # keep only the two smallest of each streaming kind (larger register
+    # tiles only add work); every prefix length of kind 3
     by_kind: dict[int, list[int]] = {}
     for kind, s4 in sorted(cands):
         by_kind.setdefault(kind, []).append(s4)
-    cands = [(kd, s) for kd, ss in by_kind.items() for s in ss[:2]]
+    cands = [(kd, s) for kd, ss in by_kind.items() for s in (ss if kd == 3 else ss[:2])]
     a1, a2 = np.tril_indices(m, -1)
     timings = {}
     long_share: dict[tuple[int, int], float] = {}
@@ -82,10 +85,15 @@ def tune_variant(rows: np.ndarray, psi: np.ndarray, k: int, natoms_probe: int = 
             timings[f"{kind}:{s4}"] = float(np.min(t))
     best = min(timings, key=timings.get)
     kind, s4 = (int(v) for v in best.split(":"))
-    if (kind, s4) == (3, 2) and long_share.get((3, 2), 0.0) > LONG_SHARE_MAX:
+    if kind == 3:
         # a probe of a few pairs leaves the GPU idle and hides the cost of
-        # the long rows, which a full launch pays at about 4x a prefix row
-        kind, s4 = 3, 4
+        # the long rows, which a full launch pays at about 4x a prefix row:
+        # the shortest prefix, no shorter than the fastest, whose share of
+        # rows beyond it stays within LONG_SHARE_MAX (else the whole store)
+        for c in range(s4, 5):
+            if c == 4 or long_share.get((3, c), 1.0) <= LONG_SHARE_MAX:
+                s4 = c
+                break
     return (kind, s4), timings
 
 

@@ -1675,8 +1675,8 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
 
 // Long-trajectory prefix kernel (kind 3): four pairs per block (a 2x2 atom
 // tile, as ksg_pair_prefix_kernel), 1 KB of LDS per wave.  C: 256-frame
-// chunks of the prefix read per row (2: 512 frames, 4: the whole 1024-frame
-// store).  A block runs one chunk of kLongChunk frame rows, and blocks are
+// chunks of the prefix read per row (2: 512 frames, 3: 768, 4: the whole
+// 1024-frame store).  A block runs one chunk of kLongChunk frame rows, and blocks are
 // ordered chunk-major: the blocks in flight on one XCD run consecutive tiles,
 // which share their first two atoms, over the same kLongChunk rows, so those
 // atoms' distance rows and prefixes are read from that XCD's L2.  Each
@@ -1835,9 +1835,9 @@ __device__ __forceinline__ void batch_long(const TrajDesc& d, const KsgBatchArgs
     const int i0 = (int)chunk * kLongChunk;
     const int i1 = min(d.N, i0 + kLongChunk);
     const LongRows lr = long_rows(d.D, d.LV, d.LI, d.LS, d.P, d.N, a1, a2);
-    const double acc = d.s4 == 2
-                           ? rows_long<2>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o)
-                           : rows_long<4>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o);
+    const double acc = d.s4 == 2   ? rows_long<2>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o)
+                       : d.s4 == 3 ? rows_long<3>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o)
+                                   : rows_long<4>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o);
     if (lane == 0) atomicAdd(&b.fixsum[gp], acc);
 }
 
@@ -2242,7 +2242,7 @@ bool variant_fits(int kind, int s4, int N, int K)
 {
     if (kind == 2) return N <= kSortedMaxFrames && K + 1 <= kPrefixMaxT;
     if (kind == 3)
-        return (s4 == 2 || s4 == 4) && N > kSortedMaxFrames && N <= kMaxFrames &&
+        return s4 >= 2 && s4 <= 4 && N > kSortedMaxFrames && N <= kMaxFrames &&
                K + 1 <= kPrefixMaxT;
     const long long cap = (kind == 0 ? 256LL : 1024LL) * s4;
     const bool inst = kind == 0 ? (s4 == 1 || s4 == 2 || s4 == 3 || s4 == 4 || s4 == 5 ||
@@ -2298,6 +2298,10 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4, i
             hipLaunchKernelGGL((ksg_pair_long_kernel<2, true>), grid, block, 0, s, c);
         else if (s4 == 2)
             hipLaunchKernelGGL((ksg_pair_long_kernel<2, false>), grid, block, 0, s, c);
+        else if (s4 == 3 && c.rc_nx)
+            hipLaunchKernelGGL((ksg_pair_long_kernel<3, true>), grid, block, 0, s, c);
+        else if (s4 == 3)
+            hipLaunchKernelGGL((ksg_pair_long_kernel<3, false>), grid, block, 0, s, c);
         else if (c.rc_nx)
             hipLaunchKernelGGL((ksg_pair_long_kernel<4, true>), grid, block, 0, s, c);
         else

@@ -185,9 +185,9 @@ hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
 // kind 1: four waves per row, N <= 1024*s4 (s4 in 2..8);
 // kind 2: one wave per row over the sorted prefix of the sparser atom,
 //         N <= kSortedMaxFrames and K + 1 <= kPrefixMaxT (s4 ignored, 1);
-// kind 3: the same over the kLongPrefix prefix stores, gathering the other
-//         atom's distances from its row, kSortedMaxFrames < N <= kMaxFrames
-//         and K + 1 <= kPrefixMaxT (s4 ignored, 1);
+// kind 3: the same over the first 256*s4 frames of the kLongPrefix prefix
+//         stores (s4 in 2, 3, 4), gathering the other atom's distances from
+//         its row, kSortedMaxFrames < N <= kMaxFrames and K + 1 <= kPrefixMaxT;
 // kind < 0: default.
 bool variant_fits(int kind, int s4, int N, int K);
 void default_variant(int N, int K, int& kind, int& s4);

@@ -120,7 +120,7 @@ class KsgPlan:
         (256*s4 frames), kind 1 four waves per row (1024*s4 frames), kind 2
         one wave per row over the sorted prefix (N <= 2048, k + 1 <= 256),
         kind 3 the same over the first 256*s4 frames of the prefix stores
-        (2048 < N <= 8192, s4 = 2 or 4), kind -1 default."""
+        (2048 < N <= 8192, s4 = 2, 3 or 4), kind -1 default."""
         _lib.check(_lib.load().gmxcorr_ksg_plan_set_variant(self._h, kind, s4))
 
     def stats(self) -> dict:

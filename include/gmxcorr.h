@@ -87,7 +87,7 @@ int  gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* plan, gmxcorr_ksg_stats* st)
  * (s4 in 2..8); kind 2 = one wavefront per row over the 512 nearest frames
  * of the sparser atom (nframes <= 2048, k + 1 <= 256; s4 ignored); kind 3 =
  * the same over the first 256*s4 frames of 1024-frame prefix stores
- * (2048 < nframes <= 8192, k + 1 <= 256, s4 = 2 or 4; built when they fit
+ * (2048 < nframes <= 8192, k + 1 <= 256, s4 = 2, 3 or 4; built when they fit
  * in HBM); kind -1 =
  * the default for the plan's frame count and k.  variant_fits: 1 if
  * (kind, s4) is compiled and covers nframes at this k. */

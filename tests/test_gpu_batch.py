@@ -68,10 +68,10 @@ def test_batch_explicit_paths(gpu, trajs, t):
         assert np.max(np.abs(mi - ref)) <= 1e-12, (kind, s4)
 
 
-@pytest.mark.parametrize("N,k,path", [(2000, 200, (2, 1)), (6000, 100, (3, 2))])
+@pytest.mark.parametrize("N,k,path", [(2000, 200, (2, 1)), (6000, 100, (3, 2)), (6000, 200, (3, 3))])
 def test_batch_long_rows(gpu, oracle_lib, N, k, path):
     """Independent atoms, whose (k+1)-th joint neighbour lies beyond the
-    prefix (512 frames) in most rows: the batch kernel queues those rows and
+    prefix read (512 frames, or 768 at k = 200) in most rows: the batch kernel queues those rows and
     the cooperative streaming pass resolves them; counts stay exact, so MI
     equals the FP64 brute recount."""
     rng = np.random.default_rng(N + k)

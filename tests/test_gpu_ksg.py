@@ -203,11 +203,11 @@ def test_prefix_overflow_rows_take_streaming_path(gpu, oracle_lib):
         assert abs(mi[p] - v) <= 1e-12
 
 
-@pytest.mark.parametrize("variant", [None, (3, 2), (3, 4), (1, 0)])
+@pytest.mark.parametrize("variant", [None, (3, 2), (3, 3), (3, 4), (1, 0)])
 @pytest.mark.parametrize("N,k,npairs", [(3000, 50, 2), (5000, 100, 1), (8000, 10, 1)])
 def test_long_trajectories(gpu, oracle_lib, N, k, npairs, variant):
-    """N > 2048: the default path, the prefix-store path (kind 3) over 512
-    and 1024 prefix frames, and the 4-waves-per-row streaming kernel
+    """N > 2048: the default path, the prefix-store path (kind 3) over 512,
+    768 and 1024 prefix frames, and the 4-waves-per-row streaming kernel
     (kind 1); counts and MI against the FP64 brute-force recount."""
     c = synth.generate(synth.TrajSpec(natoms=5, nframes=N, nmodes=4, seed=N))
     rows, psi = ksg.kraskov_prepare(ksg.center(c))
@@ -290,13 +290,13 @@ def test_rejects_bad_pair_indices(gpu, small_ws):
             plan.run_pairs([64], [0])
 
 
-@pytest.mark.parametrize("k,s4", [(250, 4), (100, 2)])
+@pytest.mark.parametrize("k,s4", [(250, 4), (180, 3), (100, 2)])
 def test_long_prefix_overflow_rows_take_cooperative_path(gpu, oracle_lib, k, s4):
     """Independent atoms at 6000 frames: the (k+1)-th joint neighbour lies
     about sqrt((k+1)*6000) frames deep in either atom's sorted row (1227 at
-    k = 250, 778 at k = 100), beyond the prefix read (1024 or 512 frames) in
-    most rows, so those rows run the cooperative streaming row; counts equal
-    the FP64 recount."""
+    k = 250, 1042 at k = 180, 778 at k = 100), beyond the prefix read (1024,
+    768 or 512 frames) in most rows, so those rows run the cooperative
+    streaming row; counts equal the FP64 recount."""
     rng = np.random.default_rng(12)
     coords = rng.standard_normal((6000, 4, 3)).astype(np.float32)
     rows, psi = ksg.kraskov_prepare(ksg.center(coords))

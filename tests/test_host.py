@@ -53,7 +53,7 @@ def test_plan_rejects_bad_shapes_without_gpu(hiplib):
 
 def test_variant_rules_without_gpu(hiplib):
     """The code-path table of the pair kernels (host logic, no GPU): kind 2
-    up to 2048 frames, kind 3 above with a 512- or 1024-frame prefix, both
+    up to 2048 frames, kind 3 above with a 512-, 768- or 1024-frame prefix, both
     for k + 1 <= 256; the streaming kinds by register tile; and the batch
     tuner's Python table agrees with the library on every batch path."""
     from gmx_correlation_amd import batch
@@ -61,7 +61,8 @@ def test_variant_rules_without_gpu(hiplib):
     assert f(2, 1, 2000, 100) == 1 and f(2, 1, 2049, 100) == 0
     assert f(2, 1, 2000, 255) == 1 and f(2, 1, 2000, 256) == 0
     assert f(3, 2, 5000, 100) == 1 and f(3, 4, 5000, 100) == 1
-    assert f(3, 1, 5000, 100) == 0 and f(3, 3, 5000, 100) == 0
+    assert f(3, 3, 5000, 100) == 1
+    assert f(3, 1, 5000, 100) == 0 and f(3, 5, 5000, 100) == 0
     assert f(3, 4, 2048, 10) == 0 and f(3, 4, 8192, 10) == 1 and f(3, 4, 8193, 10) == 0
     assert f(0, 8, 2048, 10) == 1 and f(0, 8, 2049, 10) == 0 and f(0, 7, 1000, 10) == 0
     assert f(1, 2, 2048, 10) == 1 and f(1, 8, 8192, 10) == 1 and f(1, 9, 9000, 10) == 0
