@@ -210,13 +210,18 @@ inline bool variant_prefix_store(int kind) { return kind == 3; }
 // Prefix chunks (256 frames) the long-trajectory path reads by default.  For
 // two independent atoms the (k+1)-th joint neighbour lies about
 // sqrt((k+1) N) frames deep in either atom's sorted row; correlated atoms
-// need fewer.  512 frames while that depth stays below 512, else the whole
-// 1024-frame store: rows beyond the prefix run the streaming path, about 4x
-// the cost of a prefix row.  Measured on 120-atom trajectories of 20 modes
-// with switch and rotor couplings: 512 frames is 1.1-1.4x faster at
-// (N, k) = (2500, 100), (3000, 50), (8000, 10); the two tie at (6000, 50);
-// at (5000, 100) a third of the rows overflow 512 and 1024 is 2.6x faster.
-inline int long_prefix_chunks(int N, int K) { return (long long)(K + 1) * N <= (1LL << 18) ? 2 : 4; }
+// need fewer.  Rows beyond the prefix run the streaming path, about 4x the
+// cost of a prefix row.  Measured on 120-atom trajectories of 20 modes with
+// switch and rotor couplings: 512 frames is fastest or ties at (N, k) =
+// (2500, 100), (3000, 50), (4000, 50), (7000, 30), (8000, 10), all with
+// (k+1) N <= 2^18; 768 frames is fastest, 4-9% ahead of 1024, at (4000, 100),
+// (5000, 100), (6000, 50), (8000, 50), (k+1) N up to 505,000.  Above 2^19
+// the whole 1024-frame store.
+inline int long_prefix_chunks(int N, int K)
+{
+    const long long depth2 = (long long)(K + 1) * N;
+    return depth2 <= (1LL << 18) ? 2 : depth2 <= (1LL << 19) ? 3 : 4;
+}
 // kinds: bit k set when some trajectory of the launch takes kind k.
 hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, int kinds, hipStream_t s);
 // The batch's long rows (b.longq, nlong entries): cooperative streaming row,
