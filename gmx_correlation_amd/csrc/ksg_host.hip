@@ -214,14 +214,20 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
     a.counters = pl->dCounters;
     a.npairs = npairs; a.P = pl->P; a.N = pl->N; a.K = pl->K; a.psiN_phiK = pl->psiN_phiK;
     if (detail) { a.rc_nx = pl->dRcNx; a.rc_ny = pl->dRcNy; a.rc_flag = pl->dRcFlag; }
-    // recount and long-row queues sized for 1% of the rows up front (runs
-    // measured here queue 0.04-0.2%); an overflow triggers a full re-run
+    // Queues sized up front, as an overflow re-runs the whole launch: the
+    // FP32-tie recount for 1% of the rows (runs measured here queue
+    // 0.04-0.2%), the long-row queue for 4% (a prefix path keeps its share
+    // of rows beyond the prefix below that; 1.8% at 400 atoms x 5000
+    // frames, k = 100, 768 frames), at most 64M and 256M entries (1 and 4
+    // GiB).  Both keep their size for later runs.
     {
         const long long rows = npairs * (long long)pl->N;
-        const long long want = std::min<long long>(std::max<long long>(65536, rows / 100),
-                                                   64LL << 20);
-        int rc = ensure_fix(pl, (unsigned)want);
-        if (!rc) rc = ensure_long(pl, (unsigned)want);
+        const long long want_fix = std::min<long long>(std::max<long long>(65536, rows / 100),
+                                                       64LL << 20);
+        const long long want_long = std::min<long long>(std::max<long long>(65536, rows / 25),
+                                                        256LL << 20);
+        int rc = ensure_fix(pl, (unsigned)want_fix);
+        if (!rc) rc = ensure_long(pl, (unsigned)want_long);
         if (rc) return rc;
     }
     gmxcorr::FixupArgs f{};
