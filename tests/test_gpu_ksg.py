@@ -366,3 +366,26 @@ def test_boundary_shapes_all_pairs(gpu, oracle_lib, natoms, N, k):
     _, rd = oracle_lib.ksg_pair(rows, psi, int(a1[0]), int(a2[0]), k, method="brute", detail=True)
     assert np.array_equal(d["nx"][0], rd["nx"])
     assert np.array_equal(d["ny"][0], rd["ny"])
+
+
+@pytest.mark.parametrize("N,k", [(2000, 100), (3000, 50)])
+def test_xtc_precision_coordinates(gpu, oracle_lib, N, k):
+    """Coordinates rounded to 0.001 nm, the precision XTC files store: the
+    distance rows hold many equal values and the k-th neighbour ties often.
+    Sorted-prefix path at 2000 frames, prefix-store path at 3000; per-frame
+    counts equal the FP64 brute-force recount."""
+    c = synth.generate(synth.TrajSpec(natoms=4, nframes=N, nmodes=3, seed=31))
+    c = (np.round(c.astype(np.float64) * 1000.0) / 1000.0).astype(np.float32)
+    rows, psi = ksg.kraskov_prepare(ksg.center(c))
+    a1 = np.array([1, 3, 2]); a2 = np.array([0, 2, 1])
+    with ksg.KsgPlan(4, N, k) as plan:
+        plan.load(rows, psi)
+        mi, d = plan.run_pairs(a1, a2, detail=True)
+        st = plan.stats()
+    assert st["fixup_rows"] > 0
+    for p in range(len(a1)):
+        v, ref = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), k, method="brute",
+                                     detail=True)
+        assert np.array_equal(d["nx"][p], ref["nx"]), p
+        assert np.array_equal(d["ny"][p], ref["ny"]), p
+        assert abs(mi[p] - v) <= 1e-12
