@@ -10,11 +10,13 @@
 // One lane per query frame, 64 queries per wave.  The frame loop is uniform
 // across the wave, so frame j's nine coordinates are scalar loads.
 // Distances are FP64, computed exactly as the reference does.
-//   pass 1  (te_heap_kernel, one wave per block) the k smallest FP32 keys
-//           fl32(dist) in a per-lane 8-ary max-heap in LDS, (k + 1) * 256
-//           bytes per wave (rounding to FP32 is monotone, so the heap root is
-//           fl32(eps)); 8 children per node keep a sift-down at 3 levels for
-//           k = 100, each level's 8 reads independent.  The root goes to HBM;
+//   pass 1  (te_heap_kernel, one wave per block) the k + 1 smallest FP32
+//           keys fl32(dist), the query's own frame included, in a per-lane
+//           9-ary max-heap (rounding to FP32 is monotone, so the root is
+//           fl32(eps)): the root and its 9 children in registers, the rest
+//           in LDS, (k - 9) * 256 bytes per wave (23 KB at k = 100, seven
+//           waves per CU); a sift reads 9 independent children per level.
+//           The root goes to HBM;
 //   pass 2  (te_count_kernel, four waves per block, no LDS, so occupancy
 //           is bounded by registers, not by the heap) eps and the counts
 //           together: a frame whose joint key is below
@@ -58,9 +60,11 @@ __device__ __forceinline__ void load_frame(const double* __restrict__ X,
 // nqblocks (pairs of this launch); root[64 b + lane] = fl32(eps) of its query.
 __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
 {
-    constexpr int kD = 8;   // heap arity
-    extern __shared__ float heap_lds[];   // [k + 1][64]
-    auto heap = [&](int t) -> float& { return heap_lds[t * 64 + threadIdx.x]; };
+    constexpr int kD = 9;              // heap arity
+    constexpr int kR = kTeHeapRegs;    // positions 0 .. kR - 1 (root and children) in registers
+    static_assert(kR == kD + 1, "the registers hold the root and its children");
+    extern __shared__ float heap_lds[];   // positions kR .. k: [k + 1 - kR][64]
+    auto heap = [&](int t) -> float& { return heap_lds[(t - kR) * 64 + threadIdx.x]; };
     const int lane = threadIdx.x;
     const int nqb = args.nqblocks;
     const long long p = blockIdx.x / nqb;
@@ -83,44 +87,83 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
     // included: its key 0 is the smallest, so the (k+1)-th smallest key with
     // it is the k-th smallest without it (also when other frames tie at 0),
     // and the frame loop needs no per-frame self test.  Lanes without a
-    // query fill a heap whose root is not used.
+    // query fill a heap whose root is not used.  The root and its kD
+    // children live in registers (r[0] is the root), so a sift's first level
+    // needs no LDS and the LDS part is kR slots shorter.
     const int kh = k + 1;
-    for (int t = 0; t < kh; ++t) heap(t) = INFINITY;
-    // the root lives in a register: a frame's test needs no LDS round trip
-    float top = INFINITY;
+    float r[kR];
+#pragma unroll
+    for (int t = 0; t < kR; ++t) r[t] = t < kh ? INFINITY : -INFINITY;
+    for (int t = kR; t < kh; ++t) heap(t) = INFINITY;
+    // largest of kD values: a tree of pairwise maxima, index in ix
+    auto argmax = [&](float* ch, int* ix) {
+#pragma unroll
+        for (int w = 8; w >= 1; w /= 2) {
+#pragma unroll
+            for (int c = 0; c < w; ++c) {
+                if (c + w < kD) {
+                    const bool gt = ch[c + w] > ch[c];
+                    ch[c] = gt ? ch[c + w] : ch[c];
+                    ix[c] = gt ? ix[c + w] : ix[c];
+                }
+            }
+        }
+    };
     // key below the root: replace the root and sift down, return the new root
     auto insert = [&](float key) -> float {
-        int pos = 0;
+        // level 0 -> 1 in registers (slots past kh hold -inf)
+        float ch[kD];
+        int ix[kD];
+#pragma unroll
+        for (int c = 0; c < kD; ++c) {
+            ch[c] = r[1 + c];
+            ix[c] = 1 + c;
+        }
+        argmax(ch, ix);
+        if (ch[0] <= key) {
+            r[0] = key;
+            return key;
+        }
+        r[0] = ch[0];
+        int pos = ix[0];   // 1 .. kD
+        // level 1 -> 2: the children of register slot pos are in LDS
+        const int c1 = kD * pos + 1;   // >= kR
+        bool done = c1 >= kh;
+        if (!done) {
+#pragma unroll
+            for (int c = 0; c < kD; ++c) {
+                const float h = heap(min(c1 + c, kh - 1));
+                ch[c] = c1 + c < kh ? h : -INFINITY;
+                ix[c] = c1 + c;
+            }
+            argmax(ch, ix);
+            done = ch[0] <= key;
+        }
+        // register slot pos takes the key or the larger child
+        const float up = done ? key : ch[0];
+#pragma unroll
+        for (int t = 1; t < kR; ++t) r[t] = t == pos ? up : r[t];
+        if (done) return r[0];
+        pos = ix[0];
+        // levels 2 and below in LDS
         for (;;) {
             const int c0 = kD * pos + 1;
             if (c0 >= kh) break;
-            // largest child among c0 .. min(c0 + kD, kh) - 1: a tree of
-            // pairwise maxima (log2 kD dependent steps); slots past kh are
-            // read clamped and count as -inf
-            float ch[kD];
-            int ix[kD];
 #pragma unroll
             for (int c = 0; c < kD; ++c) {
                 const float h = heap(min(c0 + c, kh - 1));
                 ch[c] = c0 + c < kh ? h : -INFINITY;
-                ix[c] = c;
+                ix[c] = c0 + c;
             }
-#pragma unroll
-            for (int w = kD / 2; w >= 1; w /= 2) {
-#pragma unroll
-                for (int c = 0; c < w; ++c) {
-                    const bool r = ch[c + w] > ch[c];
-                    ch[c] = r ? ch[c + w] : ch[c];
-                    ix[c] = r ? ix[c + w] : ix[c];
-                }
-            }
+            argmax(ch, ix);
             if (ch[0] <= key) break;
             heap(pos) = ch[0];
-            pos = c0 + ix[0];
+            pos = ix[0];
         }
         heap(pos) = key;
-        return heap(0);
+        return r[0];
     };
+    float top = INFINITY;
     // Keys below the root wait in kW registers per lane; when some lane is
     // about to run out of them, every lane inserts its waiting keys together
     // (each checked against the root again), so the wave runs the sift with
@@ -337,7 +380,9 @@ hipError_t launch_te(const TeArgs& a, hipStream_t s)
 {
     if (a.k < 1 || a.k > 128 || !a.root) return hipErrorInvalidValue;
     const long long chunk = te_chunk_pairs(a.nqblocks);
-    const size_t lds = (size_t)(a.k + 1) * 64 * sizeof(float);
+    // heap positions kTeHeapRegs .. k in LDS (one slot when all fit in registers)
+    const int slots = a.k + 1 > kTeHeapRegs ? a.k + 1 - kTeHeapRegs : 1;
+    const size_t lds = (size_t)slots * 64 * sizeof(float);
     for (long long p0 = 0; p0 < a.npairs; p0 += chunk) {
         TeArgs c = a;
         c.a1 = a.a1 + p0;

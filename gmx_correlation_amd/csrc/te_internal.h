@@ -18,6 +18,9 @@ struct TeArgs {
     float* root;          // [te_chunk_pairs(nqblocks) * nqblocks * 64] per-query fl32(eps)
 };
 
+// Heap positions per lane held in registers by the heap kernel (the root
+// and its 9 children); positions beyond them are in LDS.
+constexpr int kTeHeapRegs = 10;
 // Queries (64 per query block) per launch pair of the heap and count
 // kernels: each dispatch stays far below 2^32 work-items, and the root
 // buffer at 4 bytes per query below 2 GiB.
