@@ -23,9 +23,11 @@
 //           the root lies inside eps with all its subspace distances, and a
 //           subspace distance whose FP32 key is below (above) the root is
 //           below (above) eps.  When exactly one frame has the root key it
-//           is the k-th (eps = its FP64 distance), and when no subspace key
-//           of another frame equals the root, the counts are exact;
-//   else    (FP32 ties at the root, rare) eps as the r-th smallest FP64
+//           is the k-th (eps = its FP64 distance); subspace keys of other
+//           frames equal to the root (up to 4 frames per query, recorded in
+//           the loop) are then compared with eps in FP64;
+//   else    (several frames with the root key, or more such frames; rare)
+//           eps as the r-th smallest FP64
 //           distance among the frames whose key equals the root, r = k -
 //           #{key < root}, then the three strict counts in FP64.
 #include <hip/hip_runtime.h>
@@ -244,14 +246,21 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
 
     // ---- pass 2: eps and the strict counts from FP32 keys ----------------
     // Every frame, the self frame included (removed after the loop), adds
-    // its strict FP32 comparisons; frames with the root key are gathered
-    // under a wave-uniform branch (rarely taken) and their own terms fixed
-    // after the loop.  fl32 is monotone, so the keys are maxima of the
-    // converted 3-D distances.
-    int clt = 0, gcnt = 0, amb = 0;
+    // its strict FP32 comparisons; frames with the root key, and frames of
+    // another key with a subspace key equal to the root, are gathered under
+    // wave-uniform branches (rarely taken) and resolved in FP64 after the
+    // loop.  fl32 is monotone, so the keys are maxima of the converted 3-D
+    // distances.
+    int clt = 0, gcnt = 0;
+    int neq = 0;   // subspace keys equal to the root so far (frames to record)
     int nxfxp = 0, nypxp = 0, nxp = 0;
     double gmin = INFINITY, gmax = -INFINITY;
     double g1 = 0.0, g2 = 0.0, g3 = 0.0;   // subspace distances of a root-key frame
+    // frames of another key with a subspace key equal to the root: the first
+    // kAmb recorded per lane (na counts them all)
+    constexpr int kAmb = 4;
+    int aj[kAmb] = {0, 0, 0, 0};
+    int na = 0;
     load_frame(X, Y, 0, lag, cn);
     for (int j = 0; j < Nv; ++j) {
         double c[9];
@@ -267,7 +276,15 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
         nxfxp += k1 < root;
         nypxp += k2 < root;
         nxp += fxp < root;
-        amb += (k1 == root) + (k2 == root) + (fxp == root);
+        const int neq0 = neq;
+        neq += (k1 == root) + (k2 == root) + (fxp == root);
+        if (__ballot(neq != neq0)) {   // uniform, rare
+            if (neq != neq0 && key != root) {
+#pragma unroll
+                for (int u = 0; u < kAmb; ++u) aj[u] = na == u ? j : aj[u];
+                ++na;
+            }
+        }
         if (__ballot(key == root)) {   // uniform
             if (key == root && j != q) {
                 ++gcnt;
@@ -280,35 +297,47 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
             }
         }
     }
-    // the self frame (all distances 0): below a positive root it added one
-    // to each count; at a zero root it added three equalities
+    // the self frame (all distances 0) added one to each count below a
+    // positive root (at a zero root its key equals the root: not counted,
+    // not recorded)
     if (root > 0.0f) {
         --clt;
         --nxfxp;
         --nypxp;
         --nxp;
-    } else {
-        amb -= 3;
     }
     // a root-key frame added its subspace keys below the root to the
-    // counts and those equal to it to amb; the fast path below adds them
-    // back as FP64 comparisons with eps
+    // counts; the fast path below adds them back as FP64 comparisons with
+    // eps
     if (gcnt == 1) {
         const float r1 = (float)g1, r2 = (float)g2, r3 = (float)g3;
-        amb -= (r1 == root) + (r2 == root) + (r3 == root);
         nxfxp -= r1 < root;
         nypxp -= r2 < root;
         nxp -= r3 < root;
     }
     if (!active) {
         // no query: nothing to count
-    } else if (gcnt == 1 && amb == 0) {
+    } else if (gcnt == 1 && na <= kAmb) {
         // the one root-key frame is the k-th: eps = its distance, and its
-        // subspace distances (<= eps) count when below it
+        // subspace distances (<= eps) count when below it; so do the
+        // recorded frames' subspace distances whose keys equal the root
+        // (their keys below the root are counted already)
         const double eps = gmin;
         nxfxp += g1 < eps;
         nypxp += g2 < eps;
         nxp += g3 < eps;
+        for (int u = 0; u < kAmb; ++u) {
+            if (u < na) {
+                double c[9];
+                load_frame(X, Y, aj[u], lag, c);
+                double dxf, dxp, dyp;
+                dist9(c, dxf, dxp, dyp);
+                const double s1 = fmax(dxf, dxp), s2 = fmax(dyp, dxp);
+                nxfxp += (float)s1 == root && s1 < eps;
+                nypxp += (float)s2 == root && s2 < eps;
+                nxp += (float)dxp == root && dxp < eps;
+            }
+        }
     } else {
         // FP32 ties at the root: eps in FP64, then the counts in FP64
         const int r = k - clt;   // 1 <= r <= gcnt
