@@ -168,6 +168,7 @@ int ensure_pairs(gmxcorr_ksg_plan* pl, long long npairs)
 {
     if (npairs <= pl->pair_cap) return GMXCORR_OK;
     dfree(pl->dA1); dfree(pl->dA2); dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI);
+    pl->pair_cap = 0;   // a failed allocation below leaves no stale capacity
     HIPCHK(hipMalloc(&pl->dA1, npairs * sizeof(int)));
     HIPCHK(hipMalloc(&pl->dA2, npairs * sizeof(int)));
     HIPCHK(hipMalloc(&pl->dPartial, npairs * gmxcorr::kRowSplit * sizeof(double)));
@@ -181,6 +182,7 @@ int ensure_fix(gmxcorr_ksg_plan* pl, unsigned cap)
 {
     if (cap <= pl->fixcap) return GMXCORR_OK;
     dfree(pl->dFix);
+    pl->fixcap = 0;
     HIPCHK(hipMalloc(&pl->dFix, (size_t)cap * sizeof(FixEntry)));
     pl->fixcap = cap;
     return GMXCORR_OK;
@@ -190,6 +192,7 @@ int ensure_long(gmxcorr_ksg_plan* pl, unsigned cap)
 {
     if (cap <= pl->longcap) return GMXCORR_OK;
     dfree(pl->dLong);
+    pl->longcap = 0;
     HIPCHK(hipMalloc(&pl->dLong, (size_t)cap * sizeof(FixEntry)));
     pl->longcap = cap;
     return GMXCORR_OK;
@@ -227,7 +230,14 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
         const long long want_long = std::min<long long>(std::max<long long>(65536, rows / 25),
                                                         256LL << 20);
         int rc = ensure_fix(pl, (unsigned)want_fix);
-        if (!rc) rc = ensure_long(pl, (unsigned)want_long);
+        if (!rc) {
+            rc = ensure_long(pl, (unsigned)want_long);
+            if (rc == GMXCORR_ENOMEM && want_long > want_fix) {
+                // a plan near the HBM limit: the 1% size, overflow re-runs if needed
+                (void)hipGetLastError();
+                rc = ensure_long(pl, (unsigned)want_fix);
+            }
+        }
         if (rc) return rc;
     }
     gmxcorr::FixupArgs f{};
@@ -489,6 +499,7 @@ int gmxcorr_ksg_plan_run_pairs(gmxcorr_ksg_plan* pl, long long npairs, const int
     const long long rows = npairs * pl->N;
     if (detail && rows > pl->rc_cap) {
         dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
+        pl->rc_cap = 0;   // a failed allocation below leaves no stale capacity
         HIPCHK(hipMalloc(&pl->dRcNx, rows * sizeof(int)));
         HIPCHK(hipMalloc(&pl->dRcNy, rows * sizeof(int)));
         HIPCHK(hipMalloc(&pl->dRcFlag, rows));
