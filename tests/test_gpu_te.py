@@ -61,3 +61,18 @@ def test_te_duplicate_frames_vs_oracle(gpu, oracle_lib):
         mat = dense.te_matrix(traj, k=k, lag=lag)
         ref = oracle_lib.te_matrix(traj, k, lag)
         assert np.max(np.abs(mat - ref)) <= 1e-12, (k, lag)
+
+
+def test_te_matrix_over_two_launch_chunks(gpu, oracle_lib):
+    """2100 atoms x 100 frames: 4,407,900 ordered pairs, more than one
+    launch pair of the heap and count kernels takes (2^29 queries, 4,194,304
+    pairs at two query blocks per pair), so the matrix is computed in two
+    chunks; pairs on both sides of the chunk boundary (target atom >= 1998
+    lies in the second) equal the oracle."""
+    rng = np.random.default_rng(41)
+    c = rng.standard_normal((100, 2100, 3)).astype(np.float32)
+    traj = ksg.center(c)
+    mat = dense.te_matrix(traj, k=5, lag=1)
+    for a1, a2 in ((0, 1), (1000, 5), (1997, 2098), (1998, 300), (2050, 0), (2099, 2098)):
+        ref = oracle_lib.te_pair(traj, a1, a2, 5, 1)
+        assert abs(mat[a1, a2] - ref) <= 1e-12, (a1, a2)
