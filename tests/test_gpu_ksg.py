@@ -389,3 +389,21 @@ def test_xtc_precision_coordinates(gpu, oracle_lib, N, k):
         assert np.array_equal(d["nx"][p], ref["nx"]), p
         assert np.array_equal(d["ny"][p], ref["ny"]), p
         assert abs(mi[p] - v) <= 1e-12
+
+
+def test_all_pairs_over_two_launches(gpu, oracle_lib):
+    """1500 atoms x 64 frames: 1,124,250 pairs in 281,063 2x2 tiles times 16
+    row ranges, 4.5 M blocks, more than one dispatch carries (2^22 blocks of
+    256 work-items), so the sorted-prefix kernel goes out in two launches;
+    3000 pairs spread over the matrix equal the oracle."""
+    c = synth.generate(synth.TrajSpec(natoms=1500, nframes=64, nmodes=6, seed=64))
+    rows, psi = ksg.kraskov_prepare(ksg.center(c))
+    with ksg.KsgPlan(1500, 64, 5) as plan:
+        plan.load(rows, psi)
+        mi = plan.run_all()
+    a1, a2 = np.tril_indices(1500, -1)
+    order = np.lexsort((a2, a1))   # the reference's order: a1 = 1.., a2 < a1
+    a1, a2 = a1[order], a2[order]
+    pick = np.linspace(0, len(a1) - 1, 3000).astype(np.int64)
+    ref = oracle_lib.ksg_pairs(rows, psi, 5, a1[pick], a2[pick])
+    assert np.max(np.abs(mi[pick] - ref)) <= 1e-12
