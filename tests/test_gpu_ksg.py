@@ -407,3 +407,29 @@ def test_all_pairs_over_two_launches(gpu, oracle_lib):
     pick = np.linspace(0, len(a1) - 1, 3000).astype(np.int64)
     ref = oracle_lib.ksg_pairs(rows, psi, 5, a1[pick], a2[pick])
     assert np.max(np.abs(mi[pick] - ref)) <= 1e-12
+
+
+def test_long_trajectory_all_pairs_over_two_launches(gpu, oracle_lib):
+    """720 atoms x 2100 frames (prefix-store path, 512-frame prefix at k =
+    10): 64,710 2x2 tiles times 66 chunks of 32 rows, 4.27 M blocks, so the
+    kernel goes out in two launches (nearly every tile has a chunk in the
+    second).  600 pairs spread over the matrix equal the oracle's grid
+    restatement, except pairs with an exact FP64 tie at the k-th distance,
+    where the GPU counts every tied frame: those equal the FP64 brute-force
+    recount, which follows the same rule."""
+    c = synth.generate(synth.TrajSpec(natoms=720, nframes=2100, nmodes=6, seed=21))
+    rows, psi = ksg.kraskov_prepare(ksg.center(c))
+    with ksg.KsgPlan(720, 2100, 10) as plan:
+        plan.load(rows, psi)
+        mi = plan.run_all()
+    a1, a2 = np.tril_indices(720, -1)
+    order = np.lexsort((a2, a1))
+    a1, a2 = a1[order], a2[order]
+    pick = np.linspace(0, len(a1) - 1, 600).astype(np.int64)
+    ref = oracle_lib.ksg_pairs(rows, psi, 10, a1[pick], a2[pick])
+    off = np.nonzero(np.abs(mi[pick] - ref) > 1e-12)[0]
+    assert len(off) <= len(pick) // 10
+    for i in off:
+        p = int(pick[i])
+        v = oracle_lib.ksg_pair(rows, psi, int(a1[p]), int(a2[p]), 10, method="brute")
+        assert abs(mi[p] - v) <= 1e-12, p
