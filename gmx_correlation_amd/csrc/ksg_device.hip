@@ -865,7 +865,8 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
                                               const RowOut& o)
 {
     // R <= kPrefix holds for every caller (N <= kSortedMaxFrames rows split
-    // in at least kRowSplit = 4 parts); the side bits cover kPrefix rows.
+    // in kRowSplit ranges, or stepped by 4 in the batch kernel); the side
+    // bits cover kPrefix rows.
     const int R = min((i1 - i0 + step - 1) / step, kPrefix);
     const int nblk = (int)(pr.P / 256);   // 64-lane chunks of 4 frames in a row
     uint32_t* hist = lds.hist;
