@@ -39,6 +39,14 @@ class KsgStats(ctypes.Structure):
         ("k", _i),
         ("natoms", _i),
         ("pitch", _i),
+        ("kind", _i),
+        ("s4", _i),
+        ("nsplit", _i),
+        ("chunk", _i),
+        ("slots", _i),
+        ("wave_atoms", _i),
+        ("waves", _i),
+        ("all_pairs", _i),
     ]
 
     def as_dict(self) -> dict:
@@ -91,7 +99,8 @@ C_SYMBOLS = [
     "gmxcorr_ksg_plan_create", "gmxcorr_ksg_plan_load", "gmxcorr_ksg_plan_run_all",
     "gmxcorr_ksg_plan_run_pairs", "gmxcorr_ksg_plan_result", "gmxcorr_ksg_plan_matrix",
     "gmxcorr_ksg_plan_stats", "gmxcorr_ksg_plan_destroy", "gmxcorr_ksg_plan_set_variant",
-    "gmxcorr_ksg_variant_fits",
+    "gmxcorr_ksg_variant_fits", "gmxcorr_ksg_plan_layout", "gmxcorr_ksg_plan_create_budget",
+    "gmxcorr_ksg_plan_set_shape", "gmxcorr_ksg_pairs_device",
     "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix",
     "gmxcorr_te_matrix", "gmxcorr_te_pairs",
     "gmxcorr_ksg_batch_create", "gmxcorr_ksg_batch_add", "gmxcorr_ksg_batch_run",
@@ -129,6 +138,12 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_ksg_plan_destroy.argtypes = [ctypes.c_void_p]
     L.gmxcorr_ksg_plan_destroy.restype = None
     L.gmxcorr_ksg_plan_set_variant.argtypes = [ctypes.c_void_p, _i, _i]
+    L.gmxcorr_ksg_plan_layout.argtypes = [_i, _i, _i, _ll, _ip, _ip, _ip]
+    L.gmxcorr_ksg_plan_create_budget.argtypes = [_i, _i, _i, _ll, ctypes.POINTER(ctypes.c_void_p)]
+    L.gmxcorr_ksg_plan_set_shape.argtypes = [ctypes.c_void_p, _i, _i]
+    L.gmxcorr_ksg_pairs_device.argtypes = [ctypes.c_void_p, _i, _i, ctypes.c_void_p, _i, _ll,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p]
     L.gmxcorr_ksg_variant_fits.argtypes = [_i, _i, _i, _i]
     L.gmxcorr_gauss_matrix.argtypes = [_dp, _i, _i, _dp, _dp, ctypes.POINTER(DenseStats)]
     L.gmxcorr_gcmi_matrix.argtypes = [_dp, _i, _i, _dp, _dp, _dp, ctypes.POINTER(DenseStats)]

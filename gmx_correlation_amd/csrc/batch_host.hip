@@ -32,6 +32,7 @@ namespace {
 
 struct HostTraj {
     int natoms = 0, N = 0, K = 0, kind = -1, s4 = 0;
+    int chunk = gmxcorr::kLongChunk;   // frame rows per kind-3 work item
     long long P = 0;
     std::vector<double> rows;   // [3n][N]
     std::vector<double> psi;    // [N+1]
@@ -260,7 +261,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                 if (h.stored())
                     HIPCHK(gmxcorr::launch_build_prefix(dD[q], h.natoms, h.N, h.P, dLV, dLI, dLS, s));
                 desc[q] = gmxcorr::TrajDesc{dX[q], dD[q], dSD, dSM, dSI, dLV, dLI, dLS, dPsi[q],
-                                            h.P, h.N, h.K, h.psiN_phiK, h.kind, h.s4};
+                                            h.P, h.N, h.K, h.psiN_phiK, h.kind, h.s4, h.chunk};
             }
             // pairs a1 > a2 of each trajectory, in the work order
             std::vector<int> tq(ts.size());
@@ -291,8 +292,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
                     qoff[q] = (long long)hq.size();
                     hq.insert(hq.end(), t4.begin(), t4.end());
                     desc[q].nquads = (long long)t4.size() / 4;
-                    desc[q].nitems = desc[q].nquads *
-                                     ((h.N + gmxcorr::kLongChunk - 1) / gmxcorr::kLongChunk);
+                    desc[q].nitems = desc[q].nquads * ((h.N + h.chunk - 1) / h.chunk);
                 } else {
                     desc[q].nquads = 0;
                     desc[q].nitems = h.npairs();

@@ -865,7 +865,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
                                               const RowOut& o)
 {
     // R <= kPrefix holds for every caller (N <= kSortedMaxFrames rows split
-    // in kRowSplit ranges, or stepped by 4 in the batch kernel); the side
+    // in nsplit ranges, or stepped by 4 in the batch kernel); the side
     // bits cover kPrefix rows.
     const int R = min((i1 - i0 + step - 1) / step, kPrefix);
     const int nblk = (int)(pr.P / 256);   // 64-lane chunks of 4 frames in a row
@@ -1594,19 +1594,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // read by two waves at about the same time), or pair 4q+w (consecutive
     // pairs share their first atom).
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
-    const long long nitems = nquads * kRowSplit;
+    const int nsplit = args.nsplit;
+    const long long nitems = nquads * nsplit;
     // virtual block: the launcher splits a grid above 2^22 blocks into
     // launches at offsets vb0 (a multiple of 8)
     const long long vb = args.vb0 + blockIdx.x;
     const long long item = xcd_item(vb, nitems);
     if (item >= nitems) return;
-    const long long q = item / kRowSplit;
+    const long long q = item / nsplit;
     const long long p = args.quads ? (long long)args.quads[q * kPairWaves + w] : q * kPairWaves + w;
-    const int part = (int)(item % kRowSplit);
+    const int part = (int)(item % nsplit);
     if (p < 0 || p >= args.npairs) return;   // no block barriers below
     const int N = args.N, T = args.K + 1;
-    const int i0 = (int)((long long)N * part / kRowSplit);
-    const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
+    const int i0 = (int)((long long)N * part / nsplit);
+    const int i1 = (int)((long long)N * (part + 1) / nsplit);
     const float* Da = args.D + (size_t)args.a1[p] * N * args.P;
     const float* Db = args.D + (size_t)args.a2[p] * N * args.P;
     const RowOut o = row_out(args);
@@ -1617,7 +1618,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     sr.SMa = args.SM + (size_t)args.a1[p] * N * kSumPitch;
     sr.SMb = args.SM + (size_t)args.a2[p] * N * kSumPitch;
     const double acc = rows_single<S4>(Da, Db, sr, args.P, N, T, i0, i1, 1, p, lds[w], lane, o);
-    if (lane == 0) args.partial[p * kRowSplit + part] = acc;
+    if (lane == 0) args.partial[p * nsplit + part] = acc;
 }
 
 __device__ __forceinline__ PrefixRows prefix_rows(const float* D, const float* SD, const float* SM,
@@ -1647,7 +1648,8 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
-    const long long nitems = nquads * kRowSplit;
+    const int nsplit = args.nsplit;
+    const long long nitems = nquads * nsplit;
     // virtual block: the launcher splits a grid above 2^22 blocks into
     // launches at offsets vb0 (a multiple of 8)
     const long long vb = args.vb0 + blockIdx.x;
@@ -1660,8 +1662,8 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
     const int part = (int)(item / nquads);
     if (p < 0 || p >= args.npairs) return;   // no block barriers below
     const int N = args.N, T = args.K + 1;
-    const int i0 = (int)((long long)N * part / kRowSplit);
-    const int i1 = (int)((long long)N * (part + 1) / kRowSplit);
+    const int i0 = (int)((long long)N * part / nsplit);
+    const int i1 = (int)((long long)N * (part + 1) / nsplit);
     const PrefixRows pr = prefix_rows(args.D, args.SD, args.SM, args.SI, args.P, N, args.a1[p],
                                       args.a2[p]);
     RowOut o = row_out(args);
@@ -1671,15 +1673,15 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
         o.rc_flag = nullptr;
     }
     const double acc = rows_prefix(pr, N, T, i0, i1, 1, p, lds[w], lane, o);
-    if (lane == 0) args.partial[p * kRowSplit + part] = acc;
+    if (lane == 0) args.partial[p * nsplit + part] = acc;
 }
 
 // Long-trajectory prefix kernel (kind 3): four pairs per block (a 2x2 atom
 // tile, as ksg_pair_prefix_kernel), 1 KB of LDS per wave.  C: 256-frame
 // chunks of the prefix read per row (2: 512 frames, 3: 768, 4: the whole
-// 1024-frame store).  A block runs one chunk of kLongChunk frame rows, and blocks are
+// 1024-frame store).  A block runs one chunk of args.chunk frame rows, and blocks are
 // ordered chunk-major: the blocks in flight on one XCD run consecutive tiles,
-// which share their first two atoms, over the same kLongChunk rows, so those
+// which share their first two atoms, over the same args.chunk rows, so those
 // atoms' distance rows and prefixes are read from that XCD's L2.  Each
 // wave's sum over its rows is added to fixsum[pair].
 template <int C, bool Detail>
@@ -1689,7 +1691,7 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
-    const int nchunk = (args.N + kLongChunk - 1) / kLongChunk;
+    const int nchunk = (args.N + args.chunk - 1) / args.chunk;
     const long long nitems = nquads * nchunk;
     // virtual block: the launcher splits a grid above 2^22 blocks into
     // launches at offsets vb0 (a multiple of 8)
@@ -1701,8 +1703,8 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
     const int chunk = (int)(item / nquads);
     if (p < 0 || p >= args.npairs) return;   // no block barriers below
     const int N = args.N, T = args.K + 1;
-    const int i0 = chunk * kLongChunk;
-    const int i1 = min(N, i0 + kLongChunk);
+    const int i0 = chunk * args.chunk;
+    const int i1 = min(N, i0 + args.chunk);
     const LongRows lr = long_rows(args.D, args.LV, args.LI, args.LS, args.P, N, args.a1[p],
                                   args.a2[p]);
     RowOut o = row_out(args);
@@ -1776,7 +1778,7 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
     const RowOut o = row_out(args);
     double acc = 0.0;
     for (int i = 0; i < N; ++i) acc += row_coop<S4>(Da, Db, args.P, N, T, i, p, sh, lane, w, o);
-    if (threadIdx.x < kRowSplit) args.partial[p * kRowSplit + threadIdx.x] = threadIdx.x == 0 ? acc : 0.0;
+    if (threadIdx.x == 0) args.partial[p] = acc;   // one slot per pair
 }
 
 // ---------------------------------------------------------------------------
@@ -1821,7 +1823,7 @@ __device__ __forceinline__ double batch_prefix(const TrajDesc& d, int a1, int a2
 }
 
 // Kind 3 in the batched kernel: one work item is a 2x2 atom tile of the
-// trajectory times one chunk of kLongChunk frame rows (chunk-major, as
+// trajectory times one chunk of d.chunk frame rows (chunk-major, as
 // ksg_pair_long_kernel); wave w runs the tile's pair w over the chunk and
 // adds its row sum to fixsum[pair].
 __device__ __forceinline__ void batch_long(const TrajDesc& d, const KsgBatchArgs& b,
@@ -1833,8 +1835,8 @@ __device__ __forceinline__ void batch_long(const TrajDesc& d, const KsgBatchArgs
     if (p < 0) return;   // no block barriers in this path
     const long long gp = d.pair0 + p;
     const int a1 = uniform_copy(b.a1[gp]), a2 = uniform_copy(b.a2[gp]);
-    const int i0 = (int)chunk * kLongChunk;
-    const int i1 = min(d.N, i0 + kLongChunk);
+    const int i0 = (int)chunk * d.chunk;
+    const int i1 = min(d.N, i0 + d.chunk);
     const LongRows lr = long_rows(d.D, d.LV, d.LI, d.LS, d.P, d.N, a1, a2);
     const double acc = d.s4 == 2   ? rows_long<2>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o)
                        : d.s4 == 3 ? rows_long<3>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o)
@@ -2011,28 +2013,27 @@ __global__ __launch_bounds__(256) void ksg_batch_long_rows_kernel(KsgBatchArgs b
 }
 
 // ---------------------------------------------------------------------------
-// FP64 recount of queued rows: one wave per entry.  Recomputes dA, dB in
-// FP64 from the coordinates.  If the pair kernel left the key unresolved it
-// is found first by a three-digit radix select over the same FP32 keys.  The
-// key fixes the group G of frames whose joint distance rounds to it; the
-// k-th neighbour distance is the r-th smallest FP64 value in G with
-// r = k - #{j != i : key < eps}.  More than r frames at that FP64 value is
-// an exact tie: all of them enter the radius set (the tie is counted).
+// FP64 KSG terms of one frame row i of one atom pair, computed from the
+// coordinates by one wave (no distance rows; any N).  dA, dB are recomputed
+// in FP64 on every pass.  ek: FP32 bits of the (k+1)-th smallest joint key
+// (self included) when a pair kernel has found it, or kUnknownEps, in which
+// case a three-digit radix select over the same FP32 keys finds it first
+// (hist: the wave's 2048 LDS words).  The key fixes the group G of frames
+// whose joint distance rounds to it; the k-th neighbour distance is the r-th
+// smallest FP64 value in G with r = k - #{j != i : key < ek}.  More than r
+// frames at that FP64 value is an exact tie: all of them enter the radius
+// set (the reference CUDA kernel's rule, lib/kraskov_gpu.cu:160-171).  When
+// the select finds exactly r frames of G besides i, no FP64 walk is needed:
+// the set is {j != i : key <= ek}.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned nent)
+struct Fp64Row {
+    int nx, ny;
+    bool tie;
+};
+
+__device__ Fp64Row fp64_row(const double* __restrict__ xa, const double* __restrict__ xb, int N,
+                            int K, int i, uint32_t ek, uint32_t* hist, int lane)
 {
-    __shared__ uint32_t lds[4][kBins];
-    const int lane = threadIdx.x & (kWave - 1);
-    uint32_t* hist = lds[threadIdx.x / kWave];
-    for (unsigned e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; e < nent;
-         e += gridDim.x * (blockDim.x / kWave)) {
-    const FixEntry fe = args.fixq[e];
-    const long long p = fe.pair;
-    const int i = fe.row;
-    const TrajDesc d = args.traj[args.item_traj ? args.item_traj[p] : 0];
-    const int N = d.N, K = d.K;
-    const double* xa = d.X + (size_t)(3 * args.a1[p]) * N;
-    const double* xb = d.X + (size_t)(3 * args.a2[p]) * N;
     const double qa0 = xa[i], qa1 = xa[N + i], qa2 = xa[2 * N + i];
     const double qb0 = xb[i], qb1 = xb[N + i], qb2 = xb[2 * N + i];
     auto dist = [&](int j, double& A, double& B) {
@@ -2045,7 +2046,8 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned
         return __float_as_uint((float)fmax(A, B));
     };
 
-    uint32_t ek = fe.eps_bits;
+    int r;              // rank of the k-th distance inside G (G without i)
+    bool whole = false; // G (without i) holds exactly r frames
     if (ek == kUnknownEps) {
         // (k+1)-th smallest key, self included: digits 30..20, 19..10, 9..0
         clear_hist<2048>(hist, lane);
@@ -2075,53 +2077,64 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned
         wave_sync();
         find_bin<16>(hist, lane, rank, bin, rank, pop);
         ek = (prefix << 10) | (uint32_t)bin;
+        wave_sync();   // hist is cleared again by the wave's next row
+        // K + 1 - rank keys lie below ek; i's own key 0 is among them unless
+        // ek == 0, when i is in G itself
+        r = ek ? rank : rank - 1;
+        whole = pop == rank;
+    } else {
+        int clt = 0;
+        for (int j = lane; j < N; j += kWave) {
+            if (j == i) continue;
+            clt += key_of(j) < ek;
+        }
+        clt = wave_sum(clt);
+        r = K - clt;  // >= 1
     }
 
-    int clt = 0;
-    for (int j = lane; j < N; j += kWave) {
-        if (j == i) continue;
-        clt += key_of(j) < ek;
-    }
-    clt = wave_sum(clt);
-    const int r = K - clt;  // >= 1
-
-    // walk the distinct FP64 values of G upwards until r are covered
-    double prev = -1.0, eps64 = 0.0;
-    int cum = 0;
+    double eps64 = 0.0;
     bool tie = false;
-    for (;;) {
-        double vmin = INFINITY;
-        for (int j = lane; j < N; j += kWave) {
-            if (j == i) continue;
-            double A, B;
-            dist(j, A, B);
-            const double J = fmax(A, B);
-            if (__float_as_uint((float)J) == ek && J > prev) vmin = fmin(vmin, J);
+    if (!whole) {
+        // walk the distinct FP64 values of G upwards until r are covered
+        double prev = -1.0;
+        int cum = 0;
+        for (;;) {
+            double vmin = INFINITY;
+            for (int j = lane; j < N; j += kWave) {
+                if (j == i) continue;
+                double A, B;
+                dist(j, A, B);
+                const double J = fmax(A, B);
+                if (__float_as_uint((float)J) == ek && J > prev) vmin = fmin(vmin, J);
+            }
+            vmin = wave_min_d(vmin);
+            int m = 0;
+            for (int j = lane; j < N; j += kWave) {
+                if (j == i) continue;
+                double A, B;
+                dist(j, A, B);
+                m += fmax(A, B) == vmin;
+            }
+            m = wave_sum(m);
+            if (m == 0 || cum + m >= r) {  // m == 0 cannot happen for a consistent entry
+                eps64 = vmin;
+                tie = cum + m > r;
+                break;
+            }
+            cum += m;
+            prev = vmin;
         }
-        vmin = wave_min_d(vmin);
-        int m = 0;
-        for (int j = lane; j < N; j += kWave) {
-            if (j == i) continue;
-            double A, B;
-            dist(j, A, B);
-            m += fmax(A, B) == vmin;
-        }
-        m = wave_sum(m);
-        if (m == 0 || cum + m >= r) {  // m == 0 cannot happen for a consistent entry
-            eps64 = vmin;
-            tie = cum + m > r;
-            break;
-        }
-        cum += m;
-        prev = vmin;
     }
 
+    // radii: the largest marginal distances over the set.  With `whole` the
+    // set is {key <= ek}, which equals {J <= max_G J} as rounding is monotone.
     double ex = 0.0, ey = 0.0;
     for (int j = lane; j < N; j += kWave) {
         if (j == i) continue;
         double A, B;
         dist(j, A, B);
-        if (fmax(A, B) <= eps64) {
+        const bool in = whole ? __float_as_uint((float)fmax(A, B)) <= ek : fmax(A, B) <= eps64;
+        if (in) {
             ex = fmax(ex, A);
             ey = fmax(ey, B);
         }
@@ -2136,21 +2149,91 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned
         nx += A <= ex;
         ny += B <= ey;
     }
-    nx = wave_sum(nx);
-    ny = wave_sum(ny);
-    if (lane == 0) {
-        atomicAdd(&args.fixsum[p], d.psi[nx] + d.psi[ny]);
-        if (tie) atomicAdd(&args.counters[1], 1u);
-        if (args.rc_nx) {
-            const size_t o = (size_t)p * N + i;
-            args.rc_nx[o] = nx;
-            args.rc_ny[o] = ny;
-            args.rc_flag[o] = tie ? 2 : 1;
+    Fp64Row out;
+    out.nx = wave_sum(nx);
+    out.ny = wave_sum(ny);
+    out.tie = tie;
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 recount of queued rows: one wave per entry runs fp64_row with the
+// key the pair kernel found (or kUnknownEps), its psi terms added to
+// fixsum[pair].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned nent)
+{
+    __shared__ uint32_t lds[4][kBins];
+    const int lane = threadIdx.x & (kWave - 1);
+    uint32_t* hist = lds[threadIdx.x / kWave];
+    for (unsigned e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; e < nent;
+         e += gridDim.x * (blockDim.x / kWave)) {
+        const FixEntry fe = args.fixq[e];
+        const long long p = fe.pair;
+        const int i = fe.row;
+        const TrajDesc d = args.traj[args.item_traj ? args.item_traj[p] : 0];
+        const int N = d.N;
+        const double* xa = d.X + (size_t)(3 * args.a1[p]) * N;
+        const double* xb = d.X + (size_t)(3 * args.a2[p]) * N;
+        const Fp64Row o = fp64_row(xa, xb, N, d.K, i, fe.eps_bits, hist, lane);
+        if (lane == 0) {
+            atomicAdd(&args.fixsum[p], d.psi[o.nx] + d.psi[o.ny]);
+            if (o.tie) atomicAdd(&args.counters[1], 1u);
+            if (args.rc_nx) {
+                const size_t off = (size_t)p * N + i;
+                args.rc_nx[off] = o.nx;
+                args.rc_ny[off] = o.ny;
+                args.rc_flag[off] = o.tie ? 2 : 1;
+            }
         }
-    }
     }
 }
 
+// ---------------------------------------------------------------------------
+// Coordinate path (kind 4): every frame row of every pair by fp64_row from
+// the FP64 coordinates.  No distance rows, so device memory is O(natoms * N)
+// as in the reference's kernel (lib/kraskov_gpu.cu:107-196), and there is no
+// frame limit.  Work item = (pair p, row range r of nsplit); block = four
+// waves, wave w taking rows i0 + w, i0 + w + 4, ... of the range.  The four
+// wave sums are added in wave order into partial[p * nsplit + r], so results
+// are reproducible bit for bit.  Items are pair-major: the blocks in flight
+// work on few pairs, whose six coordinate rows stay in L2.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ksg_coord_kernel(KsgArgs args)
+{
+    __shared__ uint32_t lds[kPairWaves][kBins];
+    __shared__ double wsum[kPairWaves];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int N = args.N, nsplit = args.nsplit;
+    const long long nitems = args.npairs * nsplit;
+    for (long long item = blockIdx.x; item < nitems; item += gridDim.x) {   // uniform
+        const long long p = item / nsplit;
+        const int part = (int)(item - p * nsplit);
+        const int i0 = (int)((long long)N * part / nsplit);
+        const int i1 = (int)((long long)N * (part + 1) / nsplit);
+        const double* xa = args.X + (size_t)(3 * args.a1[p]) * N;
+        const double* xb = args.X + (size_t)(3 * args.a2[p]) * N;
+        double acc = 0.0;
+        for (int i = i0 + w; i < i1; i += kPairWaves) {
+            const Fp64Row o = fp64_row(xa, xb, N, args.K, i, kUnknownEps, lds[w], lane);
+            acc += args.psi[o.nx] + args.psi[o.ny];
+            if (lane == 0) {
+                if (o.tie) atomicAdd(&args.counters[1], 1u);
+                if (args.rc_nx) {
+                    const size_t off = (size_t)p * N + i;
+                    args.rc_nx[off] = o.nx;
+                    args.rc_ny[off] = o.ny;
+                    args.rc_flag[off] = o.tie ? 2 : 1;
+                }
+            }
+        }
+        if (lane == 0) wsum[w] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) args.partial[item] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+        __syncthreads();
+    }
+}
 __global__ void ksg_clear_kernel(double* fixsum, long long npairs, unsigned* counters,
                                  unsigned long long* queue, double* partial, int nqueue)
 {
@@ -2171,6 +2254,19 @@ __global__ void ksg_finalize_kernel(FixupArgs args, long long nitems, const doub
     double sum = 0.0;
     for (int r = 0; r < nsplit; ++r) sum += partial[p * nsplit + r];
     mi[p] = d.psiN_phiK - (sum + args.fixsum[p]) / d.N;
+}
+
+// MI of the device-pointer coordinate path: psi[N] + (psi[K] - 1/K) - sum/N
+// (lib/kraskov.cpp:1318) with psi read on the device.
+__global__ void ksg_finalize_coord_kernel(const double* __restrict__ partial, int nsplit,
+                                          const double* __restrict__ psi, int N, int K,
+                                          long long npairs, double* __restrict__ mi)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    double sum = 0.0;
+    for (int r = 0; r < nsplit; ++r) sum += partial[p * nsplit + r];
+    mi[p] = (psi[N] + (psi[K] - 1 / (1.0 * K))) - sum / N;
 }
 
 // Wave primitive self-check: out[0..63] inclusive scan of lane+1 (mod 7
@@ -2238,9 +2334,11 @@ hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
 }
 
 // Kernel variants: kind 0 = one wave per row (N <= 256*S4), kind 1 = four
-// waves per row (N <= 1024*S4).  default_variant picks the smallest that fits.
+// waves per row (N <= 1024*S4), kinds 2 and 3 the prefix paths, kind 4 the
+// coordinate path.  default_variant picks the smallest that fits.
 bool variant_fits(int kind, int s4, int N, int K)
 {
+    if (kind == 4) return K >= 1 && N > K;
     if (kind == 2) return N <= kSortedMaxFrames && K + 1 <= kPrefixMaxT;
     if (kind == 3)
         return s4 >= 2 && s4 <= 4 && N > kSortedMaxFrames && N <= kMaxFrames &&
@@ -2254,6 +2352,7 @@ bool variant_fits(int kind, int s4, int N, int K)
 
 void default_variant(int N, int K, int& kind, int& s4)
 {
+    if (N > kMaxFrames) { kind = 4; s4 = 1; return; }
     if (variant_fits(2, 1, N, K)) { kind = 2; s4 = 1; return; }
     if (variant_fits(3, 2, N, K)) { kind = 3; s4 = long_prefix_chunks(N, K); return; }
     static const int single[] = {1, 2, 3, 4, 5, 6, 8};
@@ -2262,6 +2361,11 @@ void default_variant(int N, int K, int& kind, int& s4)
     kind = 1;
     s4 = (N + 1023) / 1024;
     if (s4 < 2) s4 = 2;
+}
+
+int variant_slots(int kind, int nsplit)
+{
+    return kind == 3 ? 0 : kind == 1 ? 1 : nsplit;
 }
 
 hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4, int* nsplit)
@@ -2277,12 +2381,21 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4, i
     if (!variant_fits(kind, s4, a.N, a.K)) return hipErrorInvalidValue;
     if (variant_sorted(kind) && !(a.SD && a.SM && a.SI)) return hipErrorInvalidValue;
     if (variant_prefix_store(kind) && !(a.LV && a.LI && a.LS)) return hipErrorInvalidValue;
+    if (kind != 4 && !a.D) return hipErrorInvalidValue;
+    if (a.nsplit < 1 || a.nsplit > kMaxRowSplit || a.chunk < 1) return hipErrorInvalidValue;
+    // partial sums per pair: nsplit slots, one (kind 1), or none (kind 3
+    // adds into fixsum)
+    if (nsplit) *nsplit = variant_slots(kind, a.nsplit);
+    if (kind == 4) {
+        const long long items = a.npairs * a.nsplit;
+        hipLaunchKernelGGL(ksg_coord_kernel, dim3(capped_grid(items)), dim3(kWave * kPairWaves),
+                           0, s, a);
+        return hipGetLastError();
+    }
     const long long nquads = a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves;
     const long long nitems = kind == 1   ? a.npairs
-                             : kind == 3 ? nquads * ((a.N + kLongChunk - 1) / kLongChunk)
-                                         : nquads * kRowSplit;
-    // partial sums per pair: kRowSplit slots, or none (kind 3 adds into fixsum)
-    if (nsplit) *nsplit = kind == 3 ? 0 : kRowSplit;
+                             : kind == 3 ? nquads * ((a.N + a.chunk - 1) / a.chunk)
+                                         : nquads * a.nsplit;
     // one block per virtual block; a dispatch carries fewer than 2^32
     // work-items, so grids above 2^22 blocks go out in several launches
     const long long nvb = (nitems + 7) / 8 * 8;
@@ -2330,6 +2443,34 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4, i
     if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_ksg_coord_pairs(const double* X, int N, const double* psi, int K,
+                                 long long npairs, const int* a1, const int* a2, int nsplit,
+                                 double* partial, unsigned* counters, double* mi, hipStream_t s)
+{
+    if (!X || !psi || !a1 || !a2 || !partial || !counters || !mi || npairs < 1 || K < 1 ||
+        N <= K || nsplit < 1 || nsplit > kMaxRowSplit)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ksg_clear_kernel, dim3(1), dim3(256), 0, s, nullptr, 0LL, counters,
+                       nullptr, nullptr, 0);
+    KsgArgs a{};
+    a.X = X;
+    a.psi = psi;
+    a.a1 = a1;
+    a.a2 = a2;
+    a.partial = partial;
+    a.counters = counters;
+    a.npairs = npairs;
+    a.N = N;
+    a.K = K;
+    a.nsplit = nsplit;
+    a.chunk = kLongChunk;
+    hipLaunchKernelGGL(ksg_coord_kernel, dim3(capped_grid(npairs * nsplit)),
+                       dim3(kWave * kPairWaves), 0, s, a);
+    hipLaunchKernelGGL(ksg_finalize_coord_kernel, dim3((unsigned)((npairs + 255) / 256)),
+                       dim3(256), 0, s, partial, nsplit, psi, N, K, npairs, mi);
+    return hipGetLastError();
 }
 
 hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s)

@@ -1,6 +1,19 @@
 // Host side of libgmxcorr_hip.so: device plan for the KSG matrix, the
 // extern "C" ABI (include/gmxcorr.h) and the reference's C++ GPU boundary
 // (include/kraskov_gpu.h, replacing lib/kraskov_gpu.cu:210-308).
+//
+// A plan holds the FP64 coordinates of all atoms and, when the path needs
+// them, the FP32 distance rows of some atoms ("slots"):
+//   resident     slots = natoms: every atom's rows built once at load;
+//   memory waves slots = 2B < natoms: the atoms are cut into blocks of B,
+//                and for each pair of blocks (I >= J) block I's rows sit in
+//                slots [0, B) and block J's in [B, 2B) while the pairs
+//                between them run (rows rebuilt per wave);
+//   coordinates  slots = 0: kind 4 only, every row in FP64 from the
+//                coordinates (frame counts above kMaxFrames, or rows of even
+//                two atoms larger than the row budget).
+// The row budget is the free device memory less what the pair buffers and
+// recount queues need, or the caller's max_row_bytes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -84,6 +97,15 @@ std::vector<int> pair_quads(int n)
     return q;
 }
 
+// Row ranges per pair for the coordinate path: enough work items to fill the
+// device (about 8192 blocks of four waves), at least 16 rows per range.
+int coord_split(long long npairs, int N)
+{
+    long long r = (8192 + npairs - 1) / npairs;
+    r = std::min<long long>(r, std::max(1, N / 16));
+    return (int)std::max<long long>(1, std::min<long long>(r, kMaxRowSplit));
+}
+
 }  // namespace gmxcorr
 
 namespace {
@@ -120,6 +142,29 @@ void dfree(T*& p)
     p = nullptr;
 }
 
+// Bytes of one atom's distance rows and of its sorted rows / prefix stores.
+long long d_atom_bytes(int N) { return (long long)N * gmxcorr::row_pitch(N) * 4; }
+long long sorted_atom_bytes(int N)
+{
+    if (N > gmxcorr::kSortedMaxFrames) return 0;
+    return (long long)N * (gmxcorr::sorted_pitch(N) * 4 + gmxcorr::kSumPitch * 4 +
+                           gmxcorr::kPrefix * 2);
+}
+long long store_atom_bytes(int N)
+{
+    return (long long)N * (gmxcorr::kLongPitch * 4 + gmxcorr::kLongPrefix * 2 +
+                           gmxcorr::kLongSum * 4);
+}
+
+// Default row ranges per pair of kinds 0 and 2: kRowSplit, fewer when the
+// partial sums (8 bytes per range and pair) would pass 8 GB.
+int default_split(long long npairs)
+{
+    const long long cap = (8LL << 30) / 8;
+    const long long r = std::max<long long>(1, cap / std::max<long long>(1, npairs));
+    return (int)std::min<long long>(gmxcorr::kRowSplit, r);
+}
+
 }  // namespace
 
 struct gmxcorr_ksg_plan {
@@ -127,8 +172,13 @@ struct gmxcorr_ksg_plan {
     long long P = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
-    double* dX = nullptr;
+    double* dX = nullptr;    // [3n][N] all atoms
+    double* dXs = nullptr;   // [3 slots][N] coordinates of the atoms in the row slots (== dX when resident)
     double* dPsi = nullptr;
+    // row storage for `slots` atoms
+    int slots = 0;           // natoms (resident), 2 * wave_atoms (waves) or 0 (coordinates only)
+    int wave_atoms = 0;      // B: atoms per block in memory waves (0: resident or coordinates only)
+    long long row_budget = 0;
     float* dD = nullptr;
     int* dQuads = nullptr;  // all-pairs run: [nquads][4] pair indices (2x2 atom tiles)
     long long nquads = 0;
@@ -138,15 +188,17 @@ struct gmxcorr_ksg_plan {
     float* dLV = nullptr;   // prefix stores (kSortedMaxFrames < N <= kMaxFrames, k + 1 <= 256)
     uint16_t* dLI = nullptr;
     float* dLS = nullptr;
+    int block_in_slot[2] = {-1, -1};   // which block's rows the two wave slots hold
     int* dA1 = nullptr;
     int* dA2 = nullptr;
     long long pair_cap = 0;
     double* dPartial = nullptr;
+    long long partial_cap = 0;   // doubles
     double* dFixsum = nullptr;
     double* dMI = nullptr;
     FixEntry* dFix = nullptr;
     unsigned fixcap = 0;
-    FixEntry* dLong = nullptr;   // rows beyond the sorted prefix (kind 2)
+    FixEntry* dLong = nullptr;   // rows beyond the sorted prefix (kinds 2 and 3)
     unsigned longcap = 0;
     unsigned* dCounters = nullptr;
     int* dRcNx = nullptr;
@@ -157,24 +209,59 @@ struct gmxcorr_ksg_plan {
     bool loaded = false;
     bool all_pairs = false;
     long long last_npairs = 0;
+    std::vector<double> host_mi;   // results of the last run when it ran in memory waves
+    bool host_result = false;
     int var_kind = -1, var_s4 = 0;
+    int nsplit = 0;          // row ranges per pair (0: default for the pair count)
+    int chunk = gmxcorr::kLongChunk;
     gmxcorr::TrajDesc* dTraj = nullptr;   // one descriptor for the recount/finalise kernels
     gmxcorr_ksg_stats st{};
 };
 
 namespace {
 
-int ensure_pairs(gmxcorr_ksg_plan* pl, long long npairs)
+// Code path of a run: the caller's variant, else the default for N and k,
+// with kind 3 replaced by the cooperative path when no prefix stores were
+// built, and kind 4 whenever the plan holds no rows.
+void plan_kind(const gmxcorr_ksg_plan* pl, int& kind, int& s4)
 {
-    if (npairs <= pl->pair_cap) return GMXCORR_OK;
-    dfree(pl->dA1); dfree(pl->dA2); dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI);
-    pl->pair_cap = 0;   // a failed allocation below leaves no stale capacity
-    HIPCHK(hipMalloc(&pl->dA1, npairs * sizeof(int)));
-    HIPCHK(hipMalloc(&pl->dA2, npairs * sizeof(int)));
-    HIPCHK(hipMalloc(&pl->dPartial, npairs * gmxcorr::kRowSplit * sizeof(double)));
-    HIPCHK(hipMalloc(&pl->dFixsum, npairs * sizeof(double)));
-    HIPCHK(hipMalloc(&pl->dMI, npairs * sizeof(double)));
-    pl->pair_cap = npairs;
+    kind = pl->var_kind;
+    s4 = pl->var_s4;
+    if (pl->slots == 0) { kind = 4; s4 = 1; return; }
+    if (kind >= 0) return;
+    gmxcorr::default_variant(pl->N, pl->K, kind, s4);
+    if (kind == 3 && !pl->dLV) {
+        kind = 1;
+        s4 = std::max(2, (pl->N + 1023) / 1024);
+    }
+}
+
+int plan_split(const gmxcorr_ksg_plan* pl, int kind, long long npairs)
+{
+    if (kind == 4) return pl->nsplit > 0 ? pl->nsplit : gmxcorr::coord_split(npairs, pl->N);
+    return pl->nsplit > 0 ? pl->nsplit : default_split(npairs);
+}
+
+int ensure_pairs(gmxcorr_ksg_plan* pl, long long npairs, int slots_per_pair)
+{
+    const long long want_partial = npairs * std::max(1, slots_per_pair);
+    if (npairs <= pl->pair_cap && want_partial <= pl->partial_cap) return GMXCORR_OK;
+    if (npairs > pl->pair_cap) {
+        dfree(pl->dA1); dfree(pl->dA2); dfree(pl->dFixsum); dfree(pl->dMI);
+        pl->pair_cap = 0;   // a failed allocation below leaves no stale capacity
+        HIPCHK(hipMalloc(&pl->dA1, npairs * sizeof(int)));
+        HIPCHK(hipMalloc(&pl->dA2, npairs * sizeof(int)));
+        HIPCHK(hipMalloc(&pl->dFixsum, npairs * sizeof(double)));
+        HIPCHK(hipMalloc(&pl->dMI, npairs * sizeof(double)));
+        pl->pair_cap = npairs;
+        pl->all_pairs = false;
+    }
+    if (want_partial > pl->partial_cap) {
+        dfree(pl->dPartial);
+        pl->partial_cap = 0;
+        HIPCHK(hipMalloc(&pl->dPartial, want_partial * sizeof(double)));
+        pl->partial_cap = want_partial;
+    }
     return GMXCORR_OK;
 }
 
@@ -205,30 +292,44 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// Recount and long-row queue sizes for a run of `rows` frame rows: the
+// FP32-tie recount for 1% of the rows (runs measured here queue 0.04-0.2%),
+// the long-row queue for 4% (a prefix path keeps its share of rows beyond
+// the prefix below that; 1.8% at 400 atoms x 5000 frames, k = 100, 768
+// frames), at most 64M and 256M entries (1 and 4 GiB).
+long long want_fix_entries(long long rows)
+{
+    return std::min<long long>(std::max<long long>(65536, rows / 100), 64LL << 20);
+}
+long long want_long_entries(long long rows)
+{
+    return std::min<long long>(std::max<long long>(65536, rows / 25), 256LL << 20);
+}
 
-// Pair kernel + FP64 recount + finalise on the plan stream.  If the recount
-// queue overflowed, the run is repeated with a queue large enough.
+// Pair kernel + long rows + FP64 recount + finalise on the plan stream, for
+// npairs pairs already in dA1/dA2 (slot indices).  If a queue overflowed,
+// the run is repeated with a larger one.  Times add to the plan's stats.
 int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
 {
+    int kind, s4;
+    plan_kind(pl, kind, s4);
+    const int nsplit = plan_split(pl, kind, npairs);
+    if (int rc = ensure_pairs(pl, npairs, gmxcorr::variant_slots(kind, nsplit))) return rc;
     KsgArgs a{};
     if (tiles) { a.quads = pl->dQuads; a.nquads = pl->nquads; }
-    a.X = pl->dX; a.D = pl->dD; a.SD = pl->dSD; a.SM = pl->dSM; a.SI = pl->dSI; a.LV = pl->dLV; a.LI = pl->dLI; a.LS = pl->dLS; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
+    a.X = pl->dXs; a.D = pl->dD; a.SD = pl->dSD; a.SM = pl->dSM; a.SI = pl->dSI; a.LV = pl->dLV;
+    a.LI = pl->dLI; a.LS = pl->dLS; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
     a.partial = pl->dPartial; a.fixsum = pl->dFixsum; a.mi = pl->dMI;
     a.counters = pl->dCounters;
     a.npairs = npairs; a.P = pl->P; a.N = pl->N; a.K = pl->K; a.psiN_phiK = pl->psiN_phiK;
+    a.nsplit = nsplit; a.chunk = pl->chunk;
     if (detail) { a.rc_nx = pl->dRcNx; a.rc_ny = pl->dRcNy; a.rc_flag = pl->dRcFlag; }
-    // Queues sized up front, as an overflow re-runs the whole launch: the
-    // FP32-tie recount for 1% of the rows (runs measured here queue
-    // 0.04-0.2%), the long-row queue for 4% (a prefix path keeps its share
-    // of rows beyond the prefix below that; 1.8% at 400 atoms x 5000
-    // frames, k = 100, 768 frames), at most 64M and 256M entries (1 and 4
-    // GiB).  Both keep their size for later runs.
-    {
+    // queues sized up front, as an overflow re-runs the whole launch; both
+    // keep their size for later runs
+    if (kind != 4) {
         const long long rows = npairs * (long long)pl->N;
-        const long long want_fix = std::min<long long>(std::max<long long>(65536, rows / 100),
-                                                       64LL << 20);
-        const long long want_long = std::min<long long>(std::max<long long>(65536, rows / 25),
-                                                        256LL << 20);
+        const long long want_fix = want_fix_entries(rows);
+        const long long want_long = want_long_entries(rows);
         int rc = ensure_fix(pl, (unsigned)want_fix);
         if (!rc) {
             rc = ensure_long(pl, (unsigned)want_long);
@@ -254,8 +355,8 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
         f.fixq = pl->dFix;
         HIPCHK(gmxcorr::launch_ksg_clear(pl->dFixsum, npairs, pl->dCounters, nullptr, pl->stream));
         HIPCHK(hipEventRecord(pl->ev[2], pl->stream));
-        int nsplit = gmxcorr::kRowSplit;
-        HIPCHK(gmxcorr::launch_ksg_pairs(a, pl->stream, pl->var_kind, pl->var_s4, &nsplit));
+        int slots = nsplit;
+        HIPCHK(gmxcorr::launch_ksg_pairs(a, pl->stream, kind, s4, &slots));
         HIPCHK(hipEventRecord(pl->ev[3], pl->stream));
         unsigned counters[3] = {0, 0, 0};
         HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,
@@ -280,22 +381,157 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
             continue;
         }
         HIPCHK(gmxcorr::launch_ksg_fixup(f, counters[0], pl->stream));
-        HIPCHK(gmxcorr::launch_ksg_finalize(f, npairs, pl->dPartial, nsplit,
-                                                  pl->dMI, pl->stream));
+        HIPCHK(gmxcorr::launch_ksg_finalize(f, npairs, pl->dPartial, slots, pl->dMI, pl->stream));
         HIPCHK(hipEventRecord(pl->ev[4], pl->stream));
         HIPCHK(hipMemcpyAsync(counters, pl->dCounters, sizeof(counters), hipMemcpyDeviceToHost,
                               pl->stream));
         HIPCHK(hipStreamSynchronize(pl->stream));
-        pl->st.ms_pairs = elapsed(pl->ev[2], pl->ev[3]);
-        pl->st.ms_fixup = elapsed(pl->ev[3], pl->ev[4]);
-        pl->st.npairs = npairs;
-        pl->st.fixup_rows = counters[0];
-        pl->st.long_rows = nlong;
-        pl->st.ties = counters[1];
-        pl->last_npairs = npairs;
+        pl->st.ms_pairs += elapsed(pl->ev[2], pl->ev[3]);
+        pl->st.ms_fixup += elapsed(pl->ev[3], pl->ev[4]);
+        pl->st.npairs += npairs;
+        pl->st.fixup_rows += counters[0];
+        pl->st.long_rows += nlong;
+        pl->st.ties += counters[1];
+        pl->st.kind = kind;
+        pl->st.s4 = s4;
+        pl->st.nsplit = nsplit;
+        pl->st.chunk = pl->chunk;
         return GMXCORR_OK;
     }
     return fail(GMXCORR_EHIP, "recount queues overflowed on every attempt");
+}
+
+void reset_run_stats(gmxcorr_ksg_plan* pl)
+{
+    if (pl->wave_atoms > 0) pl->st.ms_build = 0.0;   // waves rebuild rows inside the run
+    pl->st.ms_pairs = pl->st.ms_fixup = 0.0;
+    pl->st.npairs = pl->st.fixup_rows = pl->st.long_rows = pl->st.ties = 0;
+    pl->st.waves = 0;
+}
+
+// Builds the rows of `nat` atoms whose coordinates sit at slot s0 of dXs.
+int build_slots(gmxcorr_ksg_plan* pl, int s0, int nat)
+{
+    const size_t soff = (size_t)s0 * pl->N;
+    HIPCHK(gmxcorr::launch_build_rows(
+        pl->dXs + 3 * soff, nat, pl->N, pl->P, pl->dD + soff * pl->P,
+        pl->dSD ? pl->dSD + soff * gmxcorr::sorted_pitch(pl->N) : nullptr,
+        pl->dSM ? pl->dSM + soff * gmxcorr::kSumPitch : nullptr,
+        pl->dSI ? pl->dSI + soff * gmxcorr::kPrefix : nullptr, pl->stream));
+    if (pl->dLV)
+        HIPCHK(gmxcorr::launch_build_prefix(pl->dD + soff * pl->P, nat, pl->N, pl->P,
+                                            pl->dLV + soff * gmxcorr::kLongPitch,
+                                            pl->dLI + soff * gmxcorr::kLongPrefix,
+                                            pl->dLS + soff * gmxcorr::kLongSum, pl->stream));
+    return GMXCORR_OK;
+}
+
+// Puts block b (atoms [b*B, min(n, (b+1)*B))) into wave slot w: its
+// coordinates into dXs, then its rows.
+int load_block(gmxcorr_ksg_plan* pl, int w, int b)
+{
+    if (pl->block_in_slot[w] == b) return GMXCORR_OK;
+    const int B = pl->wave_atoms;
+    const int a0 = b * B, nat = std::min(pl->natoms, a0 + B) - a0;
+    const int s0 = w * B;
+    HIPCHK(hipEventRecord(pl->ev[0], pl->stream));
+    HIPCHK(hipMemcpyAsync(pl->dXs + (size_t)3 * s0 * pl->N, pl->dX + (size_t)3 * a0 * pl->N,
+                          (size_t)3 * nat * pl->N * sizeof(double), hipMemcpyDeviceToDevice,
+                          pl->stream));
+    if (int rc = build_slots(pl, s0, nat)) return rc;
+    HIPCHK(hipEventRecord(pl->ev[1], pl->stream));
+    HIPCHK(hipStreamSynchronize(pl->stream));
+    pl->st.ms_build += elapsed(pl->ev[0], pl->ev[1]);
+    pl->block_in_slot[w] = b;
+    return GMXCORR_OK;
+}
+
+// Runs a global pair list (atom indices) in memory waves: pairs grouped by
+// their blocks (I >= J), block I's rows in slot 0 and J's in slot 1.  MI,
+// and the per-row outputs when asked, come back in the caller's order.
+int run_waves(gmxcorr_ksg_plan* pl, long long npairs, const int* a1, const int* a2, double* mi,
+              int* nx, int* ny, unsigned char* flags)
+{
+    const int B = pl->wave_atoms;
+    const int nb = (pl->natoms + B - 1) / B;
+    const bool detail = nx || ny || flags;
+    // pair p into bucket (I, J), I >= J, as slot-local indices
+    std::vector<std::vector<long long>> bucket((size_t)nb * nb);
+    for (long long p = 0; p < npairs; ++p) {
+        int I = a1[p] / B, J = a2[p] / B;
+        if (I < J) std::swap(I, J);
+        bucket[(size_t)I * nb + J].push_back(p);
+    }
+    std::vector<int> h1, h2;
+    const int N = pl->N;
+    for (int I = 0; I < nb; ++I) {
+        for (int J = 0; J <= I; ++J) {
+            const std::vector<long long>& ps = bucket[(size_t)I * nb + J];
+            if (ps.empty()) continue;
+            if (int rc = load_block(pl, 0, I)) return rc;
+            if (J != I)
+                if (int rc = load_block(pl, 1, J)) return rc;
+            const long long m = (long long)ps.size();
+            h1.resize(m);
+            h2.resize(m);
+            auto slot = [&](int a) { return a / B == I ? a - I * B : B + a - J * B; };
+            for (long long t = 0; t < m; ++t) {
+                h1[t] = slot(a1[ps[t]]);
+                h2[t] = slot(a2[ps[t]]);
+            }
+            if (int rc = ensure_pairs(pl, m, 1)) return rc;
+            HIPCHK(hipMemcpy(pl->dA1, h1.data(), m * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(pl->dA2, h2.data(), m * sizeof(int), hipMemcpyHostToDevice));
+            if (detail && m * N > pl->rc_cap) {
+                dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
+                pl->rc_cap = 0;
+                HIPCHK(hipMalloc(&pl->dRcNx, m * N * sizeof(int)));
+                HIPCHK(hipMalloc(&pl->dRcNy, m * N * sizeof(int)));
+                HIPCHK(hipMalloc(&pl->dRcFlag, m * N));
+                pl->rc_cap = m * N;
+            }
+            if (int rc = run_loaded(pl, m, detail, false)) return rc;
+            pl->st.waves += 1;
+            std::vector<double> got(m);
+            HIPCHK(hipMemcpy(got.data(), pl->dMI, m * sizeof(double), hipMemcpyDeviceToHost));
+            for (long long t = 0; t < m; ++t) mi[ps[t]] = got[t];
+            if (detail) {
+                std::vector<int> vx(m * N), vy(m * N);
+                std::vector<unsigned char> vf(m * N);
+                HIPCHK(hipMemcpy(vx.data(), pl->dRcNx, m * N * sizeof(int), hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(vy.data(), pl->dRcNy, m * N * sizeof(int), hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(vf.data(), pl->dRcFlag, m * N, hipMemcpyDeviceToHost));
+                for (long long t = 0; t < m; ++t) {
+                    const size_t dst = (size_t)ps[t] * N, src = (size_t)t * N;
+                    if (nx) std::memcpy(nx + dst, vx.data() + src, N * sizeof(int));
+                    if (ny) std::memcpy(ny + dst, vy.data() + src, N * sizeof(int));
+                    if (flags) std::memcpy(flags + dst, vf.data() + src, N);
+                }
+            }
+        }
+    }
+    return GMXCORR_OK;
+}
+
+void set_descriptor(gmxcorr_ksg_plan* pl, gmxcorr::TrajDesc& td)
+{
+    td = gmxcorr::TrajDesc{};
+    td.X = pl->dXs;
+    td.D = pl->dD;
+    td.SD = pl->dSD;
+    td.SM = pl->dSM;
+    td.SI = pl->dSI;
+    td.LV = pl->dLV;
+    td.LI = pl->dLI;
+    td.LS = pl->dLS;
+    td.psi = pl->dPsi;
+    td.P = pl->P;
+    td.N = pl->N;
+    td.K = pl->K;
+    td.psiN_phiK = pl->psiN_phiK;
+    td.kind = -1;
+    td.s4 = 0;
+    td.chunk = pl->chunk;
 }
 
 }  // namespace
@@ -303,7 +539,8 @@ int run_loaded(gmxcorr_ksg_plan* pl, long long npairs, bool detail, bool tiles)
 namespace gmxcorr {
 bool plan_view(const gmxcorr_ksg_plan* pl, PlanView* v)
 {
-    if (!pl || !pl->loaded) return false;
+    // the surrogate's features read every atom's distance rows
+    if (!pl || !pl->loaded || pl->slots != pl->natoms) return false;
     v->X = pl->dX;
     v->D = pl->dD;
     v->natoms = pl->natoms;
@@ -336,14 +573,51 @@ int gmxcorr_selftest_wave(int* out)
     return GMXCORR_OK;
 }
 
-int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** out)
+int gmxcorr_ksg_plan_layout(int natoms, int nframes, int k, long long max_row_bytes,
+                            int* slots, int* wave_atoms, int* stores)
+{
+    if (!slots || !wave_atoms || !stores) return fail(GMXCORR_EINVAL, "plan_layout: null output");
+    if (natoms < 2 || k < 1 || nframes <= k) return fail(GMXCORR_EINVAL, "need natoms >= 2 and 1 <= k < nframes");
+    if (max_row_bytes <= 0) return fail(GMXCORR_EINVAL, "plan_layout: need a row budget");
+    *slots = 0;
+    *wave_atoms = 0;
+    *stores = 0;
+    if (nframes > gmxcorr::kMaxFrames) return GMXCORR_OK;   // coordinates only
+    const long long base = d_atom_bytes(nframes) + sorted_atom_bytes(nframes);
+    const long long store = gmxcorr::variant_fits(3, 4, nframes, k) ? store_atom_bytes(nframes) : 0;
+    const long long n = natoms;
+    if (n * (base + store) <= max_row_bytes) {
+        *slots = natoms;
+        *stores = store > 0;
+    } else if (n * base <= max_row_bytes && store == 0) {
+        *slots = natoms;
+    } else {
+        // memory waves of two blocks; prefix stores kept if a block still
+        // holds at least two atoms with them
+        long long B = max_row_bytes / (2 * (base + store));
+        bool st = store > 0;
+        if (B < 2 && st) {
+            B = max_row_bytes / (2 * base);
+            st = false;
+        }
+        if (B >= 1) {
+            B = std::min<long long>(B, (n + 1) / 2);
+            *wave_atoms = (int)B;
+            *slots = (int)(2 * B);
+            *stores = st;
+        }
+    }
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_plan_create_budget(int natoms, int nframes, int k, long long max_row_bytes,
+                                   gmxcorr_ksg_plan** out)
 {
     if (!out) return fail(GMXCORR_EINVAL, "plan_create: null output");
     *out = nullptr;
     if (natoms < 2) return fail(GMXCORR_EINVAL, "need at least 2 atoms");
     if (k < 1 || nframes <= k) return fail(GMXCORR_EINVAL, "need 1 <= k < nframes");
-    if (nframes > gmxcorr::kMaxFrames)
-        return fail(GMXCORR_EUNSUP, "nframes above the register-resident kernel limit");
+    if (max_row_bytes < 0) return fail(GMXCORR_EINVAL, "negative row budget");
     if (!device_is_gfx950()) return fail(GMXCORR_ENODEV, "no gfx950 device");
     auto* pl = new gmxcorr_ksg_plan();
     pl->natoms = natoms; pl->N = nframes; pl->K = k;
@@ -355,65 +629,80 @@ int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** o
     for (auto& ev : pl->ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
     const size_t xbytes = (size_t)3 * natoms * nframes * sizeof(double);
-    const size_t dbytes = (size_t)natoms * nframes * pl->P * sizeof(float);
     if ((e = hipMalloc(&pl->dX, xbytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc(X)"));
     if ((e = hipMalloc(&pl->dPsi, (nframes + 1) * sizeof(double))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(psi)"));
-    if ((e = hipMalloc(&pl->dD, dbytes)) != hipSuccess)
-        return bail(hip_fail(e, "hipMalloc(distance rows)"));
-    size_t sbytes = 0;
-    if (nframes <= gmxcorr::kSortedMaxFrames) {
-        const size_t rows = (size_t)natoms * nframes;
-        sbytes = rows * ((gmxcorr::sorted_pitch(nframes) + gmxcorr::kSumPitch) * sizeof(float) +
-                         gmxcorr::kPrefix * sizeof(uint16_t));
-        if ((e = hipMalloc(&pl->dSD, rows * gmxcorr::sorted_pitch(nframes) * sizeof(float))) !=
-            hipSuccess)
-            return bail(hip_fail(e, "hipMalloc(sorted rows)"));
-        if ((e = hipMalloc(&pl->dSM, rows * gmxcorr::kSumPitch * sizeof(float))) != hipSuccess)
-            return bail(hip_fail(e, "hipMalloc(sorted-row summaries)"));
-        if ((e = hipMalloc(&pl->dSI, rows * gmxcorr::kPrefix * sizeof(uint16_t))) != hipSuccess)
-            return bail(hip_fail(e, "hipMalloc(sorted-row prefix indices)"));
-    }
     if ((e = hipMalloc(&pl->dCounters, 4 * sizeof(unsigned))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(counters)"));
     if ((e = hipMalloc(&pl->dTraj, sizeof(gmxcorr::TrajDesc))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(descriptor)"));
-    int rc = ensure_fix(pl, 65536);
-    if (rc) return bail(rc);
-    // the recount and long-row queues of an all-pairs run (1% of its rows
-    // each, as run_loaded sizes them), before the prefix stores take HBM
-    {
-        const long double run_rows = (long double)natoms * (natoms - 1) / 2 * nframes;
-        const long long want = (long long)std::min<long double>(
-            std::max<long double>(65536, run_rows / 100), (long double)(64LL << 20));
-        int rc2 = ensure_fix(pl, (unsigned)want);
-        if (!rc2 && gmxcorr::variant_fits(3, 4, nframes, k)) rc2 = ensure_long(pl, (unsigned)want);
-        if (rc2) return bail(rc2);
-    }
-    // prefix stores for the long-trajectory path when they fit with 4 GB to
-    // spare; otherwise the plan runs the cooperative path
-    if (gmxcorr::variant_fits(3, 4, nframes, k)) {
-        const size_t rows = (size_t)natoms * nframes;
-        const size_t lbytes = rows * (gmxcorr::kLongPitch * sizeof(float) +
-                                      gmxcorr::kLongPrefix * sizeof(uint16_t) +
-                                      gmxcorr::kLongSum * sizeof(float));
-        const size_t reserve = 4ull << 30;
+    // Row budget.  Automatic: free memory less the all-pairs run's pair
+    // buffers (ids, recount sum, MI, partial sums) and its two queues, and
+    // 2 GB for the rest.
+    const long double apairs = (long double)natoms * (natoms - 1) / 2;
+    const long long npairs = (long long)apairs;
+    if (max_row_bytes == 0) {
         size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > lbytes + reserve) {
-            if (hipMalloc(&pl->dLV, rows * gmxcorr::kLongPitch * sizeof(float)) != hipSuccess ||
-                hipMalloc(&pl->dLI, rows * gmxcorr::kLongPrefix * sizeof(uint16_t)) != hipSuccess ||
-                hipMalloc(&pl->dLS, rows * gmxcorr::kLongSum * sizeof(float)) != hipSuccess) {
+        if ((e = hipMemGetInfo(&free_b, &total_b)) != hipSuccess)
+            return bail(hip_fail(e, "hipMemGetInfo"));
+        const long long rows = npairs * (long long)nframes;
+        const long long pair_bytes =
+            npairs * (4 + 4 + 8 + 8 + 8 * (long long)default_split(npairs));
+        const long long queue_bytes =
+            (want_fix_entries(rows) + want_long_entries(rows)) * (long long)sizeof(FixEntry);
+        max_row_bytes = (long long)free_b - pair_bytes - queue_bytes - (2LL << 30);
+        if (max_row_bytes < (1LL << 20)) max_row_bytes = 1LL << 20;
+    }
+    pl->row_budget = max_row_bytes;
+    int stores = 0;
+    if (int rc = gmxcorr_ksg_plan_layout(natoms, nframes, k, max_row_bytes, &pl->slots,
+                                         &pl->wave_atoms, &stores))
+        return bail(rc);
+    if (pl->slots == natoms || pl->slots == 0) {
+        pl->dXs = pl->dX;   // pairs index atoms directly
+    } else {
+        if ((e = hipMalloc(&pl->dXs, (size_t)3 * pl->slots * nframes * sizeof(double))) != hipSuccess)
+            return bail(hip_fail(e, "hipMalloc(slot coordinates)"));
+    }
+    const size_t srows = (size_t)pl->slots * nframes;
+    long long row_bytes = 0;
+    if (pl->slots > 0) {
+        if ((e = hipMalloc(&pl->dD, srows * pl->P * sizeof(float))) != hipSuccess)
+            return bail(hip_fail(e, "hipMalloc(distance rows)"));
+        row_bytes += (long long)(srows * pl->P * sizeof(float));
+        if (nframes <= gmxcorr::kSortedMaxFrames) {
+            if ((e = hipMalloc(&pl->dSD, srows * gmxcorr::sorted_pitch(nframes) * sizeof(float))) !=
+                hipSuccess)
+                return bail(hip_fail(e, "hipMalloc(sorted rows)"));
+            if ((e = hipMalloc(&pl->dSM, srows * gmxcorr::kSumPitch * sizeof(float))) != hipSuccess)
+                return bail(hip_fail(e, "hipMalloc(sorted-row summaries)"));
+            if ((e = hipMalloc(&pl->dSI, srows * gmxcorr::kPrefix * sizeof(uint16_t))) != hipSuccess)
+                return bail(hip_fail(e, "hipMalloc(sorted-row prefix indices)"));
+            row_bytes += pl->slots * sorted_atom_bytes(nframes);
+        }
+        if (stores) {
+            if (hipMalloc(&pl->dLV, srows * gmxcorr::kLongPitch * sizeof(float)) != hipSuccess ||
+                hipMalloc(&pl->dLI, srows * gmxcorr::kLongPrefix * sizeof(uint16_t)) != hipSuccess ||
+                hipMalloc(&pl->dLS, srows * gmxcorr::kLongSum * sizeof(float)) != hipSuccess) {
+                // the plan runs the cooperative path instead
                 (void)hipGetLastError();
                 dfree(pl->dLV); dfree(pl->dLI); dfree(pl->dLS);
             } else {
-                sbytes += lbytes;
+                row_bytes += pl->slots * store_atom_bytes(nframes);
             }
         }
     }
-    pl->st.row_bytes = (long long)(dbytes + sbytes);
+    pl->st.row_bytes = row_bytes;
     pl->st.nframes = nframes; pl->st.k = k; pl->st.natoms = natoms; pl->st.pitch = (int)pl->P;
+    pl->st.slots = pl->slots;
+    pl->st.wave_atoms = pl->wave_atoms;
     *out = pl;
     return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** out)
+{
+    return gmxcorr_ksg_plan_create_budget(natoms, nframes, k, 0, out);
 }
 
 int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const double* psi)
@@ -421,37 +710,22 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
     if (!pl || !xrows || !psi) return fail(GMXCORR_EINVAL, "plan_load: null argument");
     const size_t nx = (size_t)3 * pl->natoms * pl->N;
     if (int rc = gmxcorr::check_rows(xrows, 3LL * pl->natoms, pl->N, "plan_load")) return rc;
+    pl->loaded = false;
+    pl->block_in_slot[0] = pl->block_in_slot[1] = -1;
     HIPCHK(hipEventRecord(pl->ev[0], pl->stream));
     HIPCHK(hipMemcpyAsync(pl->dX, xrows, nx * sizeof(double), hipMemcpyHostToDevice, pl->stream));
     HIPCHK(hipMemcpyAsync(pl->dPsi, psi, (pl->N + 1) * sizeof(double), hipMemcpyHostToDevice,
                           pl->stream));
     HIPCHK(hipEventRecord(pl->ev[1], pl->stream));
-    HIPCHK(gmxcorr::launch_build_rows(pl->dX, pl->natoms, pl->N, pl->P, pl->dD, pl->dSD, pl->dSM,
-                                      pl->dSI, pl->stream));
-    if (pl->dLV)
-        HIPCHK(gmxcorr::launch_build_prefix(pl->dD, pl->natoms, pl->N, pl->P, pl->dLV, pl->dLI,
-                                            pl->dLS, pl->stream));
+    if (pl->slots == pl->natoms)
+        if (int rc = build_slots(pl, 0, pl->natoms)) return rc;
     HIPCHK(hipEventRecord(pl->ev[5], pl->stream));
     HIPCHK(hipStreamSynchronize(pl->stream));
     pl->st.ms_load = elapsed(pl->ev[0], pl->ev[1]);
     pl->st.ms_build = elapsed(pl->ev[1], pl->ev[5]);
     pl->psiN_phiK = psi[pl->N] + (psi[pl->K] - 1 / (1.0 * pl->K));
-    gmxcorr::TrajDesc td{};
-    td.X = pl->dX;
-    td.D = pl->dD;
-    td.SD = pl->dSD;
-    td.SM = pl->dSM;
-    td.SI = pl->dSI;
-    td.LV = pl->dLV;
-    td.LI = pl->dLI;
-    td.LS = pl->dLS;
-    td.psi = pl->dPsi;
-    td.P = pl->P;
-    td.N = pl->N;
-    td.K = pl->K;
-    td.psiN_phiK = pl->psiN_phiK;
-    td.kind = -1;
-    td.s4 = 0;
+    gmxcorr::TrajDesc td;
+    set_descriptor(pl, td);
     HIPCHK(hipMemcpy(pl->dTraj, &td, sizeof(td), hipMemcpyHostToDevice));
     pl->loaded = true;
     return GMXCORR_OK;
@@ -462,9 +736,27 @@ int gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* pl)
     if (!pl || !pl->loaded) return fail(GMXCORR_EINVAL, "plan not loaded");
     const long long n = pl->natoms;
     const long long npairs = n * (n - 1) / 2;
-    if (!pl->all_pairs || pl->pair_cap < npairs) {
-        int rc = ensure_pairs(pl, npairs);
-        if (rc) return rc;
+    reset_run_stats(pl);
+    if (pl->wave_atoms > 0) {
+        std::vector<int> h1(npairs), h2(npairs);
+        long long idx = 0;
+        for (int a1 = 1; a1 < n; ++a1)
+            for (int a2 = 0; a2 < a1; ++a2, ++idx) { h1[idx] = a1; h2[idx] = a2; }
+        pl->host_mi.assign(npairs, 0.0);
+        pl->all_pairs = false;
+        if (int rc = run_waves(pl, npairs, h1.data(), h2.data(), pl->host_mi.data(), nullptr,
+                               nullptr, nullptr))
+            return rc;
+        pl->host_result = true;
+        pl->last_npairs = npairs;
+        pl->st.all_pairs = 1;
+        return GMXCORR_OK;
+    }
+    int kind, s4;
+    plan_kind(pl, kind, s4);
+    if (int rc = ensure_pairs(pl, npairs, gmxcorr::variant_slots(kind, plan_split(pl, kind, npairs))))
+        return rc;
+    if (!pl->all_pairs) {
         std::vector<int> h1(npairs), h2(npairs);
         long long idx = 0;
         for (int a1 = 1; a1 < n; ++a1)
@@ -479,7 +771,13 @@ int gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* pl)
         }
         pl->all_pairs = true;
     }
-    return run_loaded(pl, npairs, false, pl->dQuads != nullptr);
+    pl->host_result = false;
+    const int rc = run_loaded(pl, npairs, false, pl->dQuads != nullptr);
+    if (!rc) {
+        pl->last_npairs = npairs;
+        pl->st.all_pairs = 1;
+    }
+    return rc;
 }
 
 int gmxcorr_ksg_plan_run_pairs(gmxcorr_ksg_plan* pl, long long npairs, const int* a1,
@@ -490,10 +788,21 @@ int gmxcorr_ksg_plan_run_pairs(gmxcorr_ksg_plan* pl, long long npairs, const int
     for (long long p = 0; p < npairs; ++p)
         if (a1[p] < 0 || a2[p] < 0 || a1[p] >= pl->natoms || a2[p] >= pl->natoms || a1[p] == a2[p])
             return fail(GMXCORR_EINVAL, "pair index out of range or a1 == a2");
-    const bool detail = nx || ny || flags;
-    int rc = ensure_pairs(pl, npairs);
-    if (rc) return rc;
+    reset_run_stats(pl);
     pl->all_pairs = false;
+    pl->st.all_pairs = 0;
+    if (pl->wave_atoms > 0) {
+        pl->host_mi.assign(npairs, 0.0);
+        if (int rc = run_waves(pl, npairs, a1, a2, pl->host_mi.data(), nx, ny, flags)) return rc;
+        pl->host_result = true;
+        pl->last_npairs = npairs;
+        return GMXCORR_OK;
+    }
+    const bool detail = nx || ny || flags;
+    int kind, s4;
+    plan_kind(pl, kind, s4);
+    int rc = ensure_pairs(pl, npairs, gmxcorr::variant_slots(kind, plan_split(pl, kind, npairs)));
+    if (rc) return rc;
     HIPCHK(hipMemcpy(pl->dA1, a1, npairs * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pl->dA2, a2, npairs * sizeof(int), hipMemcpyHostToDevice));
     const long long rows = npairs * pl->N;
@@ -505,8 +814,10 @@ int gmxcorr_ksg_plan_run_pairs(gmxcorr_ksg_plan* pl, long long npairs, const int
         HIPCHK(hipMalloc(&pl->dRcFlag, rows));
         pl->rc_cap = rows;
     }
+    pl->host_result = false;
     rc = run_loaded(pl, npairs, detail, false);
     if (rc) return rc;
+    pl->last_npairs = npairs;
     if (nx) HIPCHK(hipMemcpy(nx, pl->dRcNx, rows * sizeof(int), hipMemcpyDeviceToHost));
     if (ny) HIPCHK(hipMemcpy(ny, pl->dRcNy, rows * sizeof(int), hipMemcpyDeviceToHost));
     if (flags) HIPCHK(hipMemcpy(flags, pl->dRcFlag, rows, hipMemcpyDeviceToHost));
@@ -517,6 +828,10 @@ int gmxcorr_ksg_plan_result(gmxcorr_ksg_plan* pl, double* mi, long long npairs)
 {
     if (!pl || !mi) return fail(GMXCORR_EINVAL, "plan_result: null argument");
     if (npairs != pl->last_npairs) return fail(GMXCORR_EINVAL, "pair count differs from last run");
+    if (pl->host_result) {
+        std::memcpy(mi, pl->host_mi.data(), npairs * sizeof(double));
+        return GMXCORR_OK;
+    }
     HIPCHK(hipMemcpy(mi, pl->dMI, npairs * sizeof(double), hipMemcpyDeviceToHost));
     return GMXCORR_OK;
 }
@@ -526,10 +841,10 @@ int gmxcorr_ksg_plan_matrix(gmxcorr_ksg_plan* pl, double* mat)
     if (!pl || !mat) return fail(GMXCORR_EINVAL, "plan_matrix: null argument");
     const int n = pl->natoms;
     const long long npairs = (long long)n * (n - 1) / 2;
-    if (!pl->all_pairs || pl->last_npairs != npairs)
+    if (!pl->st.all_pairs || pl->last_npairs != npairs)
         return fail(GMXCORR_EINVAL, "last run was not run_all");
     std::vector<double> mi(npairs);
-    HIPCHK(hipMemcpy(mi.data(), pl->dMI, npairs * sizeof(double), hipMemcpyDeviceToHost));
+    if (int rc = gmxcorr_ksg_plan_result(pl, mi.data(), npairs)) return rc;
     // diagonal 2000 and symmetric fill, as lib/kraskov_gpu.cu:294-303
     for (int i = 0; i < n; ++i) mat[(size_t)i * n + i] = 2000.0;
     long long idx = 0;
@@ -544,8 +859,26 @@ int gmxcorr_ksg_plan_set_variant(gmxcorr_ksg_plan* pl, int kind, int s4)
     if (!pl) return fail(GMXCORR_EINVAL, "set_variant: null plan");
     if (kind >= 0 && !gmxcorr::variant_fits(kind, s4, pl->N, pl->K))
         return fail(GMXCORR_EUNSUP, "kernel variant does not cover this frame count and k");
+    if (kind >= 0 && kind != 4 && pl->slots == 0)
+        return fail(GMXCORR_EUNSUP, "plan holds no distance rows: only the coordinate path (kind 4)");
     pl->var_kind = kind;
     pl->var_s4 = s4;
+    return GMXCORR_OK;
+}
+
+int gmxcorr_ksg_plan_set_shape(gmxcorr_ksg_plan* pl, int nsplit, int chunk)
+{
+    if (!pl) return fail(GMXCORR_EINVAL, "set_shape: null plan");
+    if (nsplit < 0 || nsplit > gmxcorr::kMaxRowSplit)
+        return fail(GMXCORR_EINVAL, "set_shape: nsplit must be in [0, 256] (0 = default)");
+    if (chunk < 0 || chunk > 1024) return fail(GMXCORR_EINVAL, "set_shape: chunk must be in [0, 1024] (0 = default)");
+    pl->nsplit = nsplit;
+    pl->chunk = chunk > 0 ? chunk : gmxcorr::kLongChunk;
+    if (pl->loaded) {   // the recount kernels read the descriptor
+        gmxcorr::TrajDesc td;
+        set_descriptor(pl, td);
+        HIPCHK(hipMemcpy(pl->dTraj, &td, sizeof(td), hipMemcpyHostToDevice));
+    }
     return GMXCORR_OK;
 }
 
@@ -565,10 +898,13 @@ void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* pl)
 {
     if (!pl) return;
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dSD); dfree(pl->dSM); dfree(pl->dSI); dfree(pl->dLV); dfree(pl->dLI); dfree(pl->dLS); dfree(pl->dQuads); dfree(pl->dA1); dfree(pl->dA2);
-    dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix); dfree(pl->dLong);
-    dfree(pl->dCounters); dfree(pl->dRcNx); dfree(pl->dRcNy); dfree(pl->dRcFlag);
-    dfree(pl->dTraj);
+    if (pl->dXs != pl->dX) dfree(pl->dXs);
+    pl->dXs = nullptr;
+    dfree(pl->dX); dfree(pl->dPsi); dfree(pl->dD); dfree(pl->dSD); dfree(pl->dSM); dfree(pl->dSI);
+    dfree(pl->dLV); dfree(pl->dLI); dfree(pl->dLS); dfree(pl->dQuads); dfree(pl->dA1);
+    dfree(pl->dA2); dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix);
+    dfree(pl->dLong); dfree(pl->dCounters); dfree(pl->dRcNx); dfree(pl->dRcNy);
+    dfree(pl->dRcFlag); dfree(pl->dTraj);
     for (auto& ev : pl->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (pl->stream) (void)hipStreamDestroy(pl->stream);
@@ -587,6 +923,31 @@ int gmxcorr_ksg_matrix(const double* xrows, int natoms, int nframes, const doubl
     if (!rc) rc = gmxcorr_ksg_plan_matrix(pl, mat);
     gmxcorr_ksg_plan_destroy(pl);
     return rc;
+}
+
+// ---- Device-pointer pair list on the coordinate path ---------------------
+int gmxcorr_ksg_pairs_device(const double* xrows, int natoms, int nframes, const double* psi,
+                             int k, long long npairs, const int* a1, const int* a2, double* mi,
+                             void* stream)
+{
+    if (!xrows || !psi || !a1 || !a2 || !mi) return fail(GMXCORR_EINVAL, "pairs_device: null argument");
+    if (natoms < 2 || k < 1 || nframes <= k || npairs < 1)
+        return fail(GMXCORR_EINVAL, "pairs_device: need natoms >= 2, 1 <= k < nframes, npairs >= 1");
+    if (!device_is_gfx950()) return fail(GMXCORR_ENODEV, "no gfx950 device");
+    const hipStream_t s = (hipStream_t)stream;
+    const int nsplit = gmxcorr::coord_split(npairs, nframes);
+    double* part = nullptr;
+    unsigned* counters = nullptr;
+    HIPCHK(hipMalloc(&part, (size_t)npairs * nsplit * sizeof(double)));
+    hipError_t e = hipMalloc(&counters, 4 * sizeof(unsigned));
+    if (e == hipSuccess)
+        e = gmxcorr::launch_ksg_coord_pairs(xrows, nframes, psi, k, npairs, a1, a2, nsplit, part,
+                                            counters, mi, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(part);
+    if (counters) (void)hipFree(counters);
+    if (e != hipSuccess) return hip_fail(e, "pairs_device");
+    return GMXCORR_OK;
 }
 
 }  // extern "C"

@@ -36,7 +36,7 @@ struct KsgArgs {
     const double* psi;    // [N+1]
     const int*    a1;     // [npairs]
     const int*    a2;     // [npairs]
-    double*       partial;   // [npairs][kRowSplit] sums of psi[nx]+psi[ny] over unflagged rows
+    double*       partial;   // [npairs][nsplit] sums of psi[nx]+psi[ny] over unflagged rows
     double*       fixsum;    // [npairs] same over FP64-recounted rows
     double*       mi;        // [npairs]
     FixEntry*     fixq;      // [fixcap]
@@ -55,6 +55,8 @@ struct KsgArgs {
     int           K;
     double        psiN_phiK; // psi[N] + (psi[K] - 1/K)
     long long     vb0;       // first virtual block of this launch (set by the launcher)
+    int           nsplit;    // row ranges per pair (kinds 0, 2 and 4; partial slots per pair)
+    int           chunk;     // frame rows per block of kind 3
 };
 
 // One trajectory's device data and shape.  kind/s4 select the pair-kernel
@@ -75,8 +77,9 @@ struct TrajDesc {
     double psiN_phiK;
     int kind;
     int s4;
+    int chunk;           // frame rows per kind-3 work item
     // batched launch: this trajectory's work items (kind 3: 2x2 atom tiles
-    // times kLongChunk-row chunks, chunk-major; otherwise one per pair), the
+    // times chunk-row chunks, chunk-major; otherwise one per pair), the
     // global index of its first pair, and its tile table
     long long nitems;
     long long pair0;
@@ -117,17 +120,24 @@ struct KsgBatchArgs {
     int ntraj;         // traj[] in work order (decreasing frame count)
 };
 
-// Largest N handled by the pair kernels (cooperative kernel, 4 x 2048 frames).
+// Largest N handled by the distance-row pair kernels (cooperative kernel,
+// 4 x 2048 frames).  Longer trajectories take the coordinate path (kind 4),
+// which has no frame limit.
 constexpr int kMaxFrames = 8192;
 
-// The single-trajectory pair kernels of kinds 0-2 write kRowSplit partial
-// sums per pair (frame rows split into that many ranges, summed in a fixed
-// order); partial holds npairs*kRowSplit.  Kind 3 adds its row sums into
-// fixsum instead.  Short ranges keep the blocks in flight on one XCD on the
-// same few rows of neighbouring tiles, whose shared atoms' rows then come
-// from L2: at 247 atoms x 2000 frames, k = 100, the sorted-prefix kernel ran
-// 57.4 ms with 16 ranges, 56.2 with 32, 53.9 with 64 and 49.0 with 128.
+// The single-trajectory pair kernels of kinds 0, 2 and 4 split a pair's
+// frame rows into nsplit ranges (KsgArgs::nsplit) and write one partial sum
+// per range, summed in a fixed order by the finalise kernel; partial holds
+// npairs * nsplit.  Kind 1 writes one partial per pair and kind 3 adds its
+// row sums into fixsum.  Short ranges keep the blocks in flight on one XCD
+// on the same few rows of neighbouring tiles, whose shared atoms' rows then
+// come from L2: at 247 atoms x 2000 frames, k = 100, the sorted-prefix
+// kernel ran 57.4 ms with 16 ranges, 56.2 with 32, 53.9 with 64 and 49.0
+// with 128.  kRowSplit is the default; the autotuner may choose another
+// value up to kMaxRowSplit, and plans with very many pairs take fewer so
+// that the partial sums stay small against the distance rows.
 constexpr int kRowSplit = 128;
+constexpr int kMaxRowSplit = 256;
 
 // Distance-row pitch in floats: a multiple of 256 frames, so the single-wave
 // kernels read whole 64-lane chunks of 4 frames without a per-lane bound.
@@ -154,10 +164,11 @@ __host__ __device__ inline long long sorted_pitch(int N)
 // zeros (pitch kLongPitch floats), their frames (u16), and kLongSum summary
 // values (value 64m + 63).  6.4 KB per row instead of a full sorted row.
 constexpr int kLongPrefix = 1024;
-// Frame rows per block of the long-trajectory pair kernel.  Short chunks
-// keep the blocks in flight on one XCD on the same rows (as kRowSplit does
-// for the sorted-prefix kernel): 16 rows ran 4-11% faster than 32 on 400
-// atoms x 5000 frames (k = 100) and x 8000 frames (k = 10).
+// Frame rows per block of the long-trajectory pair kernel (KsgArgs::chunk,
+// TrajDesc::chunk; this is the default, the autotuner may choose another).
+// Short chunks keep the blocks in flight on one XCD on the same rows (as the
+// row ranges do for the sorted-prefix kernel): 16 rows ran 4-11% faster
+// than 32 on 400 atoms x 5000 frames (k = 100) and x 8000 frames (k = 10).
 constexpr int kLongChunk = 16;
 constexpr int kLongPitch = 64 + kLongPrefix;
 constexpr int kLongSum = kLongPrefix / 64;
@@ -195,14 +206,28 @@ hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
 // kind 3: the same over the first 256*s4 frames of the kLongPrefix prefix
 //         stores (s4 in 2, 3, 4), gathering the other atom's distances from
 //         its row, kSortedMaxFrames < N <= kMaxFrames and K + 1 <= kPrefixMaxT;
+// kind 4: every row in FP64 from the coordinates (no distance rows, any N
+//         and K; s4 ignored);
 // kind < 0: default.
 bool variant_fits(int kind, int s4, int N, int K);
 void default_variant(int N, int K, int& kind, int& s4);
+// Partial-sum slots per pair that a launch of this kind writes (the
+// finalise step's nsplit; 0: all of it goes to fixsum).
+int variant_slots(int kind, int nsplit);
 // nsplit (optional): partial-sum slots per pair the launch wrote, for the
-// finalise step (0: all of it went to fixsum).
+// finalise step (0: all of it went to fixsum).  a.nsplit / a.chunk: row
+// ranges per pair (kinds 0, 2, 4) and rows per block (kind 3).
 hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind = -1, int s4 = 0,
                             int* nsplit = nullptr);
 hipError_t launch_ksg_fixup(const FixupArgs& f, unsigned nqueued, hipStream_t s);
+// Coordinate path (kind 4) over a device pair list with caller-provided
+// scratch: partial [npairs * nsplit] doubles and counters [4]; MI to mi.
+// Clears the counters, runs the pairs and the finalise on stream s.
+hipError_t launch_ksg_coord_pairs(const double* X, int N, const double* psi, int K,
+                                 long long npairs, const int* a1, const int* a2, int nsplit,
+                                 double* partial, unsigned* counters, double* mi, hipStream_t s);
+// Row ranges per pair the coordinate path uses for npairs pairs of N frames.
+int coord_split(long long npairs, int N);
 // Rows that the sorted-prefix path queued to a.longq (k-th neighbour beyond
 // the prefix): the streaming one-wave path per row, psi sums into a.fixsum.
 hipError_t launch_ksg_long_rows(const KsgArgs& a, unsigned nlong, hipStream_t s);

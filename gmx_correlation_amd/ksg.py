@@ -60,13 +60,31 @@ def kraskov_prepare(traj: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     return rows, psi
 
 
-class KsgPlan:
-    """Device-resident KSG run: distance rows stay in HBM between runs."""
+def plan_layout(natoms: int, nframes: int, k: int, max_row_bytes: int) -> dict:
+    """How a plan of this shape holds its distance rows for a row budget
+    (host arithmetic, no device): slots = natoms (all rows resident),
+    2 * wave_atoms (memory waves of two atom blocks) or 0 (coordinate path
+    only); stores = 1 when the long-trajectory prefix stores fit too."""
+    slots, wave, stores = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _lib.check(_lib.load().gmxcorr_ksg_plan_layout(natoms, nframes, k, int(max_row_bytes),
+                                                   ctypes.byref(slots), ctypes.byref(wave),
+                                                   ctypes.byref(stores)))
+    return {"slots": slots.value, "wave_atoms": wave.value, "stores": stores.value}
 
-    def __init__(self, natoms: int, nframes: int, k: int):
+
+class KsgPlan:
+    """Device-resident KSG run: distance rows stay in HBM between runs.
+
+    max_row_bytes: budget for distance rows (0: free device memory less the
+    pair buffers).  When every atom's rows do not fit, pairs run in memory
+    waves of two atom blocks; above 8192 frames (or when not even two atoms'
+    rows fit) every row is computed from the coordinates (kind 4)."""
+
+    def __init__(self, natoms: int, nframes: int, k: int, max_row_bytes: int = 0):
         L = _lib.load()
         h = ctypes.c_void_p()
-        _lib.check(L.gmxcorr_ksg_plan_create(natoms, nframes, k, ctypes.byref(h)))
+        _lib.check(L.gmxcorr_ksg_plan_create_budget(natoms, nframes, k, int(max_row_bytes),
+                                                    ctypes.byref(h)))
         self._h = h
         self.natoms, self.nframes, self.k = natoms, nframes, k
 
@@ -120,8 +138,14 @@ class KsgPlan:
         (256*s4 frames), kind 1 four waves per row (1024*s4 frames), kind 2
         one wave per row over the sorted prefix (N <= 2048, k + 1 <= 256),
         kind 3 the same over the first 256*s4 frames of the prefix stores
-        (2048 < N <= 8192, s4 = 2, 3 or 4), kind -1 default."""
+        (2048 < N <= 8192, s4 = 2, 3 or 4), kind 4 every row in FP64 from
+        the coordinates (any N), kind -1 default."""
         _lib.check(_lib.load().gmxcorr_ksg_plan_set_variant(self._h, kind, s4))
+
+    def set_shape(self, nsplit: int = 0, chunk: int = 0) -> None:
+        """Row-block shape: nsplit row ranges per pair (kinds 0, 2, 4; 0 =
+        default), chunk frame rows per block of kind 3 (0 = default 16)."""
+        _lib.check(_lib.load().gmxcorr_ksg_plan_set_shape(self._h, nsplit, chunk))
 
     def stats(self) -> dict:
         st = _lib.KsgStats()
@@ -144,6 +168,16 @@ class KsgPlan:
             self.close()
         except Exception:
             pass
+
+
+def ksg_pairs_device(xrows_ptr: int, natoms: int, nframes: int, psi_ptr: int, k: int,
+                     npairs: int, a1_ptr: int, a2_ptr: int, mi_ptr: int, stream: int = 0) -> None:
+    """MI of a pair list on the coordinate path, all arrays already in device
+    memory (raw device addresses): xrows [3*natoms][nframes] float64, psi
+    [nframes+1] float64, a1/a2 [npairs] int32, mi [npairs] float64 (out).
+    The caller guarantees 0 <= a1, a2 < natoms and a1 != a2."""
+    _lib.check(_lib.load().gmxcorr_ksg_pairs_device(
+        xrows_ptr, natoms, nframes, psi_ptr, k, npairs, a1_ptr, a2_ptr, mi_ptr, stream or None))
 
 
 def kraskov_corrmatrix(traj: np.ndarray, k: int, use_gpu: bool = True) -> np.ndarray:

@@ -35,7 +35,8 @@ int gmxcorr_probe(void);
  * device 0 and copies their 67 results to out (test hook). */
 int gmxcorr_selftest_wave(int* out);
 
-/* Largest frame count the KSG kernels accept. */
+/* Largest frame count of the distance-row pair kernels (kinds 0-3).  Longer
+ * trajectories run on the coordinate path (kind 4), which has no limit. */
 int gmxcorr_ksg_max_frames(void);
 
 /* One-shot KSG-2 matrix: the flat-array form of kraskov_corrmatrix_gpu()
@@ -55,19 +56,42 @@ typedef struct gmxcorr_ksg_plan gmxcorr_ksg_plan;
 
 typedef struct {
     double ms_load;        /* host->device copy of rows and psi */
-    double ms_build;       /* per-atom distance-row build */
-    double ms_pairs;       /* main pair kernel (HIP events on the plan stream) */
+    double ms_build;       /* per-atom distance-row build (in memory waves: all of
+                              the last run's rebuilds) */
+    double ms_pairs;       /* main pair kernel (HIP events on the plan stream;
+                              summed over memory waves) */
     double ms_fixup;       /* long rows + FP64 recount of flagged rows + finalise */
     long long npairs;      /* pairs in the last run */
     long long fixup_rows;  /* rows recounted in FP64 (FP32 near-ties) */
     long long long_rows;   /* rows whose k-th neighbour lay beyond the sorted
                               prefix, rerun on the streaming path */
     long long ties;        /* rows with an exact FP64 tie at the k-th distance */
-    long long row_bytes;   /* bytes of distance rows resident in HBM */
+    long long row_bytes;   /* bytes of distance rows (and sorted rows / prefix
+                              stores) the plan holds in HBM */
     int nframes, k, natoms, pitch;
+    int kind, s4;          /* code path of the last run (kind 4: coordinates) */
+    int nsplit, chunk;     /* its row ranges per pair and kind-3 rows per block */
+    int slots;             /* atoms whose rows fit at once (natoms: resident;
+                              0: coordinate path only) */
+    int wave_atoms;        /* atoms per block in memory waves (0: no waves) */
+    int waves;             /* block pairs the last run went through */
+    int all_pairs;         /* 1 if the last run was run_all */
 } gmxcorr_ksg_stats;
 
+/* Layout of a plan: rows resident for every atom, memory waves of two
+ * blocks of wave_atoms atoms, or the coordinate path only (slots = 0; also
+ * for nframes > gmxcorr_ksg_max_frames()), for a budget of max_row_bytes of
+ * distance rows, sorted rows and prefix stores.  Host-only arithmetic (no
+ * device needed).  stores: 1 if the long-trajectory prefix stores are
+ * included. */
+int  gmxcorr_ksg_plan_layout(int natoms, int nframes, int k, long long max_row_bytes,
+                             int* slots, int* wave_atoms, int* stores);
+/* Plan with the automatic row budget (free device memory less the pair
+ * buffers and recount queues of an all-pairs run and 2 GB). */
 int  gmxcorr_ksg_plan_create(int natoms, int nframes, int k, gmxcorr_ksg_plan** out);
+/* Same with an explicit budget (0: automatic). */
+int  gmxcorr_ksg_plan_create_budget(int natoms, int nframes, int k, long long max_row_bytes,
+                                    gmxcorr_ksg_plan** out);
 int  gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* plan, const double* xrows, const double* psi);
 int  gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* plan);
 /* MI of an explicit pair list (a1[p] != a2[p]), e.g. the exact-KSG fallback
@@ -88,12 +112,32 @@ int  gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* plan, gmxcorr_ksg_stats* st)
  * of the sparser atom (nframes <= 2048, k + 1 <= 256; s4 ignored); kind 3 =
  * the same over the first 256*s4 frames of 1024-frame prefix stores
  * (2048 < nframes <= 8192, k + 1 <= 256, s4 = 2, 3 or 4; built when they fit
- * in HBM); kind -1 =
+ * in HBM); kind 4 = every row in FP64 from the coordinates (any nframes and
+ * k, no distance rows; the default above 8192 frames); kind -1 =
  * the default for the plan's frame count and k.  variant_fits: 1 if
  * (kind, s4) is compiled and covers nframes at this k. */
 int  gmxcorr_ksg_plan_set_variant(gmxcorr_ksg_plan* plan, int kind, int s4);
+/* Row-block shape (the autotuner's second knob): nsplit row ranges per pair
+ * for kinds 0, 2 and 4 (1..256; 0 = default: 128 for kinds 0 and 2, fewer
+ * when the partial sums would pass 8 GB, and enough to fill the device for
+ * kind 4), chunk frame rows per block of kind 3 (1..1024; 0 = default 16). */
+int  gmxcorr_ksg_plan_set_shape(gmxcorr_ksg_plan* plan, int nsplit, int chunk);
 int  gmxcorr_ksg_variant_fits(int kind, int s4, int nframes, int k);
 void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* plan);
+
+/* ---- Device-pointer pair list on the coordinate path -----------------------
+ * MI of npairs pairs (a1[p] != a2[p], both < natoms) computed from the
+ * coordinates alone: every frame row in FP64 (kind 4), no distance rows, any
+ * nframes and k < nframes, device memory O(natoms * nframes) as the
+ * reference's kernel (lib/kraskov_gpu.cu:107-196).  xrows [3*natoms][nframes],
+ * psi [nframes+1], a1, a2 [npairs] and mi [npairs] are DEVICE pointers; the
+ * work runs on `stream` (a hipStream_t, NULL = default stream).  Allocates its
+ * scratch (8 * npairs * nsplit bytes, nsplit <= 256) and returns when mi is
+ * written.  Indices are not checked on the device: the caller guarantees
+ * them. */
+int gmxcorr_ksg_pairs_device(const double* xrows, int natoms, int nframes, const double* psi,
+                             int k, long long npairs, const int* a1, const int* a2, double* mi,
+                             void* stream);
 
 /* ---- Gaussian and Gaussian-copula matrices --------------------------------
  * traj: mean-centred fluctuations [natoms][nframes][3] (t_traj.x flattened).

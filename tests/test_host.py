@@ -186,3 +186,30 @@ def test_read_dump_rejects_malformed(tmp_path):
     p.write_text("oops\n")
     with pytest.raises(ValueError):
         ksg.read_dump(str(p))
+
+
+def test_plan_layout_memory_sizing_without_gpu(hiplib):
+    """Host arithmetic of the plan layout: all rows resident when they fit
+    the budget, memory waves of two atom blocks when they do not, the
+    coordinate path above 8192 frames or below two atoms' rows."""
+    P = lambda n: (n + 255) // 256 * 256
+    per_sorted = lambda n: n * P(n) * 4 + n * ((64 + (max(n, 512) + 127) // 64 * 64) * 4
+                                               + 64 * 4 + 512 * 2)
+    per_store = lambda n: n * P(n) * 4 + n * ((64 + 1024) * 4 + 1024 * 2 + 16 * 4)
+    # 247 x 2000 (8.98 GB of rows) with 10 GB: resident
+    assert ksg.plan_layout(247, 2000, 100, 10 * 10**9) == {"slots": 247, "wave_atoms": 0,
+                                                           "stores": 0}
+    assert 247 * per_sorted(2000) == 8978944000
+    # with 1 GB: two blocks of B atoms per wave
+    B = 10**9 // (2 * per_sorted(2000))
+    assert ksg.plan_layout(247, 2000, 100, 10**9) == {"slots": 2 * B, "wave_atoms": B,
+                                                      "stores": 0}
+    # long trajectories keep the prefix stores while a block holds two atoms
+    lay = ksg.plan_layout(2000, 5000, 100, 200 * 10**9)
+    assert lay["stores"] == 1 and lay["wave_atoms"] == 200 * 10**9 // (2 * per_store(5000))
+    assert ksg.plan_layout(50, 5000, 300, 10**12)["stores"] == 0   # k + 1 > 256: no stores
+    # above 8192 frames, or not even two atoms' rows: coordinates only
+    assert ksg.plan_layout(247, 8193, 100, 10**12) == {"slots": 0, "wave_atoms": 0, "stores": 0}
+    assert ksg.plan_layout(10, 600, 10, 10**6) == {"slots": 0, "wave_atoms": 0, "stores": 0}
+    with pytest.raises(_lib.GmxcorrError):
+        ksg.plan_layout(1, 600, 10, 10**9)
