@@ -101,6 +101,7 @@ C_SYMBOLS = [
     "gmxcorr_ksg_plan_stats", "gmxcorr_ksg_plan_destroy", "gmxcorr_ksg_plan_set_variant",
     "gmxcorr_ksg_variant_fits", "gmxcorr_ksg_plan_layout", "gmxcorr_ksg_plan_create_budget",
     "gmxcorr_ksg_plan_set_shape", "gmxcorr_ksg_pairs_device",
+    "gmxcorr_ksg_pairs_device_noalloc",
     "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix",
     "gmxcorr_te_matrix", "gmxcorr_te_pairs",
     "gmxcorr_ksg_batch_create", "gmxcorr_ksg_batch_add", "gmxcorr_ksg_batch_run",
@@ -144,6 +145,7 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_ksg_pairs_device.argtypes = [ctypes.c_void_p, _i, _i, ctypes.c_void_p, _i, _ll,
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_void_p]
+    L.gmxcorr_ksg_pairs_device_noalloc.argtypes = L.gmxcorr_ksg_pairs_device.argtypes
     L.gmxcorr_ksg_variant_fits.argtypes = [_i, _i, _i, _i]
     L.gmxcorr_gauss_matrix.argtypes = [_dp, _i, _i, _dp, _dp, ctypes.POINTER(DenseStats)]
     L.gmxcorr_gcmi_matrix.argtypes = [_dp, _i, _i, _dp, _dp, _dp, ctypes.POINTER(DenseStats)]

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -951,6 +952,70 @@ int gmxcorr_ksg_pairs_device(const double* xrows, int natoms, int nframes, const
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of gmxcorr_ksg_pairs_device.  Its scratch (the
+// coordinate path's counters and per-range partial sums) is this file-scope
+// device array, one copy per GPU, held until the process ends.  Host code
+// never passes the array's name to a kernel: its device address is looked
+// up once per device with hipGetSymbolAddress and kept in g_scratch_dev.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchBytes = 32u << 20;   // 32 MiB
+constexpr size_t kScratchHead = 256;          // counters[4] at the start
+constexpr long long kScratchPartials = (long long)((kScratchBytes - kScratchHead) / sizeof(double));
+constexpr int kScratchMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+void* g_scratch_dev[kScratchMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (looked up once).
+int scratch_address(void** out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchMaxDevices) return fail(GMXCORR_EUNSUP, "device index above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(GMXCORR_EHIP, "hipGetSymbolAddress(scratch) gave a null address");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return GMXCORR_OK;
+}
+}  // namespace
+
+extern "C" int gmxcorr_ksg_pairs_device_noalloc(const double* xrows, int natoms, int nframes,
+                                                const double* psi, int k, long long npairs,
+                                                const int* a1, const int* a2, double* mi,
+                                                void* stream)
+{
+    if (!xrows || !psi || !a1 || !a2 || !mi)
+        return fail(GMXCORR_EINVAL, "pairs_device_noalloc: null argument");
+    if (natoms < 2 || k < 1 || nframes <= k || npairs < 1)
+        return fail(GMXCORR_EINVAL,
+                    "pairs_device_noalloc: need natoms >= 2, 1 <= k < nframes, npairs >= 1");
+    if (npairs > kScratchPartials)
+        return fail(GMXCORR_EINVAL, "pairs_device_noalloc: npairs above the scratch maximum of " +
+                                        std::to_string(kScratchPartials) + " pairs");
+    if (!device_is_gfx950()) return fail(GMXCORR_ENODEV, "no gfx950 device");
+    void* base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    // the allocating entry point's row ranges when they fit, else fewer
+    int nsplit = gmxcorr::coord_split(npairs, nframes);
+    if (npairs * nsplit > kScratchPartials) nsplit = (int)(kScratchPartials / npairs);
+    unsigned* counters = reinterpret_cast<unsigned*>(base);
+    double* part = reinterpret_cast<double*>(static_cast<unsigned char*>(base) + kScratchHead);
+    HIPCHK(gmxcorr::launch_ksg_coord_pairs(xrows, nframes, psi, k, npairs, a1, a2, nsplit, part,
+                                           counters, mi, (hipStream_t)stream));
+    return GMXCORR_OK;
+}
 
 // ---------------------------------------------------------------------------
 // The reference's C++ boundary (lib/kraskov_gpu.h:20,32).

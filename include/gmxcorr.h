@@ -139,6 +139,20 @@ int gmxcorr_ksg_pairs_device(const double* xrows, int natoms, int nframes, const
                              int k, long long npairs, const int* a1, const int* a2, double* mi,
                              void* stream);
 
+/* Allocation-free variant of gmxcorr_ksg_pairs_device.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results; it calls no hipMalloc/hipFree and
+ * does not synchronise: the work is enqueued on `stream` and mi is valid once
+ * the stream reaches it.  Scratch is a 32 MiB device array inside the library,
+ * one per GPU, held until the process ends, so npairs is limited to 4,194,272
+ * (larger lists are rejected with GMXCORR_EINVAL).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams or host
+ * threads) race and must be serialised by the caller. */
+int gmxcorr_ksg_pairs_device_noalloc(const double* xrows, int natoms, int nframes,
+                                     const double* psi, int k, long long npairs, const int* a1,
+                                     const int* a2, double* mi, void* stream);
+
 /* ---- Gaussian and Gaussian-copula matrices --------------------------------
  * traj: mean-centred fluctuations [natoms][nframes][3] (t_traj.x flattened).
  * mat:  [natoms*natoms], diagonal 2000, symmetric MI in nats.
