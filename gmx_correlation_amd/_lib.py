@@ -102,7 +102,7 @@ C_SYMBOLS = [
     "gmxcorr_ksg_variant_fits", "gmxcorr_ksg_plan_layout", "gmxcorr_ksg_plan_create_budget",
     "gmxcorr_ksg_plan_set_shape", "gmxcorr_ksg_pairs_device",
     "gmxcorr_ksg_pairs_device_noalloc",
-    "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix",
+    "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix", "gmxcorr_dense_set_slab_frames",
     "gmxcorr_te_matrix", "gmxcorr_te_pairs",
     "gmxcorr_ksg_batch_create", "gmxcorr_ksg_batch_add", "gmxcorr_ksg_batch_run",
     "gmxcorr_ksg_batch_traj_bytes",
@@ -147,6 +147,7 @@ def load() -> ctypes.CDLL:
                                            ctypes.c_void_p]
     L.gmxcorr_ksg_pairs_device_noalloc.argtypes = L.gmxcorr_ksg_pairs_device.argtypes
     L.gmxcorr_ksg_variant_fits.argtypes = [_i, _i, _i, _i]
+    L.gmxcorr_dense_set_slab_frames.argtypes = [_i]
     L.gmxcorr_gauss_matrix.argtypes = [_dp, _i, _i, _dp, _dp, ctypes.POINTER(DenseStats)]
     L.gmxcorr_gcmi_matrix.argtypes = [_dp, _i, _i, _dp, _dp, _dp, ctypes.POINTER(DenseStats)]
     _fp = ctypes.POINTER(ctypes.c_float)

@@ -485,7 +485,8 @@ int dense_cov_splits(int M, int Kp)
 }
 
 hipError_t launch_dense_cov(const _Float16* hi, const _Float16* lo, int M, int Kp,
-                            float out_scale, float* Cp, int nsplit, float* C, hipStream_t s)
+                            float out_scale, float* Cp, int nsplit, float* C, hipStream_t s,
+                            int nreduce)
 {
     if (Kp % kGStep != 0 || nsplit < 1 || Kp / kGStep < nsplit ||
         (long long)M * Kp >= (1LL << 31))
@@ -494,9 +495,14 @@ hipError_t launch_dense_cov(const _Float16* hi, const _Float16* lo, int M, int K
     const unsigned nblk = (unsigned)(ntile * (ntile + 1) / 2 * nsplit);
     hipLaunchKernelGGL(dense_cov_tile_kernel, dim3(nblk), dim3(256), 0, s, hi, lo, M, Kp, ntile,
                        nsplit, out_scale, Cp);
-    if (nsplit == 1 && Cp == C) return hipGetLastError();
+    if (nreduce == 0 || (nsplit == 1 && Cp == C)) return hipGetLastError();
+    return launch_dense_reduce(Cp, M, nreduce < 0 ? nsplit : nreduce, C, s);
+}
+
+hipError_t launch_dense_reduce(const float* Cp, int M, int nparts, float* C, hipStream_t s)
+{
     const long long n = (long long)M * M;
-    hipLaunchKernelGGL(dense_cov_reduce_kernel, dim3(1024), dim3(256), 0, s, Cp, n, nsplit, C);
+    hipLaunchKernelGGL(dense_cov_reduce_kernel, dim3(1024), dim3(256), 0, s, Cp, n, nparts, C);
     return hipGetLastError();
 }
 
@@ -511,7 +517,12 @@ hipError_t launch_dense_pairs(const float* C, int natoms, int mode, double* mat,
 hipError_t launch_dense_copula(const double* V, int M, int N, long long ldv, double* Z,
                                long long ldz, hipStream_t s)
 {
-    if (N > kCopulaMaxFrames) return hipErrorInvalidValue;
+    if (N > kCopulaMaxFrames) {
+        // the row does not fit in LDS: the counting kernel reads it from
+        // global memory (L2), O(N^2) per row
+        hipLaunchKernelGGL(dense_copula_kernel, dim3(M), dim3(256), 0, s, V, N, ldv, Z, ldz, 0);
+        return hipGetLastError();
+    }
     if (N <= kCopulaSortFrames) {
         int Mp = 64;
         while (Mp < N) Mp <<= 1;

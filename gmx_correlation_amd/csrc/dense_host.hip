@@ -53,6 +53,10 @@ float ms_between(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// Frames per GEMM slab set by gmxcorr_dense_set_slab_frames (0: the kernel's
+// own limit).
+int g_slab_frames = 0;
+
 // mode 0 Gaussian, 1 GCMI.
 int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* mat,
                  double* cov_out, double* z_out, gmxcorr_dense_stats* st)
@@ -63,11 +67,16 @@ int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* 
         return gmxcorr::set_error(GMXCORR_EINVAL,
                                   "gcmi: need at least 6 frames for a non-singular 6x6 covariance");
     if (nframes < 1) return gmxcorr::set_error(GMXCORR_EINVAL, "need at least 1 frame");
-    if (mode == 1 && nframes > gmxcorr::kCopulaMaxFrames)
-        return gmxcorr::set_error(GMXCORR_EUNSUP, "gcmi: frame count above the copula kernel limit");
     if (!gmxcorr::device_ok()) return gmxcorr::set_error(GMXCORR_ENODEV, "no gfx950 device");
     const int M = 3 * natoms, N = nframes;
-    const int Kp = (N + 63) / 64 * 64;
+    // frame slabs of Ks frames (a multiple of 64) whose M x Ks operands stay
+    // within the GEMM kernel's 32-bit offsets; one slab for most inputs
+    const long long Kall = (N + 63) / 64 * 64;
+    long long Kmax = gmxcorr::kGemmMaxElems / M / 64 * 64;
+    if (g_slab_frames > 0) Kmax = std::min<long long>(Kmax, (g_slab_frames + 63) / 64 * 64);
+    if (Kmax < 64) return gmxcorr::set_error(GMXCORR_EUNSUP, "dense: more than 2^25 atoms");
+    const int Kp = (int)std::min(Kall, Kmax);
+    const int nslab = (int)((Kall + Kp - 1) / Kp);
     const size_t tbytes = (size_t)natoms * N * 3 * sizeof(double);
     DenseBuffers b;
     HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
@@ -79,7 +88,8 @@ int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* 
     HIPCHK(hipMalloc(&b.lo, (size_t)M * Kp * sizeof(_Float16)));
     HIPCHK(hipMalloc(&b.C, (size_t)M * M * sizeof(float)));
     const int nsplit = gmxcorr::dense_cov_splits(M, Kp);
-    if (nsplit > 1) HIPCHK(hipMalloc(&b.Cp, (size_t)nsplit * M * M * sizeof(float)));
+    if (nsplit * nslab > 1)
+        HIPCHK(hipMalloc(&b.Cp, (size_t)nsplit * nslab * M * M * sizeof(float)));
     HIPCHK(hipMalloc(&b.mat, (size_t)natoms * natoms * sizeof(double)));
 
     // power-of-two scale putting the largest |value| near 2^10 for FP16
@@ -103,11 +113,33 @@ int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* 
         V = b.z;
     }
     HIPCHK(hipEventRecord(b.ev[2], b.s));
-    HIPCHK(gmxcorr::launch_dense_split(V, M, N, N, Kp, scale, b.hi, b.lo, b.s));
-    HIPCHK(hipEventRecord(b.ev[3], b.s));
-    HIPCHK(gmxcorr::launch_dense_cov(b.hi, b.lo, M, Kp, out_scale, nsplit > 1 ? b.Cp : b.C, nsplit,
-                                     b.C, b.s));
-    HIPCHK(hipEventRecord(b.ev[4], b.s));
+    float split_ms = 0.f, gemm_ms = 0.f;
+    if (nslab == 1) {
+        HIPCHK(gmxcorr::launch_dense_split(V, M, N, N, Kp, scale, b.hi, b.lo, b.s));
+        HIPCHK(hipEventRecord(b.ev[3], b.s));
+        HIPCHK(gmxcorr::launch_dense_cov(b.hi, b.lo, M, Kp, out_scale, nsplit > 1 ? b.Cp : b.C,
+                                         nsplit, b.C, b.s));
+        HIPCHK(hipEventRecord(b.ev[4], b.s));
+    } else {
+        // each slab's split, then its GEMM into its own nsplit partial
+        // matrices; one reduce over all of them
+        for (int sl = 0; sl < nslab; ++sl) {
+            const long long f0 = (long long)sl * Kp;
+            const int nf = (int)std::min<long long>(Kp, N - f0);
+            HIPCHK(hipEventRecord(b.ev[2], b.s));
+            HIPCHK(gmxcorr::launch_dense_split(V + f0, M, nf, N, Kp, scale, b.hi, b.lo, b.s));
+            HIPCHK(hipEventRecord(b.ev[3], b.s));
+            HIPCHK(gmxcorr::launch_dense_cov(b.hi, b.lo, M, Kp, out_scale,
+                                             b.Cp + (size_t)sl * nsplit * M * M, nsplit, b.C, b.s,
+                                             0));
+            HIPCHK(hipEventRecord(b.ev[4], b.s));
+            HIPCHK(hipEventSynchronize(b.ev[4]));
+            split_ms += ms_between(b.ev[2], b.ev[3]);
+            gemm_ms += ms_between(b.ev[3], b.ev[4]);
+        }
+        HIPCHK(gmxcorr::launch_dense_reduce(b.Cp, M, nsplit * nslab, b.C, b.s));
+        HIPCHK(hipEventRecord(b.ev[4], b.s));
+    }
     HIPCHK(gmxcorr::launch_dense_pairs(b.C, natoms, mode, b.mat, b.s));
     HIPCHK(hipEventRecord(b.ev[5], b.s));
     HIPCHK(hipMemcpyAsync(mat, b.mat, (size_t)natoms * natoms * sizeof(double),
@@ -123,8 +155,8 @@ int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* 
     if (st) {
         st->ms_load = ms_between(b.ev[0], b.ev[1]);
         st->ms_copula = ms_between(b.ev[1], b.ev[2]);
-        st->ms_split = ms_between(b.ev[2], b.ev[3]);
-        st->ms_gemm = ms_between(b.ev[3], b.ev[4]);
+        st->ms_split = nslab == 1 ? ms_between(b.ev[2], b.ev[3]) : split_ms;
+        st->ms_gemm = nslab == 1 ? ms_between(b.ev[3], b.ev[4]) : gemm_ms;
         st->ms_pairs = ms_between(b.ev[4], b.ev[5]);
         st->gemm_flops = 2.0 * M * (double)M * N;
         st->natoms = natoms;
@@ -136,6 +168,13 @@ int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* 
 }  // namespace
 
 extern "C" {
+
+int gmxcorr_dense_set_slab_frames(int frames)
+{
+    if (frames < 0) return gmxcorr::set_error(GMXCORR_EINVAL, "slab frames must be >= 0");
+    g_slab_frames = frames;
+    return GMXCORR_OK;
+}
 
 int gmxcorr_gauss_matrix(const double* traj, int natoms, int nframes, double* mat,
                          double* cov_out, gmxcorr_dense_stats* st)

@@ -170,12 +170,19 @@ typedef struct {
     int natoms, nframes;
 } gmxcorr_dense_stats;
 
+/* Frames per slab of the second-moment GEMM (0, the default: as many as the
+ * kernel's 32-bit operand offsets allow, 2^31 / (3 natoms)).  Longer
+ * trajectories run slab by slab and the slabs' partial matrices are summed;
+ * smaller slabs bound the FP16 operand buffers.  Process-wide setting. */
+int gmxcorr_dense_set_slab_frames(int frames);
+
 /* lib/correlation_core.cpp:280-311 (gauss_corrmatrix). */
 int gmxcorr_gauss_matrix(const double* traj, int natoms, int nframes, double* mat,
                          double* cov_out, gmxcorr_dense_stats* st);
 /* lib/gcmi.cpp:246-317 (gcmi_corrmatrix); the reference's GPU signature for
  * this estimator is gcmi_corrmatrix_metal (lib/gcmi_metal.h), which takes the
- * copula scores; here the transform runs on the GPU too.  nframes >= 6. */
+ * copula scores; here the transform runs on the GPU too.  nframes >= 6, no
+ * upper limit (rows above 19,456 frames are ranked from global memory). */
 int gmxcorr_gcmi_matrix(const double* traj, int natoms, int nframes, double* mat,
                         double* cov_out, double* z_out, gmxcorr_dense_stats* st);
 
