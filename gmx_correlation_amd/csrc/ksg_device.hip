@@ -347,101 +347,268 @@ __device__ __forceinline__ void clear_hist(uint32_t* hist, int lane)
 // Per-atom distance rows: D[a][i][j] = fl32(max_d |X[3a+d][i] - X[3a+d][j]|)
 // computed in FP64 exactly as the reference does (fabs of the difference,
 // lib/kraskov.cpp:131-132, 280-281), then rounded once.  j in [N, P) = 3072.
-// One block per row (a, i).  With SD != nullptr (N <= kSortedMaxFrames) the
-// row's (value, frame) pairs are also sorted in LDS (bitonic, ties in
-// ascending frame order) and written as
+// One block per row (a, i); used where no sorted rows are kept (N >
+// kSortedMaxFrames).  With sorted rows, ksg_build_sorted_kernel below writes
+// the same D and also
 //   SD[a][i][0..64)        0 (front pad)
 //   SD[a][i][64 + t]       t-th smallest value, t < N; 3072 up to the pitch
 //   SM[a][i][m]            SD value t = 64m + 63 (3072 past N), m < 64
 //   SI[a][i][t]            frame of the t-th smallest value, t < kPrefix (0 past N)
 // so that #{j : D[a][i][j] <= x} is found with two 64-lane loads, and the
-// kPrefix nearest frames of i are known in order.
+// kPrefix nearest frames of i are known in order (ties in frame order).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __restrict__ X, int N,
                                                              long long P, long long nrows,
-                                                             float* __restrict__ D,
-                                                             float* __restrict__ SD,
-                                                             float* __restrict__ SM,
-                                                             uint16_t* __restrict__ SI)
+                                                             float* __restrict__ D)
 {
-    __shared__ uint64_t sv[kSortedMaxFrames];   // (value bits << 32) | frame
     // grid-stride over rows: a launch's work-item count must stay below 2^32
     for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {
-    const int a = (int)(row / N), i = (int)(row % N);
-    const double* x0 = X + (size_t)(3 * a) * N;
-    const double* x1 = x0 + N;
-    const double* x2 = x1 + N;
-    const double q0 = x0[i], q1 = x1[i], q2 = x2[i];
-    float4* out = reinterpret_cast<float4*>(D + row * P);
-    const int nc = (int)(P / 4);
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-        float v[4];
+        const int a = (int)(row / N), i = (int)(row % N);
+        const double* x0 = X + (size_t)(3 * a) * N;
+        const double* x1 = x0 + N;
+        const double* x2 = x1 + N;
+        const double q0 = x0[i], q1 = x1[i], q2 = x2[i];
+        float4* out = reinterpret_cast<float4*>(D + row * P);
+        const int nc = (int)(P / 4);
+        for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+            float v[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int j = 4 * c + e;
+            for (int e = 0; e < 4; ++e) {
+                const int j = 4 * c + e;
+                if (j < N) {
+                    double d = fabs(q0 - x0[j]);
+                    d = fmax(d, fabs(q1 - x1[j]));
+                    d = fmax(d, fabs(q2 - x2[j]));
+                    v[e] = (float)d;
+                } else {
+                    v[e] = __int_as_float((int)kPadBits);
+                }
+            }
+            out[c] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sorted-row build (N <= kSortedMaxFrames): one block of 256 threads per row
+// (a, i).  Thread t computes frames E*t .. E*t+E-1 of the row (M = 256*E >=
+// N, a power of two; frames >= N hold the padding 3072) and keeps them in
+// registers, so:
+//   - D[a][i][j] (j < P) is written with vector stores;
+//   - the row's 32-bit values are bitonic-sorted in registers: strides below
+//     E inside a thread (min/max), strides up to 64E across the wave's lanes
+//     (ds_bpermute), the larger ones through LDS;
+//   - the kPrefix smallest (value, frame) keys, ties in frame order, are
+//     selected below the kPrefix-th value T (frames at T taken in frame
+//     order by a block scan) and sorted as 64-bit keys in LDS for SI.
+// Outputs as described at ksg_build_rows_kernel.  Sorting values alone is
+// min/max work; only the prefix needs its frames.
+// ---------------------------------------------------------------------------
+
+// Exchange partner's value across lanes lane ^ m (m < 64).
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int lane, int m)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ m) << 2, (int)v);
+}
+
+// In-thread bitonic step of stride J (J < E): pairs (e, e + J) with bit J of
+// e clear; ascending where bit k of the global index is clear.
+template <int E, int J>
+__device__ __forceinline__ void sort_step_inthread(uint32_t (&r)[E], int base, int k)
+{
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (e & J) continue;
+        const uint32_t a = r[e], b = r[e + J];
+        const uint32_t lo = min(a, b), hi = max(a, b);
+        const bool asc = ((base + e) & k) == 0;
+        r[e] = asc ? lo : hi;
+        r[e + J] = asc ? hi : lo;
+    }
+}
+
+// Ascending sort of the M = 256*E values held E per thread (thread t owns
+// positions E*t .. E*t+E-1).  lds: M words, used for strides >= 64E.
+template <int E>
+__device__ __forceinline__ void sort_u32_blocked(uint32_t (&r)[E], uint32_t* lds, int t)
+{
+    constexpr int M = 256 * E;
+    const int lane = t & (kWave - 1);
+    const int base = E * t;
+    for (int k = 2; k <= M; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < E) {
+                if constexpr (E > 4) { if (j == 4) { sort_step_inthread<E, (E > 4 ? 4 : 1)>(r, base, k); continue; } }
+                if constexpr (E > 2) { if (j == 2) { sort_step_inthread<E, (E > 2 ? 2 : 1)>(r, base, k); continue; } }
+                if constexpr (E > 1) sort_step_inthread<E, 1>(r, base, k);
+                continue;
+            }
+            const int m = j / E;   // thread distance of the partner
+            if (m < kWave) {
+                const bool upper = (lane & m) != 0;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const uint32_t o = lane_xor(r[e], lane, m);
+                    const bool asc = ((base + e) & k) == 0;
+                    r[e] = (asc != upper) ? min(r[e], o) : max(r[e], o);
+                }
+            } else {
+                __syncthreads();   // earlier readers of lds are done
+#pragma unroll
+                for (int e = 0; e < E; ++e) lds[base + e] = r[e];
+                __syncthreads();
+                const bool upper = (t & m) != 0;
+                const int pb = E * (t ^ m);
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const uint32_t o = lds[pb + e];
+                    const bool asc = ((base + e) & k) == 0;
+                    r[e] = (asc != upper) ? min(r[e], o) : max(r[e], o);
+                }
+            }
+        }
+    }
+}
+
+// Exclusive prefix sum over the block's 256 threads (4 waves); total out.
+__device__ __forceinline__ int block_excl_scan(int v, int* wtot, int t, int& total)
+{
+    const int lane = t & (kWave - 1), w = t / kWave;
+    const int incl = wave_incl_scan(v);
+    if (lane == kWave - 1) wtot[w] = incl;
+    __syncthreads();
+    int off = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) off += q < w ? wtot[q] : 0;
+    total = (wtot[0] + wtot[1]) + (wtot[2] + wtot[3]);
+    __syncthreads();   // wtot is reused by the next scan
+    return off + incl - v;
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void ksg_build_sorted_kernel(const double* __restrict__ X,
+                                                               int N, long long P,
+                                                               long long nrows,
+                                                               float* __restrict__ D,
+                                                               float* __restrict__ SD,
+                                                               float* __restrict__ SM,
+                                                               uint16_t* __restrict__ SI)
+{
+    constexpr int M = 256 * E;
+    __shared__ uint32_t lds[M];
+    __shared__ uint64_t sp[kPrefix];
+    __shared__ int wtot[4];
+    __shared__ uint32_t s_thr;
+    const int t = threadIdx.x;
+    const int base = E * t;
+    const long long ps = sorted_pitch(N);
+    const int nP = N < kPrefix ? N : kPrefix;
+    for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {   // uniform
+        const int a = (int)(row / N), i = (int)(row % N);
+        const double* x0 = X + (size_t)(3 * a) * N;
+        const double* x1 = x0 + N;
+        const double* x2 = x1 + N;
+        const double q0 = x0[i], q1 = x1[i], q2 = x2[i];
+        uint32_t raw[E], r[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int j = base + e;
+            uint32_t v = kPadBits;
             if (j < N) {
                 double d = fabs(q0 - x0[j]);
                 d = fmax(d, fabs(q1 - x1[j]));
                 d = fmax(d, fabs(q2 - x2[j]));
-                v[e] = (float)d;
-            } else {
-                v[e] = __int_as_float((int)kPadBits);
+                v = __float_as_uint((float)d);
             }
+            raw[e] = v;
+            r[e] = v;
         }
-        out[c] = make_float4(v[0], v[1], v[2], v[3]);
-        if (SD) {
+        // distance row, pitch P <= M
+        uint32_t* drow = reinterpret_cast<uint32_t*>(D + row * P);
+        if constexpr (E >= 4) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (4 * c + e < kSortedMaxFrames)
-                    sv[4 * c + e] = ((uint64_t)__float_as_uint(v[e]) << 32) | (uint32_t)(4 * c + e);
+            for (int e = 0; e < E; e += 4)
+                if (base + e < P)
+                    *reinterpret_cast<uint4*>(drow + base + e) =
+                        make_uint4(raw[e], raw[e + 1], raw[e + 2], raw[e + 3]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (base + e < P) drow[base + e] = raw[e];
         }
-    }
-    if (!SD) continue;
-    int M = 64;
-    while (M < N) M <<= 1;
-    for (int t = nc * 4 + threadIdx.x; t < M; t += blockDim.x)
-        sv[t] = ((uint64_t)kPadBits << 32) | (uint32_t)t;
-    __syncthreads();
-    // bitonic sort of sv[0..M), ascending (non-negative float bits, then frame)
-    // Exchange t pairs keys lo = 2t - (t & (j - 1)) and lo + j; for j <= 64
-    // both lie among 128 keys owned by t's wave (blockDim a multiple of 64),
-    // so those passes need the wave's own in-order LDS accesses and no
-    // block barrier.
-    int pj = 0;
-    for (int k = 2; k <= M; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 128 || pj >= 128) {
-                __syncthreads();
-            } else {
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                __builtin_amdgcn_wave_barrier();
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        sort_u32_blocked<E>(r, lds, t);
+        // sorted row: 64 zeros, the N values, padding to the pitch
+        uint32_t* srow = reinterpret_cast<uint32_t*>(SD) + row * ps;
+        if (t < 64) srow[t] = 0u;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int q = base + e;
+            if (64 + q < ps) srow[64 + q] = q < N ? r[e] : kPadBits;
+        }
+        for (long long q = 64 + M + t; q < ps; q += 256) srow[q] = kPadBits;
+        // summary: value 64m + 63 (owned by the thread whose last slot it is)
+        uint32_t* sm = reinterpret_cast<uint32_t*>(SM) + row * kSumPitch;
+        if ((base + E - 1) % 64 == 63) {
+            const int q = base + E - 1;
+            sm[q / 64] = q < N ? r[E - 1] : kPadBits;
+        }
+        for (int m = M / 64 + t; m < kSumPitch; m += 256) sm[m] = kPadBits;
+        // the kPrefix-th smallest value T
+        if (base <= nP - 1 && nP - 1 < base + E) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (base + e == nP - 1) s_thr = r[e];
+        }
+        __syncthreads();
+        const uint32_t T = s_thr;
+        // frames below T, and the first frames at T in frame order
+        int cl = 0, ce = 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const bool real = base + e < N;
+            cl += real && raw[e] < T;
+            ce += real && raw[e] == T;
+        }
+        int LT, EQ;
+        int pl = block_excl_scan(cl, wtot, t, LT);
+        int pe = block_excl_scan(ce, wtot, t, EQ);
+        const int need_eq = nP - LT;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int j = base + e;
+            if (j >= N) continue;
+            const uint64_t key = ((uint64_t)raw[e] << 32) | (uint32_t)j;
+            if (raw[e] < T) {
+                sp[pl++] = key;
+            } else if (raw[e] == T) {
+                if (pe < need_eq) sp[LT + pe] = key;
+                ++pe;
             }
-            for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
-                const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
-                const uint64_t x = sv[lo], y = sv[hi];
-                if ((x > y) == ((lo & k) == 0)) {
-                    sv[lo] = y;
-                    sv[hi] = x;
+        }
+        for (int q = nP + t; q < kPrefix; q += 256) sp[q] = ~0ull;
+        __syncthreads();
+        // sort the kPrefix keys (one exchange per thread per pass); passes
+        // with j <= 64 stay within the wave's 128 keys
+        int pj = 0;
+        for (int k = 2; k <= kPrefix; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                if (j >= 128 || pj >= 128) {
+                    __syncthreads();
+                } else {
+                    lds_order();
                 }
+                const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
+                const uint64_t x = sp[lo], y = sp[hi];
+                if ((x > y) == ((lo & k) == 0)) {
+                    sp[lo] = y;
+                    sp[hi] = x;
+                }
+                pj = j;
             }
-            pj = j;
-        }
-    __syncthreads();
-    const long long ps = sorted_pitch(N);
-    uint32_t* srow = reinterpret_cast<uint32_t*>(SD) + row * ps;
-    for (int t = threadIdx.x; t < ps; t += blockDim.x) {
-        const int q = t - 64;
-        srow[t] = q < 0 ? 0u : q < N ? (uint32_t)(sv[q] >> 32) : kPadBits;
-    }
-    if (threadIdx.x < kSumPitch) {
-        const int q = 64 * threadIdx.x + 63;
-        reinterpret_cast<uint32_t*>(SM)[row * kSumPitch + threadIdx.x] =
-            q < N ? (uint32_t)(sv[q] >> 32) : kPadBits;
-    }
-    for (int t = threadIdx.x; t < kPrefix; t += blockDim.x)
-        SI[row * kPrefix + t] = t < N ? (uint16_t)(sv[t] & 0xffffu) : (uint16_t)0;
-    __syncthreads();   // sv is rewritten by the next row
+        __syncthreads();
+        for (int q = t; q < kPrefix; q += 256)
+            SI[row * kPrefix + q] = q < N ? (uint16_t)(sp[q] & 0xffffu) : (uint16_t)0;
+        __syncthreads();   // sp, lds and s_thr are rewritten by the next row
     }
 }
 
@@ -2306,8 +2473,25 @@ hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, fl
         (SD && N > kSortedMaxFrames))
         return hipErrorInvalidValue;
     const long long rows = (long long)natoms * N;
-    hipLaunchKernelGGL(ksg_build_rows_kernel, dim3(capped_grid(rows)), dim3(256), 0, s, X, N, P,
-                       rows, D, SD, SM, SI);
+    if (!SD) {
+        hipLaunchKernelGGL(ksg_build_rows_kernel, dim3(capped_grid(rows)), dim3(256), 0, s, X, N,
+                           P, rows, D);
+        return hipGetLastError();
+    }
+    // sorted rows: E frames per thread, 256 * E >= N
+    const unsigned g = capped_grid(rows);
+    if (N <= 256)
+        hipLaunchKernelGGL(ksg_build_sorted_kernel<1>, dim3(g), dim3(256), 0, s, X, N, P, rows, D,
+                           SD, SM, SI);
+    else if (N <= 512)
+        hipLaunchKernelGGL(ksg_build_sorted_kernel<2>, dim3(g), dim3(256), 0, s, X, N, P, rows, D,
+                           SD, SM, SI);
+    else if (N <= 1024)
+        hipLaunchKernelGGL(ksg_build_sorted_kernel<4>, dim3(g), dim3(256), 0, s, X, N, P, rows, D,
+                           SD, SM, SI);
+    else
+        hipLaunchKernelGGL(ksg_build_sorted_kernel<8>, dim3(g), dim3(256), 0, s, X, N, P, rows, D,
+                           SD, SM, SI);
     return hipGetLastError();
 }
 

@@ -216,6 +216,8 @@ struct gmxcorr_ksg_plan {
     int nsplit = 0;          // row ranges per pair (0: default for the pair count)
     int chunk = gmxcorr::kLongChunk;
     gmxcorr::TrajDesc* dTraj = nullptr;   // one descriptor for the recount/finalise kernels
+    unsigned char* hStage = nullptr;      // pinned staging for the upload: two halves
+    hipEvent_t stage_ev[2] = {};          // copy of each half done
     gmxcorr_ksg_stats st{};
 };
 
@@ -408,6 +410,28 @@ void reset_run_stats(gmxcorr_ksg_plan* pl)
     pl->st.ms_pairs = pl->st.ms_fixup = 0.0;
     pl->st.npairs = pl->st.fixup_rows = pl->st.long_rows = pl->st.ties = 0;
     pl->st.waves = 0;
+}
+
+// Pinned staging for uploads: kStageBytes per half.  Copying pageable host
+// memory straight to the device runs at about 1.3 GB/s here; through two
+// alternating pinned halves the host copy of one chunk overlaps the DMA of
+// the previous one.
+constexpr size_t kStageBytes = 4u << 20;
+
+int upload(gmxcorr_ksg_plan* pl, void* dst, const void* src, size_t bytes)
+{
+    const unsigned char* s = static_cast<const unsigned char*>(src);
+    unsigned char* d = static_cast<unsigned char*>(dst);
+    int half = 0;
+    for (size_t off = 0; off < bytes; off += kStageBytes, half ^= 1) {
+        const size_t n = std::min(kStageBytes, bytes - off);
+        unsigned char* st = pl->hStage + half * kStageBytes;
+        HIPCHK(hipEventSynchronize(pl->stage_ev[half]));   // this half's last DMA is done
+        std::memcpy(st, s + off, n);
+        HIPCHK(hipMemcpyAsync(d + off, st, n, hipMemcpyHostToDevice, pl->stream));
+        HIPCHK(hipEventRecord(pl->stage_ev[half], pl->stream));
+    }
+    return GMXCORR_OK;
 }
 
 // Builds the rows of `nat` atoms whose coordinates sit at slot s0 of dXs.
@@ -637,6 +661,14 @@ int gmxcorr_ksg_plan_create_budget(int natoms, int nframes, int k, long long max
         return bail(hip_fail(e, "hipMalloc(counters)"));
     if ((e = hipMalloc(&pl->dTraj, sizeof(gmxcorr::TrajDesc))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(descriptor)"));
+    if ((e = hipHostMalloc((void**)&pl->hStage, 2 * kStageBytes, hipHostMallocDefault)) != hipSuccess)
+        return bail(hip_fail(e, "hipHostMalloc(upload staging)"));
+    for (auto& ev : pl->stage_ev) {
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+            return bail(hip_fail(e, "hipEventCreate"));
+        if ((e = hipEventRecord(ev, pl->stream)) != hipSuccess)
+            return bail(hip_fail(e, "hipEventRecord"));
+    }
     // Row budget.  Automatic: free memory less the all-pairs run's pair
     // buffers (ids, recount sum, MI, partial sums) and its two queues, and
     // 2 GB for the rest.
@@ -714,9 +746,8 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
     pl->loaded = false;
     pl->block_in_slot[0] = pl->block_in_slot[1] = -1;
     HIPCHK(hipEventRecord(pl->ev[0], pl->stream));
-    HIPCHK(hipMemcpyAsync(pl->dX, xrows, nx * sizeof(double), hipMemcpyHostToDevice, pl->stream));
-    HIPCHK(hipMemcpyAsync(pl->dPsi, psi, (pl->N + 1) * sizeof(double), hipMemcpyHostToDevice,
-                          pl->stream));
+    if (int rc = upload(pl, pl->dX, xrows, nx * sizeof(double))) return rc;
+    if (int rc = upload(pl, pl->dPsi, psi, (pl->N + 1) * sizeof(double))) return rc;
     HIPCHK(hipEventRecord(pl->ev[1], pl->stream));
     if (pl->slots == pl->natoms)
         if (int rc = build_slots(pl, 0, pl->natoms)) return rc;
@@ -906,6 +937,9 @@ void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* pl)
     dfree(pl->dA2); dfree(pl->dPartial); dfree(pl->dFixsum); dfree(pl->dMI); dfree(pl->dFix);
     dfree(pl->dLong); dfree(pl->dCounters); dfree(pl->dRcNx); dfree(pl->dRcNy);
     dfree(pl->dRcFlag); dfree(pl->dTraj);
+    if (pl->hStage) (void)hipHostFree(pl->hStage);
+    for (auto& e : pl->stage_ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto& ev : pl->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (pl->stream) (void)hipStreamDestroy(pl->stream);

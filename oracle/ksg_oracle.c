@@ -418,6 +418,265 @@ void oracle_ksg_pairs(const double *kx, int N, int K, const double *psi, int npa
         out[p] = oracle_ksg_pair_grid(kx, N, a1[p], a2[p], K, psi, NULL, NULL, NULL, NULL, NULL);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Timed CPU baseline: the same KSG-2 pair with the reference's memory       */
+/* layout and allocation pattern                                            */
+/* ------------------------------------------------------------------------ */
+
+/* The grid restatement above groups frame indices by cell and reads the
+ * coordinates in place; it runs about twice as fast as the reference's own
+ * code.  The baseline below restates the reference's cost structure as well
+ * as its arithmetic, so that its time stands for the reference's:
+ *   - each pair copies its six rows, and each marginal its three, into fresh
+ *     allocations (lib/kraskov.cpp:2253-2262, 1282-1287);
+ *   - each grid physically reorders its copy into cell order and chains the
+ *     points of a cell in a linked list, built, reordered and rebuilt as in
+ *     lib/kraskov.cpp:56-91; cells are rows of a pointer array;
+ *   - every neighbour query allocates its scratch (lib/kraskov.cpp:264-268,
+ *     120-121), and the marginal count stops evaluating components once one
+ *     exceeds the radius (lib/kraskov.cpp:130-134).
+ * The traversal order, and hence the neighbour set and the digamma sum, is
+ * the grid restatement's: both return the same MI bit for bit. */
+
+typedef struct {
+    int bs;
+    int **head;   /* [bs][bs] first point of each cell, -1 when empty */
+    int *next;    /* [N] next point of the same cell, -1 at the end */
+} listgrid;
+
+static int lg_cell(double v, int inveps, int mask) { return ((int)(v * inveps)) & mask; }
+
+/* Reorders x[0..dim) in place into cell order of components c1, c2 and
+ * builds the lists; ind[orig] = new index. */
+static void lg_build(double **x, int dim, int N, int c1, int c2, int bs, int inveps, int *ind,
+                     listgrid *g)
+{
+    const int mask = bs - 1;
+    g->bs = bs;
+    g->head = (int **)calloc((size_t)bs, sizeof(int *));
+    for (int r = 0; r < bs; ++r) g->head[r] = (int *)calloc((size_t)bs, sizeof(int));
+    g->next = (int *)calloc((size_t)N, sizeof(int));
+    double **tmp = (double **)calloc((size_t)dim, sizeof(double *));
+    for (int d = 0; d < dim; ++d) {
+        tmp[d] = (double *)calloc((size_t)N, sizeof(double));
+        memcpy(tmp[d], x[d], (size_t)N * sizeof(double));
+    }
+    for (int r = 0; r < bs; ++r)
+        for (int c = 0; c < bs; ++c) g->head[r][c] = -1;
+    for (int i = 0; i < N; ++i) {
+        const int cx = lg_cell(x[c1][i], inveps, mask), cy = lg_cell(x[c2][i], inveps, mask);
+        g->next[i] = g->head[cx][cy];
+        g->head[cx][cy] = i;
+    }
+    int m = -1;
+    for (int cx = 0; cx < bs; ++cx)
+        for (int cy = 0; cy < bs; ++cy) {
+            for (int el = g->head[cx][cy]; el >= 0; el = g->next[el]) {
+                ++m;
+                for (int d = 0; d < dim; ++d) x[d][m] = tmp[d][el];
+                ind[el] = m;
+            }
+            g->head[cx][cy] = -1;
+        }
+    for (int i = 0; i < N; ++i) {
+        const int cx = lg_cell(x[c1][i], inveps, mask), cy = lg_cell(x[c2][i], inveps, mask);
+        g->next[i] = g->head[cx][cy];
+        g->head[cx][cy] = i;
+    }
+    for (int d = 0; d < dim; ++d) free(tmp[d]);
+    free(tmp);
+}
+
+static void lg_free(listgrid *g)
+{
+    for (int r = 0; r < g->bs; ++r) free(g->head[r]);
+    free(g->head);
+    free(g->next);
+}
+
+/* K nearest of point q (reordered index) in the max-norm over dim rows. */
+static void lg_knn(double **x, int dim, int c1, int c2, int q, const listgrid *g, double cellw,
+                   int K, int *nn)
+{
+    const int bs = g->bs, mask = bs - 1;
+    double *dn = (double *)calloc((size_t)K + 1, sizeof(double));
+    double *xq = (double *)calloc((size_t)dim, sizeof(double));
+    for (int d = 0; d < dim; ++d) xq[d] = x[d][q];
+    dn[0] = 0.0;
+    for (int k = 1; k <= K; ++k) dn[k] = 1e30;
+    const int ix = ((int)(xq[c1] / cellw)) & mask, iy = ((int)(xq[c2] / cellw)) & mask;
+    /* REPEAT1: the reference's vertical-edge loop evaluates component 1 a
+     * second time (lib/kraskov.cpp:297); the max is unchanged, the work is
+     * kept so that the time is the reference's */
+#define LG_KNN_VISIT(REPEAT1)                                                        \
+    for (int el = g->head[x1_ & mask][y1_ & mask]; el != -1; el = g->next[el]) {     \
+        if (el == q) continue;                                                       \
+        double dd = fabs(xq[0] - x[0][el]);                                          \
+        for (int d = 1; d < dim; ++d) {                                              \
+            const double t = fabs(xq[d] - x[d][el]);                                 \
+            if (t > dd) dd = t;                                                      \
+        }                                                                            \
+        if (REPEAT1) {                                                               \
+            const double t = fabs(xq[1] - x[1][el]);                                 \
+            if (t > dd) dd = t;                                                      \
+        }                                                                            \
+        if (dd < dn[K]) {                                                            \
+            int k = K;                                                               \
+            while (dd < dn[k]) {                                                     \
+                if (k < K) { dn[k + 1] = dn[k]; nn[k + 1] = nn[k]; }                 \
+                k--;                                                                 \
+            }                                                                        \
+            dn[k + 1] = dd;                                                          \
+            nn[k + 1] = el;                                                          \
+        }                                                                            \
+    }
+    int jj = 0;
+    while (dn[K] > cellw * (jj - 1)) {
+        const int step = jj ? 2 * jj : 1;
+        for (int x1_ = ix - jj; x1_ <= ix + jj; ++x1_)
+            for (int y1_ = iy - jj; y1_ <= iy + jj; y1_ += step) { LG_KNN_VISIT(0) }
+        for (int x1_ = ix - jj; x1_ <= ix + jj; x1_ += step)
+            for (int y1_ = iy - jj + 1; y1_ <= iy + jj - 1; ++y1_) { LG_KNN_VISIT(1) }
+        jj++;
+        if (jj == bs / 2) break;
+    }
+    if (jj == bs / 2) {
+        for (int x1_ = ix - jj; x1_ < ix + jj; ++x1_) {
+            const int y1_ = iy - jj;
+            LG_KNN_VISIT(0)
+        }
+        for (int y1_ = iy - jj + 1; y1_ <= iy + jj - 1; ++y1_) {
+            const int x1_ = ix - jj;
+            LG_KNN_VISIT(0)
+        }
+    }
+#undef LG_KNN_VISIT
+    free(dn);
+    free(xq);
+}
+
+/* #{j != q : max-norm distance over dim rows <= eps}, duplicates counted. */
+static int lg_count(double **x, int dim, int c1, int c2, int q, const listgrid *g, double cellw,
+                    double eps)
+{
+    const int bs = g->bs, mask = bs - 1;
+    double *xq = (double *)calloc((size_t)dim, sizeof(double));
+    for (int d = 0; d < dim; ++d) xq[d] = x[d][q];
+    const int ix = ((int)(xq[c1] / cellw)) & mask, iy = ((int)(xq[c2] / cellw)) & mask;
+    int n = 0;
+#define LG_CNT_VISIT                                                                 \
+    for (int el = g->head[x1_ & mask][y1_ & mask]; el != -1; el = g->next[el]) {     \
+        double dd = fabs(xq[0] - x[0][el]);                                          \
+        for (int d = 1; d < dim; ++d) {                                              \
+            const double t = fabs(xq[d] - x[d][el]);                                 \
+            if (t > dd && (dd = t) > eps) break;                                     \
+        }                                                                            \
+        if (dd <= eps) n++;                                                          \
+    }
+    int jj = 0;
+    while (eps > cellw * (jj - 1)) {
+        const int step = jj ? 2 * jj : 1;
+        for (int x1_ = ix - jj; x1_ <= ix + jj; ++x1_)
+            for (int y1_ = iy - jj; y1_ <= iy + jj; y1_ += step) { LG_CNT_VISIT }
+        for (int x1_ = ix - jj; x1_ <= ix + jj; x1_ += step)
+            for (int y1_ = iy - jj + 1; y1_ <= iy + jj - 1; ++y1_) { LG_CNT_VISIT }
+        jj++;
+        if (jj == bs / 2) break;
+    }
+    if (jj == bs / 2) {
+        for (int x1_ = ix - jj; x1_ < ix + jj; ++x1_) {
+            const int y1_ = iy - jj;
+            LG_CNT_VISIT
+        }
+        for (int y1_ = iy - jj + 1; y1_ <= iy + jj - 1; ++y1_) {
+            const int x1_ = ix - jj;
+            LG_CNT_VISIT
+        }
+    }
+#undef LG_CNT_VISIT
+    free(xq);
+    return n - 1;
+}
+
+double oracle_ksg_pair_baseline(const double *kx, int N, int a1, int a2, int K, const double *psi)
+{
+    double **x = (double **)calloc(6, sizeof(double *));
+    for (int d = 0; d < 3; ++d) {
+        x[d] = (double *)calloc((size_t)N, sizeof(double));
+        x[3 + d] = (double *)calloc((size_t)N, sizeof(double));
+        memcpy(x[d], kx + (size_t)(3 * a1 + d) * N, (size_t)N * sizeof(double));
+        memcpy(x[3 + d], kx + (size_t)(3 * a2 + d) * N, (size_t)N * sizeof(double));
+    }
+    double **xx = (double **)calloc(3, sizeof(double *));
+    double **yy = (double **)calloc(3, sizeof(double *));
+    for (int d = 0; d < 3; ++d) {
+        xx[d] = (double *)calloc((size_t)N, sizeof(double));
+        yy[d] = (double *)calloc((size_t)N, sizeof(double));
+        memcpy(xx[d], x[d], (size_t)N * sizeof(double));
+        memcpy(yy[d], x[3 + d], (size_t)N * sizeof(double));
+    }
+    const int bs = grid_side(N, K), inveps = bs / 4;
+    const double cellw = 4.0 / bs;
+    int *ind = (int *)calloc((size_t)N, sizeof(int));
+    int *indx = (int *)calloc((size_t)N, sizeof(int));
+    int *indy = (int *)calloc((size_t)N, sizeof(int));
+    int *nn = (int *)calloc((size_t)K + 1, sizeof(int));
+    listgrid gj, gx, gy;
+    lg_build(x, 6, N, 0, 3, bs, inveps, ind, &gj);
+    lg_build(xx, 3, N, 0, 2, bs, inveps, indx, &gx);
+    lg_build(yy, 3, N, 0, 2, bs, inveps, indy, &gy);
+    double acc = 0.0, xc[6];
+    for (int i = 0; i < N; ++i) {
+        const int q = ind[i];
+        for (int d = 0; d < 6; ++d) xc[d] = x[d][q];
+        lg_knn(x, 6, 0, 3, q, &gj, cellw, K, nn);
+        double ex = 0.0, ey = 0.0;
+        for (int d = 0; d < 3; ++d)
+            for (int k = 1; k <= K; ++k) {
+                const double t = fabs(xc[d] - x[d][nn[k]]);
+                if (t > ex) ex = t;
+            }
+        for (int d = 3; d < 6; ++d)
+            for (int k = 1; k <= K; ++k) {
+                const double t = fabs(xc[d] - x[d][nn[k]]);
+                if (t > ey) ey = t;
+            }
+        const int nx = lg_count(xx, 3, 0, 2, indx[i], &gx, cellw, ex);
+        const int ny = lg_count(yy, 3, 0, 2, indy[i], &gy, cellw, ey);
+        acc += psi[nx] + psi[ny];
+    }
+    acc /= N;
+    lg_free(&gj);
+    lg_free(&gx);
+    lg_free(&gy);
+    free(ind);
+    free(indx);
+    free(indy);
+    free(nn);
+    for (int d = 0; d < 3; ++d) {
+        free(xx[d]);
+        free(yy[d]);
+    }
+    free(xx);
+    free(yy);
+    for (int d = 0; d < 6; ++d) free(x[d]);
+    free(x);
+    return psi[N] + (psi[K] - 1 / (1.0 * K)) - acc;
+}
+
+/* The baseline over a pair list, OpenMP over pairs as the reference's loop
+ * (lib/correlation_core.cpp:403, schedule(dynamic, 1)). */
+void oracle_ksg_pairs_baseline(const double *kx, int N, int K, const double *psi, int npairs,
+                               const int *a1, const int *a2, double *out, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int p = 0; p < npairs; ++p)
+        out[p] = oracle_ksg_pair_baseline(kx, N, a1[p], a2[p], K, psi);
+}
+
 int oracle_max_threads(void)
 {
 #ifdef _OPENMP

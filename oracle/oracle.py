@@ -56,6 +56,7 @@ def lib():
         L.oracle_ksg_pair_brute.restype = ctypes.c_double
         L.oracle_ksg_matrix.argtypes = [_dp, _i, _i, _i, _dp, _dp, _i, _i]
         L.oracle_ksg_pairs.argtypes = [_dp, _i, _i, _dp, _i, _ip, _ip, _dp, _i]
+        L.oracle_ksg_pairs_baseline.argtypes = [_dp, _i, _i, _dp, _i, _ip, _ip, _dp, _i]
         L.oracle_max_threads.restype = _i
         L.oracle_pearsify.argtypes = [_dp, _i, _i]
         L.oracle_lu_det.argtypes = [_dp, _i]
@@ -163,6 +164,18 @@ def ksg_pairs(kx, psi, k, a1, a2, nthreads=0) -> np.ndarray:
     out = np.empty(len(a1))
     lib().oracle_ksg_pairs(_ptr(kx, _dp), kx.shape[1], k, _ptr(psi, _dp), len(a1),
                            _ptr(a1, _ip), _ptr(a2, _ip), _ptr(out, _dp), nthreads)
+    return out
+
+
+def ksg_pairs_baseline(kx, psi, k, a1, a2, nthreads=0) -> np.ndarray:
+    """The timed CPU baseline: same MI as ksg_pairs, with the reference's
+    memory layout and allocation pattern (oracle_ksg_pair_baseline)."""
+    kx = np.ascontiguousarray(kx, dtype=np.float64)
+    a1 = np.ascontiguousarray(a1, dtype=np.int32)
+    a2 = np.ascontiguousarray(a2, dtype=np.int32)
+    out = np.empty(len(a1))
+    lib().oracle_ksg_pairs_baseline(_ptr(kx, _dp), kx.shape[1], k, _ptr(psi, _dp), len(a1),
+                                    _ptr(a1, _ip), _ptr(a2, _ip), _ptr(out, _dp), nthreads)
     return out
 
 

@@ -2,8 +2,8 @@
 
     python oracle/time_vs_reference.py [--threads N] [--pairs P]
 
-The restatement (liboracle.so, oracle/ksg_oracle.c) is the timed CPU
-baseline of bench.py, because the reference-compiled library never leaves
+The baseline restatement (oracle_ksg_pair_baseline in oracle/ksg_oracle.c)
+is the timed CPU baseline of bench.py, because the reference-compiled library never leaves
 this checkout.  For that to be fair it must run the reference's algorithm at
 the reference's speed: both sides get the same pairs of the same workload,
 OpenMP over pairs with the same thread count, and the same compiler flags
@@ -51,7 +51,7 @@ def main() -> int:
             v_ref = O.ref_ksg_pairs(coords, k, a1, a2, nth)
             t_ref.append(time.perf_counter() - t0)
             t0 = time.perf_counter()
-            v_port = O.ksg_pairs(rows, psi, k, a1, a2, nth)
+            v_port = O.ksg_pairs_baseline(rows, psi, k, a1, a2, nth)
             t_port.append(time.perf_counter() - t0)
         print(json.dumps({
             "workload": f"{spec.natoms} atoms x {spec.nframes} frames, k={k}",
