@@ -241,32 +241,55 @@ def bench_hybrid(args) -> int:
         out["plain_pairs_per_s"] = len(mi) / t_plain
         out["plain_kernel_ms"] = st["ms_pairs"]
         log(f"bench: plain {len(mi)} pairs in {t_plain:.1f} s")
-        res = surrogate.hybrid(plan, a1, a2, model, sigma_max=2e-4, audit=0.01, seed=0)
+        res = surrogate.hybrid(plan, a1, a2, model, band=1e-3, audit=0.01, alpha=0.01, seed=0)
         err = np.abs(res["r"] - r_exact)
-        out["hybrid_sigma_max"] = 2e-4
+        kept = res["kept"]
+        out["hybrid_band"] = 1e-3
+        out["hybrid_conformal_alpha"] = 0.01
+        out["hybrid_sigma_threshold"] = (res["sigma_threshold"]
+                                         if np.isfinite(res["sigma_threshold"]) else None)
+        out["hybrid_failed_closed"] = res["failed_closed"]
         out["hybrid_s"] = res["wall_s"]
         out["hybrid_pairs_per_s"] = len(mi) / res["wall_s"]
         out["hybrid_speedup_vs_plain"] = t_plain / res["wall_s"]
         out["hybrid_fallback_fraction"] = res["fallback_fraction"]
         out["hybrid_max_abs_dr_all_pairs"] = float(err.max())
-        out["hybrid_audit_max_abs_dr"] = res["audit_max_abs_dr"]
+        out["hybrid_max_abs_dr_kept_pairs"] = float(err[kept].max()) if kept.any() else 0.0
         out["ms_features"] = res["ms_features"]
         out["ms_inference_bf16"] = res["ms_inference"]
         mu, sigma = res["mu"], res["sigma"]
         surr_err = np.abs(mu - r_exact)
         out["surrogate_rmse_r"] = float(np.sqrt(np.mean(surr_err ** 2)))
-        out["gaussian_r_rmse"] = None
-        # trade-off from the exact r of every pair: kept pairs' worst error,
-        # fallback fraction and the time model t = features + inference + f * t_plain
+        # calibrated trade-off: for each band the conformal sigma threshold
+        # from the same audit's calibration half, the share of non-audit
+        # pairs it keeps, and the kept pairs' error against the exact r of
+        # every pair; time model t = features + inference + (1 - kept) * plain
+        aud = np.flatnonzero(res["audited"])
+        cal = aud[::2]
         t_fixed = (res["ms_features"] + res["ms_inference"]) / 1e3
         curve = []
-        for smax in (2e-4, 1e-3, 3e-3, 1e-2, 3e-2, 1e-1):
-            keep = sigma <= smax
-            f = 1.0 - keep.mean() + 0.01
-            curve.append({"sigma_max": smax, "fallback_fraction": float(min(1.0, f)),
-                          "max_abs_dr": float(surr_err[keep].max()) if keep.any() else 0.0,
-                          "modelled_speedup": t_plain / (t_fixed + min(1.0, f) * t_plain)})
-        out["tradeoff"] = curve
+        for band in (1e-3, 2e-3, 5e-3, 1e-2, 2e-2, 5e-2):
+            tau = surrogate.calibrate_threshold(mu[cal] - r_exact[cal], sigma[cal], band, 0.01)
+            keep_b = (sigma <= tau) & ~res["audited"]
+            f = 1.0 - keep_b.mean()
+            curve.append({
+                "band": band, "sigma_threshold": tau if np.isfinite(tau) else None,
+                "kept_fraction": float(keep_b.mean()),
+                "max_abs_dr_kept": float(surr_err[keep_b].max()) if keep_b.any() else 0.0,
+                "frac_kept_within_band": float(np.mean(surr_err[keep_b] <= band))
+                if keep_b.any() else 1.0,
+                "modelled_speedup": t_plain / (t_fixed + f * t_plain)})
+        out["tradeoff_calibrated"] = curve
+        summary = {k2: out[k2] for k2 in (
+            "infer_pairs", "plain_pairs_per_s", "hybrid_band", "hybrid_speedup_vs_plain",
+            "hybrid_fallback_fraction", "hybrid_max_abs_dr_all_pairs",
+            "hybrid_max_abs_dr_kept_pairs", "hybrid_failed_closed", "surrogate_rmse_r",
+            "tradeoff_calibrated")}
+        summary["measured_by"] = "python bench.py --workload hybrid (train 500x4000, infer " \
+                                 "2000x5000, k=100)"
+        summary["target"] = ">= 5x at max |dr| <= 1e-3: not met"
+        with open(os.path.join(ROOT, "profiles", "hybrid_surrogate_2000x5000.json"), "w") as f2:
+            json.dump(summary, f2, indent=1)
     log(f"bench: hybrid {out}")
     print(json.dumps(out), flush=True)
     return 0

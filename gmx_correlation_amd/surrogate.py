@@ -159,39 +159,105 @@ def _all_pairs_in_order(a1: np.ndarray, a2: np.ndarray, natoms: int) -> bool:
     return bool(np.array_equal(a1, t1) and np.array_equal(a2, t2))
 
 
-def hybrid(plan: KsgPlan, a1, a2, model: Surrogate, sigma_max: float = 2e-4,
-           audit: float = 0.01, seed: int = 0, nsample: int = 32) -> dict:
-    """r(MI) of the pairs: surrogate where its predicted sigma <= sigma_max,
-    exact GPU KSG for the rest and for a random audit fraction.  The audited
-    pairs that the surrogate would have kept measure its error."""
+def calibrate_threshold(resid: np.ndarray, sigma: np.ndarray, band: float, alpha: float) -> float:
+    """Largest sigma threshold tau whose kept calibration pairs meet the band:
+    among the m calibration pairs with sigma <= tau, the
+    ceil((m + 1)(1 - alpha))-th smallest |residual| (the split-conformal
+    quantile of that subset) is at most band.  Calibrating on the subset the
+    gate keeps, not on all pairs, is what makes the bound hold for the kept
+    pairs: a network's sigma is least reliable exactly where it is small.
+    Returns -inf when no threshold qualifies (nothing is kept)."""
+    err = np.abs(np.asarray(resid, np.float64))
+    sig = np.asarray(sigma, np.float64)
+    order = np.argsort(sig, kind="stable")
+    s_sorted, e_sorted = sig[order], err[order]
+    n = len(order)
+    best = 0
+    m_min = int(np.ceil((1.0 - alpha) / alpha))          # smallest m with a finite quantile
+    # the gate keeps every pair with sigma <= tau, so a kept prefix ends at
+    # the last of the pairs sharing its sigma; beyond 8192 such ends, steps
+    # of 0.2% in m (the kept share is resolved to that)
+    ends = np.flatnonzero(np.r_[s_sorted[1:] != s_sorted[:-1], True]) + 1
+    if len(ends) > 8192:
+        grid = np.geomspace(max(m_min, 1), max(n, 1), 4096)
+        ends = np.unique(ends[np.maximum(np.searchsorted(ends, grid, side="right") - 1, 0)])
+    for m in ends:
+        if m < m_min:
+            continue
+        rank = int(np.ceil((m + 1) * (1.0 - alpha)))
+        if rank > m:
+            continue
+        q = np.partition(e_sorted[:m], rank - 1)[rank - 1]
+        if q <= band:
+            best = max(best, int(m))
+    return float(s_sorted[best - 1]) if best else float("-inf")
+
+
+def hybrid(plan: KsgPlan, a1, a2, model: Surrogate, band: float = 1e-3, audit: float = 0.01,
+           alpha: float = 0.01, seed: int = 0, nsample: int = 32,
+           sigma_max: float | None = None) -> dict:
+    """r(MI) of the pairs: the surrogate's mu where it is trusted, exact GPU
+    KSG everywhere else.
+
+    A random audit fraction of the pairs is always run as exact KSG first.
+    Half of the audit calibrates the gate (calibrate_threshold: the largest
+    sigma threshold whose kept calibration pairs have their (1 - alpha)
+    split-conformal error quantile within band).  The other half checks it:
+    if more than an alpha share (plus one pair) of its pairs that the gate
+    keeps is off by more than band, the hybrid fails closed and every pair
+    runs as exact KSG.  The gate bounds the (1 - alpha) quantile of the kept
+    pairs' error, not its maximum.  With sigma_max set, the raw gate sigma <=
+    sigma_max is used instead (no calibration)."""
     a1 = np.asarray(a1, np.int32)
     a2 = np.asarray(a2, np.int32)
     t0 = time.perf_counter()
     feat, ms_feat = pair_features(plan, a1, a2, nsample)
     mu, sigma, ms_inf = model.predict(feat, bf16=True)
+    r = np.clip(mu, 0.0, 1.0)
     rng = np.random.default_rng(seed)
     audited = rng.random(len(a1)) < audit
-    fallback = (sigma > sigma_max) | audited
-    r = np.clip(mu, 0.0, 1.0)
+    aud_idx = np.flatnonzero(audited)
     ms_ksg = 0.0
-    if fallback.any():
-        if fallback.all() and _all_pairs_in_order(a1, a2, plan.natoms):
-            mi = plan.run_all()   # the same values; the all-pairs tiles keep more rows in L2
-        else:
-            mi = plan.run_pairs(a1[fallback], a2[fallback])
-        ms_ksg = plan.stats()["ms_pairs"] + plan.stats()["ms_fixup"]
-        r_exact = mi_to_r(mi)
-        kept_audit = audited[fallback] & (sigma[fallback] <= sigma_max)
-        audit_err = np.abs(r[fallback][kept_audit] - r_exact[kept_audit])
-        r[fallback] = r_exact
+    r_aud = np.zeros(0)
+    if len(aud_idx):
+        r_aud = mi_to_r(plan.run_pairs(a1[aud_idx], a2[aud_idx]))
+        ms_ksg += plan.stats()["ms_pairs"] + plan.stats()["ms_fixup"]
+    res_aud = r[aud_idx] - r_aud
+    cal = np.zeros(len(aud_idx), bool)
+    cal[::2] = True
+    if sigma_max is not None:
+        thresh = sigma_max
     else:
-        audit_err = np.zeros(0)
+        thresh = calibrate_threshold(res_aud[cal], sigma[aud_idx][cal], band, alpha)
+    gate = sigma <= thresh
+    chk = ~cal & gate[aud_idx]
+    check_err = np.abs(res_aud[chk])
+    check_max = float(check_err.max()) if chk.any() else 0.0
+    check_miss = int(np.sum(check_err > band))
+    failed_closed = sigma_max is None and check_miss > alpha * len(check_err) + 1
+    keep = gate & ~audited & (not failed_closed)
+    fallback = ~keep
+    r[aud_idx] = r_aud
+    rest = fallback & ~audited
+    if rest.any():
+        if fallback.all() and _all_pairs_in_order(a1, a2, plan.natoms):
+            # every pair exact: the all-pairs launch (2x2 atom tiles keep more
+            # rows in L2) gives the same values
+            r = mi_to_r(plan.run_all())
+        else:
+            r[rest] = mi_to_r(plan.run_pairs(a1[rest], a2[rest]))
+        ms_ksg += plan.stats()["ms_pairs"] + plan.stats()["ms_fixup"]
     wall = time.perf_counter() - t0
+    kept_aud = audited & gate
     return {
         "r": r, "mu": mu, "sigma": sigma, "fallback": fallback, "audited": audited,
-        "fallback_fraction": float(fallback.mean()),
-        "audit_max_abs_dr": float(audit_err.max()) if len(audit_err) else 0.0,
-        "audit_kept_pairs": int(len(audit_err)),
+        "kept": keep, "fallback_fraction": float(fallback.mean()),
+        "band": band, "alpha": alpha, "sigma_threshold": thresh,
+        "failed_closed": bool(failed_closed), "check_max_abs_dr": check_max,
+        "check_pairs": int(len(check_err)), "check_beyond_band": check_miss,
+        "audit_max_abs_dr": float(np.max(np.abs(res_aud[gate[aud_idx]])))
+        if kept_aud.any() else 0.0,
+        "audit_kept_pairs": int(kept_aud.sum()),
         "ms_features": ms_feat, "ms_inference": ms_inf, "ms_exact_ksg": ms_ksg,
         "wall_s": wall,
     }

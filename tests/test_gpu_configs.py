@@ -153,12 +153,17 @@ def test_surrogate_train_500x4000_hybrid_2000x5000(gpu, oracle_lib):
         plan.load(rows, psi)
         mi_plain = plan.run_all()
         r_plain = surrogate.mi_to_r(mi_plain)
-        res = surrogate.hybrid(plan, a1, a2, model, sigma_max=3e-3, audit=0.01, seed=1)
+        res = surrogate.hybrid(plan, a1, a2, model, band=3e-2, audit=0.01, alpha=0.01, seed=1)
     fb = res["fallback"]
-    print(f"hybrid 2000x5000: fallback {res['fallback_fraction']:.3f}, kept pairs' max|dr| "
-          f"{np.max(np.abs(res['r'][~fb] - r_plain[~fb])) if (~fb).any() else 0.0:.3e}",
-          flush=True)
-    assert 0.0 < res["fallback_fraction"] < 1.0
+    kept = res["kept"]
+    err_kept = np.abs(res["r"][kept] - r_plain[kept])
+    print(f"hybrid 2000x5000 at band 3e-2: fallback {res['fallback_fraction']:.3f}, sigma "
+          f"threshold {res['sigma_threshold']:.3e}, kept pairs within band "
+          f"{np.mean(err_kept <= 3e-2) if kept.any() else 1.0:.4f}, max|dr| "
+          f"{err_kept.max() if kept.any() else 0.0:.3e}", flush=True)
+    assert 0.0 < res["fallback_fraction"] < 1.0 and not res["failed_closed"]
+    # split-conformal coverage at level 0.99 over about 10^4 calibration pairs
+    assert np.mean(err_kept <= 3e-2) >= 0.98
     # every pair sent to exact KSG equals the plain run; kind 3's row sums
     # are added in launch order, so allow the last bits
     assert float(np.max(np.abs(res["r"][fb] - r_plain[fb]))) <= 1e-12

@@ -104,9 +104,13 @@ def test_hybrid_fallback_is_exact_and_gains(small_plan):
     rng = np.random.default_rng(0)
     tr = rng.random(len(a1)) < 0.5
     model = surrogate.Surrogate(1024).fit(feat[tr], r_exact[tr], epochs=150, batch=128, lr=3e-3)
-    res = surrogate.hybrid(plan, a1[~tr], a2[~tr], model, sigma_max=2e-2, audit=0.2, seed=1)
+    res = surrogate.hybrid(plan, a1[~tr], a2[~tr], model, band=3e-2, audit=0.2, alpha=0.05,
+                           seed=1)
     fb = res["fallback"]
     assert np.array_equal(res["r"][fb], r_exact[~tr][fb])
+    kept = res["kept"]
+    if kept.any():   # conformal coverage at level 0.95, on a few hundred pairs
+        assert np.mean(np.abs(res["r"][kept] - r_exact[~tr][kept]) <= 3e-2) >= 0.85
     mu = res["mu"]
     rmse_net = np.sqrt(np.mean((mu - r_exact[~tr]) ** 2))
     rmse_feat = np.sqrt(np.mean((feat[~tr, 13] - r_exact[~tr]) ** 2))
