@@ -313,12 +313,15 @@ def bench_batched(args) -> int:
         rows, psi = ksg.kraskov_prepare(ksg.center(coords))
         t0 = time.perf_counter()
         kind, s4 = tuner.choose(rows, psi, k)
+        chunk = tuner.choose_chunk(rows, psi, k, kind, s4)
         t_tune += time.perf_counter() - t0
         choices.append({"natoms": spec.natoms, "nframes": spec.nframes, "k": k, "kind": kind,
-                        "s4": s4})
-        b.add(rows, psi, k, kind, s4)
+                        "s4": s4, "chunk": chunk})
+        tb = b.add(rows, psi, k, kind, s4)
+        b.set_chunk(tb, chunk)
         keep[t] = (rows, psi)
-        log(f"bench: traj {t} {spec.natoms}x{spec.nframes} k={k} -> path ({kind},{s4})")
+        log(f"bench: traj {t} {spec.natoms}x{spec.nframes} k={k} -> path ({kind},{s4}) "
+            f"chunk {chunk}")
     st = b.run(max_row_bytes=int(args.row_budget_gb * 1e9), nblocks=2048)
     npairs = st["npairs"]
     lat = []

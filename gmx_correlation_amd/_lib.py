@@ -99,13 +99,13 @@ C_SYMBOLS = [
     "gmxcorr_ksg_plan_create", "gmxcorr_ksg_plan_load", "gmxcorr_ksg_plan_run_all",
     "gmxcorr_ksg_plan_run_pairs", "gmxcorr_ksg_plan_result", "gmxcorr_ksg_plan_matrix",
     "gmxcorr_ksg_plan_stats", "gmxcorr_ksg_plan_destroy", "gmxcorr_ksg_plan_set_variant",
-    "gmxcorr_ksg_variant_fits", "gmxcorr_ksg_plan_layout", "gmxcorr_ksg_plan_create_budget",
+    "gmxcorr_ksg_variant_fits", "gmxcorr_ksg_plan_probe", "gmxcorr_ksg_plan_layout", "gmxcorr_ksg_plan_create_budget",
     "gmxcorr_ksg_plan_set_shape", "gmxcorr_ksg_pairs_device",
     "gmxcorr_ksg_pairs_device_noalloc",
     "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix", "gmxcorr_dense_set_slab_frames",
     "gmxcorr_te_matrix", "gmxcorr_te_pairs",
     "gmxcorr_ksg_batch_create", "gmxcorr_ksg_batch_add", "gmxcorr_ksg_batch_run",
-    "gmxcorr_ksg_batch_traj_bytes",
+    "gmxcorr_ksg_batch_traj_bytes", "gmxcorr_ksg_batch_set_chunk",
     "gmxcorr_ksg_batch_result", "gmxcorr_ksg_batch_stats", "gmxcorr_ksg_batch_destroy",
     "gmxcorr_ksg_plan_features", "gmxcorr_mlp_num_params", "gmxcorr_mlp_create",
     "gmxcorr_mlp_set_params", "gmxcorr_mlp_get_params", "gmxcorr_mlp_grad",
@@ -141,12 +141,14 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_ksg_plan_set_variant.argtypes = [ctypes.c_void_p, _i, _i]
     L.gmxcorr_ksg_plan_layout.argtypes = [_i, _i, _i, _ll, _ip, _ip, _ip]
     L.gmxcorr_ksg_plan_create_budget.argtypes = [_i, _i, _i, _ll, ctypes.POINTER(ctypes.c_void_p)]
-    L.gmxcorr_ksg_plan_set_shape.argtypes = [ctypes.c_void_p, _i, _i]
+    L.gmxcorr_ksg_plan_set_shape.argtypes = [ctypes.c_void_p, _i, _i, _i]
     L.gmxcorr_ksg_pairs_device.argtypes = [ctypes.c_void_p, _i, _i, ctypes.c_void_p, _i, _ll,
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_void_p]
     L.gmxcorr_ksg_pairs_device_noalloc.argtypes = L.gmxcorr_ksg_pairs_device.argtypes
     L.gmxcorr_ksg_variant_fits.argtypes = [_i, _i, _i, _i]
+    L.gmxcorr_ksg_plan_probe.argtypes = [ctypes.c_void_p, ctypes.c_double,
+                                         ctypes.POINTER(ctypes.c_double)]
     L.gmxcorr_dense_set_slab_frames.argtypes = [_i]
     L.gmxcorr_gauss_matrix.argtypes = [_dp, _i, _i, _dp, _dp, ctypes.POINTER(DenseStats)]
     L.gmxcorr_gcmi_matrix.argtypes = [_dp, _i, _i, _dp, _dp, _dp, ctypes.POINTER(DenseStats)]
@@ -169,6 +171,7 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_ksg_batch_traj_bytes.argtypes = [_vp, _i]
     L.gmxcorr_ksg_batch_traj_bytes.restype = _ll
     L.gmxcorr_ksg_batch_run.argtypes = [_vp, _ll, _i]
+    L.gmxcorr_ksg_batch_set_chunk.argtypes = [_vp, _i, _i]
     L.gmxcorr_ksg_batch_result.argtypes = [_vp, _i, _dp, _ll, _dp, _ip]
     L.gmxcorr_ksg_batch_stats.argtypes = [_vp, ctypes.POINTER(BatchStats)]
     L.gmxcorr_ksg_batch_destroy.argtypes = [_vp]

@@ -97,6 +97,70 @@ def tune_variant(rows: np.ndarray, psi: np.ndarray, k: int, natoms_probe: int = 
     return (kind, s4), timings
 
 
+# Shape candidates: row ranges per pair and the all-pairs block shape (2x2
+# atom tile, or four consecutive pairs) for the one-wave kinds, frame rows
+# per block for kind 3.  The kinds' defaults are 128 ranges, 16 rows, tiles.
+SHAPE_SPLITS = (32, 64, 128, 256)
+SHAPE_CHUNKS = (4, 8, 16, 32, 64)
+SHAPE_TILES = (1, 2)
+
+
+def shape_candidates(kind: int) -> list[dict]:
+    if kind == 3:
+        return [{"nsplit": 0, "chunk": c, "tile": t} for c in SHAPE_CHUNKS for t in SHAPE_TILES]
+    if kind in (0, 2, 4):
+        return [{"nsplit": n, "chunk": 0, "tile": t} for n in SHAPE_SPLITS for t in SHAPE_TILES]
+    return [{"nsplit": 0, "chunk": 0, "tile": 0}]
+
+
+def shape_key(shape: dict) -> str:
+    return f"{shape['nsplit']}:{shape['chunk']}:{shape['tile']}"
+
+
+def time_shapes(rows: np.ndarray, psi: np.ndarray, k: int, kind: int, s4: int,
+                shapes: list[dict], reps: int = 3, fraction: float | None = None) -> dict:
+    """ms of the trajectory's all-pairs run for each shape, the best of
+    `reps` after one warm-up: the whole run (pair kernel, long rows,
+    recount, finalise) or, with `fraction`, the pair kernel over that
+    fraction of its work items (KsgPlan.probe)."""
+    natoms, N = rows.shape[0] // 3, rows.shape[1]
+    out = {}
+    with KsgPlan(natoms, N, k) as plan:
+        plan.load(np.ascontiguousarray(rows), psi)
+        plan.set_variant(kind, s4)
+        for shp in shapes:
+            plan.set_shape(shp["nsplit"], shp["chunk"], shp["tile"])
+            t = []
+            for r in range(reps + 1):
+                if fraction is not None:
+                    ms = plan.probe(fraction)
+                else:
+                    plan.run_all_device()
+                    st = plan.stats()
+                    ms = st["ms_pairs"] + st["ms_fixup"]
+                if r:          # the first run is the warm-up
+                    t.append(ms)
+            out[shape_key(shp)] = float(np.min(t))
+    return out
+
+
+def tune_shape(rows: np.ndarray, psi: np.ndarray, k: int, kind: int, s4: int,
+               fraction: float = 0.125, reps: int = 3) -> tuple[dict, dict]:
+    """Row-block and tile shape of path (kind, s4) for this trajectory: every
+    candidate of shape_candidates(kind) timed by the pair kernel's probe over
+    the first `fraction` of the all-pairs work items.  The shapes act
+    through which distance rows the blocks in flight on one XCD share in its
+    L2; the probe keeps every atom and the launch's block order (tiles,
+    row-range-major or chunk-major), so the blocks in flight share rows as
+    in the full run, over fewer frame rows.  (Timing a subset of the atoms
+    does not: with fewer tiles per row range the order, and the best shape,
+    change.)  Returns the fastest shape and all probe timings (ms)."""
+    shapes = shape_candidates(kind)
+    timings = time_shapes(rows, psi, k, kind, s4, shapes, reps, fraction)
+    best = min(shapes, key=lambda sh: timings[shape_key(sh)])
+    return best, timings
+
+
 class Tuner:
     """Per-(frame count, k) cache of tuned paths, optionally kept in a JSON
     file between runs."""
@@ -113,12 +177,28 @@ class Tuner:
         if key not in self.cache:
             (kind, s4), timings = tune_variant(rows, psi, k)
             self.cache[key] = {"kind": kind, "s4": s4, "ms": timings}
-            if self.path:
-                os.makedirs(os.path.dirname(os.path.abspath(self.path)), exist_ok=True)
-                with open(self.path, "w") as f:
-                    json.dump(self.cache, f, indent=1)
+            self._save()
         c = self.cache[key]
         return c["kind"], c["s4"]
+
+    def choose_chunk(self, rows, psi, k, kind, s4) -> int:
+        """Frame rows per kind-3 work item of the batch (0 for other kinds)."""
+        if kind != 3:
+            return 0
+        key = f"{rows.shape[1]}:{k}:{kind}:{s4}:chunk"
+        if key not in self.cache:
+            shapes = [{"nsplit": 0, "chunk": c, "tile": 1} for c in SHAPE_CHUNKS]
+            t = time_shapes(rows, psi, k, kind, s4, shapes, fraction=0.125)
+            best = min(shapes, key=lambda sh: t[shape_key(sh)])
+            self.cache[key] = {"chunk": best["chunk"], "ms": t}
+            self._save()
+        return self.cache[key]["chunk"]
+
+    def _save(self):
+        if self.path:
+            os.makedirs(os.path.dirname(os.path.abspath(self.path)), exist_ok=True)
+            with open(self.path, "w") as f:
+                json.dump(self.cache, f, indent=1)
 
 
 class KsgBatch:
@@ -139,6 +219,10 @@ class KsgBatch:
             _lib.check(rc)
         self.shapes.append((natoms, N, k))
         return rc
+
+    def set_chunk(self, t: int, chunk: int) -> None:
+        """Frame rows per kind-3 work item of trajectory t (0 = default 16)."""
+        _lib.check(_lib.load().gmxcorr_ksg_batch_set_chunk(self._h, t, chunk))
 
     def traj_bytes(self, t: int) -> int:
         """Device bytes of trajectory t's rows in a memory wave."""

@@ -129,6 +129,16 @@ int gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const doubl
     return (int)b->traj.size() - 1;
 }
 
+int gmxcorr_ksg_batch_set_chunk(gmxcorr_ksg_batch* b, int t, int chunk)
+{
+    if (!b || t < 0 || t >= (int)b->traj.size())
+        return gmxcorr::set_error(GMXCORR_EINVAL, "batch_set_chunk: no such trajectory");
+    if (chunk < 0 || chunk > 1024)
+        return gmxcorr::set_error(GMXCORR_EINVAL, "batch_set_chunk: chunk must be in [0, 1024]");
+    b->traj[t].chunk = chunk > 0 ? chunk : gmxcorr::kLongChunk;
+    return GMXCORR_OK;
+}
+
 // Run every pair of every trajectory.  Trajectories are packed greedily (in
 // order of decreasing row bytes) into memory waves of at most
 // max_row_bytes of distance rows; each wave is one row build per trajectory

@@ -1740,6 +1740,13 @@ __device__ __forceinline__ double row_coop(const float* Da, const float* Db, lon
 // vb0 and every grid but the last a multiple of 8): each XCD (vb mod 8)
 // receives a contiguous range of items, so neighbouring items keep their
 // shared atoms' rows in that XCD's L2.
+// Work items of a launch: all n, or the first `limit` of them (limit > 0: a
+// timing probe of the same block order over fewer frame rows).
+__host__ __device__ __forceinline__ long long limit_items(long long n, long long limit)
+{
+    return limit > 0 && limit < n ? limit : n;
+}
+
 __device__ __forceinline__ long long xcd_item(long long vb, long long n)
 {
     const long long per_xcd = (n + 7) / 8;
@@ -1762,7 +1769,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // pairs share their first atom).
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
     const int nsplit = args.nsplit;
-    const long long nitems = nquads * nsplit;
+    const long long nitems = limit_items(nquads * nsplit, args.item_limit);
     // virtual block: the launcher splits a grid above 2^22 blocks into
     // launches at offsets vb0 (a multiple of 8)
     const long long vb = args.vb0 + blockIdx.x;
@@ -1816,7 +1823,7 @@ __global__ __launch_bounds__(256) void ksg_pair_prefix_kernel(KsgArgs args)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
     const int nsplit = args.nsplit;
-    const long long nitems = nquads * nsplit;
+    const long long nitems = limit_items(nquads * nsplit, args.item_limit);
     // virtual block: the launcher splits a grid above 2^22 blocks into
     // launches at offsets vb0 (a multiple of 8)
     const long long vb = args.vb0 + blockIdx.x;
@@ -1859,7 +1866,7 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long nquads = args.quads ? args.nquads : (args.npairs + kPairWaves - 1) / kPairWaves;
     const int nchunk = (args.N + args.chunk - 1) / args.chunk;
-    const long long nitems = nquads * nchunk;
+    const long long nitems = limit_items(nquads * nchunk, args.item_limit);
     // virtual block: the launcher splits a grid above 2^22 blocks into
     // launches at offsets vb0 (a multiple of 8)
     const long long vb = args.vb0 + blockIdx.x;
@@ -1937,8 +1944,9 @@ __global__ __launch_bounds__(256) void ksg_pair_coop_kernel(KsgArgs args)
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     const long long vb = args.vb0 + blockIdx.x;
-    const long long p = xcd_item(vb, args.npairs);
-    if (p >= args.npairs) return;   // uniform over the block
+    const long long np = limit_items(args.npairs, args.item_limit);
+    const long long p = xcd_item(vb, np);
+    if (p >= np) return;   // uniform over the block
     const int N = args.N, T = args.K + 1;
     const float* Da = args.D + (size_t)args.a1[p] * N * args.P;
     const float* Db = args.D + (size_t)args.a2[p] * N * args.P;
@@ -2373,7 +2381,7 @@ __global__ __launch_bounds__(256) void ksg_coord_kernel(KsgArgs args)
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int N = args.N, nsplit = args.nsplit;
-    const long long nitems = args.npairs * nsplit;
+    const long long nitems = limit_items(args.npairs * nsplit, args.item_limit);
     for (long long item = blockIdx.x; item < nitems; item += gridDim.x) {   // uniform
         const long long p = item / nsplit;
         const int part = (int)(item - p * nsplit);
@@ -2547,6 +2555,15 @@ void default_variant(int N, int K, int& kind, int& s4)
     if (s4 < 2) s4 = 2;
 }
 
+long long pair_items(int kind, const KsgArgs& a)
+{
+    const long long nquads = a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves;
+    return kind == 1   ? a.npairs
+           : kind == 3 ? nquads * ((a.N + a.chunk - 1) / a.chunk)
+           : kind == 4 ? a.npairs * a.nsplit
+                       : nquads * a.nsplit;
+}
+
 int variant_slots(int kind, int nsplit)
 {
     return kind == 3 ? 0 : kind == 1 ? 1 : nsplit;
@@ -2571,15 +2588,12 @@ hipError_t launch_ksg_pairs(const KsgArgs& a, hipStream_t s, int kind, int s4, i
     // adds into fixsum)
     if (nsplit) *nsplit = variant_slots(kind, a.nsplit);
     if (kind == 4) {
-        const long long items = a.npairs * a.nsplit;
+        const long long items = limit_items(a.npairs * a.nsplit, a.item_limit);
         hipLaunchKernelGGL(ksg_coord_kernel, dim3(capped_grid(items)), dim3(kWave * kPairWaves),
                            0, s, a);
         return hipGetLastError();
     }
-    const long long nquads = a.quads ? a.nquads : (a.npairs + kPairWaves - 1) / kPairWaves;
-    const long long nitems = kind == 1   ? a.npairs
-                             : kind == 3 ? nquads * ((a.N + a.chunk - 1) / a.chunk)
-                                         : nquads * a.nsplit;
+    const long long nitems = limit_items(pair_items(kind, a), a.item_limit);
     // one block per virtual block; a dispatch carries fewer than 2^32
     // work-items, so grids above 2^22 blocks go out in several launches
     const long long nvb = (nitems + 7) / 8 * 8;

@@ -215,6 +215,7 @@ struct gmxcorr_ksg_plan {
     int var_kind = -1, var_s4 = 0;
     int nsplit = 0;          // row ranges per pair (0: default for the pair count)
     int chunk = gmxcorr::kLongChunk;
+    bool tiles = true;       // all-pairs blocks: 2x2 atom tiles, or runs of four pairs
     gmxcorr::TrajDesc* dTraj = nullptr;   // one descriptor for the recount/finalise kernels
     unsigned char* hStage = nullptr;      // pinned staging for the upload: two halves
     hipEvent_t stage_ev[2] = {};          // copy of each half done
@@ -804,12 +805,43 @@ int gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* pl)
         pl->all_pairs = true;
     }
     pl->host_result = false;
-    const int rc = run_loaded(pl, npairs, false, pl->dQuads != nullptr);
+    const int rc = run_loaded(pl, npairs, false, pl->tiles && pl->dQuads != nullptr);
     if (!rc) {
         pl->last_npairs = npairs;
         pl->st.all_pairs = 1;
     }
     return rc;
+}
+
+int gmxcorr_ksg_plan_probe(gmxcorr_ksg_plan* pl, double fraction, double* ms)
+{
+    if (!pl || !pl->loaded || !ms) return fail(GMXCORR_EINVAL, "plan_probe: plan not loaded");
+    if (!(fraction > 0.0 && fraction <= 1.0)) return fail(GMXCORR_EINVAL, "plan_probe: fraction in (0, 1]");
+    if (pl->wave_atoms > 0) return fail(GMXCORR_EUNSUP, "plan_probe: plan runs in memory waves");
+    // the all-pairs list and tiles, as run_all sets them up
+    if (int rc = gmxcorr_ksg_plan_run_all(pl)) return rc;
+    const long long npairs = (long long)pl->natoms * (pl->natoms - 1) / 2;
+    int kind, s4;
+    plan_kind(pl, kind, s4);
+    KsgArgs a{};
+    if (pl->tiles && pl->dQuads) { a.quads = pl->dQuads; a.nquads = pl->nquads; }
+    a.X = pl->dXs; a.D = pl->dD; a.SD = pl->dSD; a.SM = pl->dSM; a.SI = pl->dSI; a.LV = pl->dLV;
+    a.LI = pl->dLI; a.LS = pl->dLS; a.psi = pl->dPsi; a.a1 = pl->dA1; a.a2 = pl->dA2;
+    a.partial = pl->dPartial; a.fixsum = pl->dFixsum; a.mi = pl->dMI; a.counters = pl->dCounters;
+    a.fixq = pl->dFix; a.fixcap = pl->fixcap; a.longq = pl->dLong; a.longcap = pl->longcap;
+    a.npairs = npairs; a.P = pl->P; a.N = pl->N; a.K = pl->K; a.psiN_phiK = pl->psiN_phiK;
+    a.nsplit = plan_split(pl, kind, npairs); a.chunk = pl->chunk;
+    const long long items = gmxcorr::pair_items(kind, a);
+    a.item_limit = std::max<long long>(1, (long long)std::ceil(fraction * (double)items));
+    HIPCHK(gmxcorr::launch_ksg_clear(pl->dFixsum, npairs, pl->dCounters, nullptr, pl->stream));
+    HIPCHK(hipEventRecord(pl->ev[2], pl->stream));
+    HIPCHK(gmxcorr::launch_ksg_pairs(a, pl->stream, kind, s4, nullptr));
+    HIPCHK(hipEventRecord(pl->ev[3], pl->stream));
+    HIPCHK(hipStreamSynchronize(pl->stream));
+    *ms = elapsed(pl->ev[2], pl->ev[3]);
+    pl->last_npairs = -1;   // the probe leaves no result
+    pl->st.all_pairs = 0;
+    return GMXCORR_OK;
 }
 
 int gmxcorr_ksg_plan_run_pairs(gmxcorr_ksg_plan* pl, long long npairs, const int* a1,
@@ -898,14 +930,16 @@ int gmxcorr_ksg_plan_set_variant(gmxcorr_ksg_plan* pl, int kind, int s4)
     return GMXCORR_OK;
 }
 
-int gmxcorr_ksg_plan_set_shape(gmxcorr_ksg_plan* pl, int nsplit, int chunk)
+int gmxcorr_ksg_plan_set_shape(gmxcorr_ksg_plan* pl, int nsplit, int chunk, int tile)
 {
     if (!pl) return fail(GMXCORR_EINVAL, "set_shape: null plan");
     if (nsplit < 0 || nsplit > gmxcorr::kMaxRowSplit)
         return fail(GMXCORR_EINVAL, "set_shape: nsplit must be in [0, 256] (0 = default)");
     if (chunk < 0 || chunk > 1024) return fail(GMXCORR_EINVAL, "set_shape: chunk must be in [0, 1024] (0 = default)");
+    if (tile < 0 || tile > 2) return fail(GMXCORR_EINVAL, "set_shape: tile must be 0, 1 or 2");
     pl->nsplit = nsplit;
     pl->chunk = chunk > 0 ? chunk : gmxcorr::kLongChunk;
+    pl->tiles = tile != 2;
     if (pl->loaded) {   // the recount kernels read the descriptor
         gmxcorr::TrajDesc td;
         set_descriptor(pl, td);

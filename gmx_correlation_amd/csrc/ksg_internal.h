@@ -57,6 +57,7 @@ struct KsgArgs {
     long long     vb0;       // first virtual block of this launch (set by the launcher)
     int           nsplit;    // row ranges per pair (kinds 0, 2 and 4; partial slots per pair)
     int           chunk;     // frame rows per block of kind 3
+    long long     item_limit;  // > 0: run only the first item_limit work items (timing probe)
 };
 
 // One trajectory's device data and shape.  kind/s4 select the pair-kernel
@@ -214,6 +215,9 @@ void default_variant(int N, int K, int& kind, int& s4);
 // Partial-sum slots per pair that a launch of this kind writes (the
 // finalise step's nsplit; 0: all of it goes to fixsum).
 int variant_slots(int kind, int nsplit);
+// Work items (blocks' worth of work) of a pair launch of this kind, in the
+// launch's block order (row-range-major, chunk-major or pair-major).
+long long pair_items(int kind, const KsgArgs& a);
 // nsplit (optional): partial-sum slots per pair the launch wrote, for the
 // finalise step (0: all of it went to fixsum).  a.nsplit / a.chunk: row
 // ranges per pair (kinds 0, 2, 4) and rows per block (kind 3).

@@ -142,10 +142,19 @@ class KsgPlan:
         the coordinates (any N), kind -1 default."""
         _lib.check(_lib.load().gmxcorr_ksg_plan_set_variant(self._h, kind, s4))
 
-    def set_shape(self, nsplit: int = 0, chunk: int = 0) -> None:
-        """Row-block shape: nsplit row ranges per pair (kinds 0, 2, 4; 0 =
-        default), chunk frame rows per block of kind 3 (0 = default 16)."""
-        _lib.check(_lib.load().gmxcorr_ksg_plan_set_shape(self._h, nsplit, chunk))
+    def set_shape(self, nsplit: int = 0, chunk: int = 0, tile: int = 0) -> None:
+        """Row-block and tile shape: nsplit row ranges per pair (kinds 0, 2,
+        4; 0 = default), chunk frame rows per block of kind 3 (0 = default
+        16), tile 0/1 = 2x2 atom tiles in the all-pairs run, 2 = runs of four
+        pairs sharing their first atom."""
+        _lib.check(_lib.load().gmxcorr_ksg_plan_set_shape(self._h, nsplit, chunk, tile))
+
+    def probe(self, fraction: float) -> float:
+        """ms of the all-pairs pair kernel over the first `fraction` of its
+        work items (the autotuner's timing probe; leaves no result)."""
+        ms = ctypes.c_double(0.0)
+        _lib.check(_lib.load().gmxcorr_ksg_plan_probe(self._h, float(fraction), ctypes.byref(ms)))
+        return ms.value
 
     def stats(self) -> dict:
         st = _lib.KsgStats()

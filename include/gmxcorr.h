@@ -117,12 +117,20 @@ int  gmxcorr_ksg_plan_stats(const gmxcorr_ksg_plan* plan, gmxcorr_ksg_stats* st)
  * the default for the plan's frame count and k.  variant_fits: 1 if
  * (kind, s4) is compiled and covers nframes at this k. */
 int  gmxcorr_ksg_plan_set_variant(gmxcorr_ksg_plan* plan, int kind, int s4);
-/* Row-block shape (the autotuner's second knob): nsplit row ranges per pair
- * for kinds 0, 2 and 4 (1..256; 0 = default: 128 for kinds 0 and 2, fewer
- * when the partial sums would pass 8 GB, and enough to fill the device for
- * kind 4), chunk frame rows per block of kind 3 (1..1024; 0 = default 16). */
-int  gmxcorr_ksg_plan_set_shape(gmxcorr_ksg_plan* plan, int nsplit, int chunk);
+/* Row-block and tile shape (the autotuner's second knob): nsplit row ranges
+ * per pair for kinds 0, 2 and 4 (1..256; 0 = default: 128 for kinds 0 and 2,
+ * fewer when the partial sums would pass 8 GB, and enough to fill the device
+ * for kind 4), chunk frame rows per block of kind 3 (1..1024; 0 = default
+ * 16), tile: the four pairs of a block in the all-pairs run (0 or 1: a 2x2
+ * atom tile, 2: four consecutive pairs sharing their first atom). */
+int  gmxcorr_ksg_plan_set_shape(gmxcorr_ksg_plan* plan, int nsplit, int chunk, int tile);
 int  gmxcorr_ksg_variant_fits(int kind, int s4, int nframes, int k);
+/* Timing probe of the autotuner: the all-pairs pair kernel with the plan's
+ * current variant and shape over only the first `fraction` of its work items
+ * (the same block order, so the same rows shared in L2 by the blocks in
+ * flight, over fewer frame rows); ms receives its time (HIP events).  Leaves
+ * no result: plan_result fails until the next run.  Resident plans only. */
+int  gmxcorr_ksg_plan_probe(gmxcorr_ksg_plan* plan, double fraction, double* ms);
 void gmxcorr_ksg_plan_destroy(gmxcorr_ksg_plan* plan);
 
 /* ---- Device-pointer pair list on the coordinate path -----------------------
@@ -225,6 +233,9 @@ int  gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const doub
  * gmxcorr_ksg_batch_run must be at least the largest of these. */
 long long gmxcorr_ksg_batch_traj_bytes(const gmxcorr_ksg_batch* b, int t);
 int  gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nblocks);
+/* Frame rows per work item of trajectory t when it runs on kind 3 (1..1024;
+ * 0 = default 16); the autotuner's shape knob for the batch. */
+int  gmxcorr_ksg_batch_set_chunk(gmxcorr_ksg_batch* b, int t, int chunk);
 /* MI of trajectory t's pairs (a1 = 1.., a2 < a1); ms_done: batch-relative
  * completion time of its wave; wave: its wave index (both optional). */
 int  gmxcorr_ksg_batch_result(gmxcorr_ksg_batch* b, int t, double* mi, long long npairs,
