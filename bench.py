@@ -376,6 +376,47 @@ def bench_te(args) -> int:
     return 0
 
 
+def bench_te_full(args) -> int:
+    """The whole transfer-entropy matrix of 1000 atoms x 10000 frames
+    (Student-t noise), k = 100, at one lag: all 999,000 ordered pairs in
+    launches of 50,000 (progress on stderr), wall time, and 24 sampled
+    ordered pairs against the oracle's FP64 restatement."""
+    import concurrent.futures as cf
+    from gmx_correlation_amd import dense, ksg, synth
+    from oracle import oracle as O
+    spec = synth.DENSE_1000
+    log(f"bench: generating workload {spec}")
+    traj = ksg.center(synth.generate(spec))
+    n = spec.natoms
+    lag, k = args.lag, 100
+    a1, a2 = np.nonzero(~np.eye(n, dtype=bool))
+    a1 = a1.astype(np.int32)
+    a2 = a2.astype(np.int32)
+    te = np.empty(len(a1))
+    chunk = 50000
+    kern_ms = 0.0
+    t0 = time.perf_counter()
+    for c0 in range(0, len(a1), chunk):
+        v, ms = dense.te_pairs(traj, a1[c0:c0 + chunk], a2[c0:c0 + chunk], k=k, lag=lag)
+        te[c0:c0 + chunk] = v
+        kern_ms += ms
+        log(f"bench: te lag {lag}: {min(c0 + chunk, len(a1))} / {len(a1)} ordered pairs, "
+            f"{time.perf_counter() - t0:.1f} s")
+    wall = time.perf_counter() - t0
+    rng = np.random.default_rng(lag)
+    pick = rng.choice(len(a1), 24, replace=False)
+    with cf.ThreadPoolExecutor(max_workers=16) as ex:
+        want = list(ex.map(lambda p: O.te_pair(traj, int(a1[p]), int(a2[p]), k, lag), pick))
+    d = float(np.max(np.abs(te[pick] - np.array(want))))
+    out = {"workload": f"te matrix 1000 atoms x 10000 frames k={k} lag {lag}",
+           "ordered_pairs": int(len(a1)), "wall_s": wall, "kernel_s": kern_ms / 1e3,
+           "ordered_pairs_per_s": len(a1) / wall,
+           "sampled_pairs_vs_fp64_oracle": 24, "max_abs_dte": d,
+           "te_mean": float(te.mean()), "te_max": float(te.max())}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def _gcmi_from_cov(c, n):
     """GCMI pair MI from an FP64 second-moment matrix, batched determinants."""
     a1, a2 = np.triu_indices(n, 1)
@@ -403,11 +444,12 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU-baseline work (0 disables)")
     ap.add_argument("--workload", default="ksg",
-                    choices=["ksg", "dense", "hybrid", "batched", "te", "noise"],
+                    choices=["ksg", "dense", "hybrid", "batched", "te", "te_full", "noise"],
                     help="ksg: the headline metric; dense: Gaussian/GCMI GEMM path; "
                          "hybrid: surrogate + KSG fallback; batched: uneven trajectories")
     ap.add_argument("--epochs", type=int, default=120, help="surrogate training epochs")
     ap.add_argument("--te-pairs", type=int, default=512, help="ordered pairs timed (te)")
+    ap.add_argument("--lag", type=int, default=1, help="lag of the full TE matrix (te_full)")
     ap.add_argument("--row-budget-gb", type=float, default=200.0,
                     help="distance-row bytes per memory wave (batched)")
     args = ap.parse_args()
@@ -424,6 +466,8 @@ def main() -> int:
         return bench_batched(args)
     if args.workload == "te":
         return bench_te(args)
+    if args.workload == "te_full":
+        return bench_te_full(args)
     if args.workload == "noise":
         return bench_noise(args)
 
