@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG, "libgmxcorr_hip.so")
 
 GMXCORR_OK = 0
 STATUS_NAMES = {-1: "EINVAL", -2: "ENODEV", -3: "ENOMEM", -4: "EHIP", -5: "EUNSUP"}
+GMXCORR_EOF = 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -110,6 +111,7 @@ C_SYMBOLS = [
     "gmxcorr_ksg_plan_features", "gmxcorr_mlp_num_params", "gmxcorr_mlp_create",
     "gmxcorr_mlp_set_params", "gmxcorr_mlp_get_params", "gmxcorr_mlp_grad",
     "gmxcorr_mlp_train_epoch", "gmxcorr_mlp_forward", "gmxcorr_mlp_destroy",
+    "gmxcorr_xtc_open", "gmxcorr_xtc_read", "gmxcorr_xtc_write", "gmxcorr_xtc_close",
 ]
 CXX_SYMBOLS = ["_Z17kraskov_gpu_probev", "_Z22kraskov_corrmatrix_gpuPK9t_kraskoviPdi"]
 
@@ -176,6 +178,11 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_ksg_batch_stats.argtypes = [_vp, ctypes.POINTER(BatchStats)]
     L.gmxcorr_ksg_batch_destroy.argtypes = [_vp]
     L.gmxcorr_ksg_batch_destroy.restype = None
+    L.gmxcorr_xtc_open.argtypes = [ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_void_p)]
+    L.gmxcorr_xtc_read.argtypes = [_vp, _ip, _ip, _fp, _fp, _fp, _i, _fp]
+    L.gmxcorr_xtc_write.argtypes = [_vp, _i, _i, ctypes.c_float, _fp, _fp, ctypes.c_float]
+    L.gmxcorr_xtc_close.argtypes = [_vp]
+    L.gmxcorr_xtc_close.restype = None
     _lib = L
     return L
 

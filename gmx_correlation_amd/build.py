@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(PKG, LIB_NAME)
 SOURCES = [
     "ksg_device.hip", "ksg_host.hip", "dense_device.hip", "dense_host.hip",
     "surrogate_device.hip", "surrogate_host.hip", "batch_host.hip",
-    "te_device.hip", "te_host.hip",
+    "te_device.hip", "te_host.hip", "xtc_host.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -7,6 +7,8 @@ Names, argument meaning and error behaviour follow the reference:
   pearsify()           lib/correlation_core.cpp:461-469
   write_matrix()       lib/correlation_core.cpp:202-225
   read_matrix()        bin/analyze_matrix.py:71-102 (the format's consumer)
+  read_xtc()           the frames GROMACS hands to the analysis module
+                       (src/gmx_correlation.cpp:158-182), from an XTC file
 
 One deliberate difference: the reference's dispatcher silently falls back to
 its CPU loop when no GPU is present or k > 128.  This package has no CPU
@@ -308,3 +310,61 @@ def read_dump(path: str) -> np.ndarray:
     if vals.size != natoms * nframes * 3:
         raise ValueError(f"{path}: expected {natoms * nframes * 3} values, found {vals.size}")
     return np.ascontiguousarray(vals.reshape(nframes, natoms, 3).transpose(1, 0, 2))
+
+
+def read_xtc(path: str, atoms=None, max_frames: int | None = None):
+    """Frames of an XTC trajectory: float32 coordinates [nframes][natoms][3]
+    in nm (the layout center() takes), plus steps, times, boxes [nframes][3][3]
+    and the file's precision.  atoms: optional atom indices to keep (a static
+    selection, as the reference's -select).  Host-only: no device needed."""
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    _lib.check(L.gmxcorr_xtc_open(path.encode(), 0, ctypes.byref(h)))
+    frames, steps, times, boxes = [], [], [], []
+    prec = ctypes.c_float(0.0)
+    buf = np.empty((0, 3), np.float32)
+    try:
+        while max_frames is None or len(frames) < max_frames:
+            n, st, tm = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0.0)
+            box = np.empty(9, np.float32)
+            args = (ctypes.byref(n), ctypes.byref(st), ctypes.byref(tm), _lib.ptr(box, _lib._fp))
+            rc = L.gmxcorr_xtc_read(h, *args, _lib.ptr(buf, _lib._fp), len(buf),
+                                    ctypes.byref(prec))
+            if rc == _lib.GMXCORR_EOF:
+                break
+            if rc != _lib.GMXCORR_OK and n.value > len(buf):
+                buf = np.empty((n.value, 3), np.float32)      # sized from the header
+                rc = L.gmxcorr_xtc_read(h, *args, _lib.ptr(buf, _lib._fp), len(buf),
+                                        ctypes.byref(prec))
+            _lib.check(rc)
+            x = buf[:n.value] if atoms is None else buf[np.asarray(atoms)]
+            frames.append(x.copy())
+            steps.append(st.value)
+            times.append(tm.value)
+            boxes.append(box.reshape(3, 3).copy())
+    finally:
+        L.gmxcorr_xtc_close(h)
+    if not frames:
+        raise ValueError(f"{path}: no XTC frames")
+    return (np.ascontiguousarray(np.stack(frames)), np.array(steps), np.array(times, np.float32),
+            np.stack(boxes), float(prec.value))
+
+
+def write_xtc(path: str, coords, precision: float = 1000.0, steps=None, times=None,
+              boxes=None) -> None:
+    """Write float32 frames [nframes][natoms][3] (nm) as an XTC trajectory."""
+    c = np.ascontiguousarray(coords, dtype=np.float32)
+    T, n, _ = c.shape
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    _lib.check(L.gmxcorr_xtc_open(path.encode(), 1, ctypes.byref(h)))
+    try:
+        for f in range(T):
+            box = (np.zeros(9, np.float32) if boxes is None
+                   else np.ascontiguousarray(boxes[f], np.float32).reshape(9))
+            _lib.check(L.gmxcorr_xtc_write(h, n, int(steps[f]) if steps is not None else f,
+                                           float(times[f]) if times is not None else float(f),
+                                           _lib.ptr(box, _lib._fp), _lib.ptr(c[f], _lib._fp),
+                                           float(precision)))
+    finally:
+        L.gmxcorr_xtc_close(h)

@@ -24,6 +24,7 @@ extern "C" {
 #define GMXCORR_ENOMEM  -3   /* device allocation failed */
 #define GMXCORR_EHIP    -4   /* HIP runtime error */
 #define GMXCORR_EUNSUP  -5   /* shape outside what the kernels support */
+#define GMXCORR_EOF      1   /* gmxcorr_xtc_read: no further frame */
 
 const char* gmxcorr_last_error(void);
 
@@ -266,6 +267,23 @@ int  gmxcorr_mlp_train_epoch(gmxcorr_mlp* m, const float* x, const float* y, lon
 int  gmxcorr_mlp_forward(gmxcorr_mlp* m, const float* x, long long n, int bf16, float* out,
                          double* ms);
 void gmxcorr_mlp_destroy(gmxcorr_mlp* m);
+
+/* ---- XTC trajectories (GROMACS compressed coordinates) ----------------------
+ * Frame ingest without GROMACS: the reference's frames come from the GROMACS
+ * analysis framework (src/gmx_correlation.cpp:158-182), which reads XTC.
+ * Host-only code, no device needed.  write: 0 = read, 1 = write.  read fills
+ * natoms (always), step, time, box [9] and precision when non-NULL, and x
+ * [natoms][3] in nm (max_atoms: the capacity of x); returns GMXCORR_EOF after
+ * the last frame.  With x NULL or too small it sets natoms, leaves the file
+ * at the same frame and returns GMXCORR_EINVAL (size the buffer, call again).  write: precision is the XTC's (1000 = 0.001 nm);
+ * natoms <= 9 frames are stored uncompressed. */
+typedef struct gmxcorr_xtc gmxcorr_xtc;
+int  gmxcorr_xtc_open(const char* path, int write, gmxcorr_xtc** out);
+int  gmxcorr_xtc_read(gmxcorr_xtc* f, int* natoms, int* step, float* time, float* box, float* x,
+                      int max_atoms, float* precision);
+int  gmxcorr_xtc_write(gmxcorr_xtc* f, int natoms, int step, float time, const float* box,
+                       const float* x, float precision);
+void gmxcorr_xtc_close(gmxcorr_xtc* f);
 
 #ifdef __cplusplus
 }
