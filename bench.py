@@ -349,9 +349,10 @@ def bench_batched(args) -> int:
 
 
 def bench_te(args) -> int:
-    """Transfer entropy at 1000 atoms x 10000 frames (Student-t noise), k=100,
-    lags 1, 5, 10, timed on a sample of ordered pairs; three pairs checked
-    against the FP64 oracle."""
+    """Transfer entropy at 1000 atoms x 10000 frames (Student-t noise), k=100
+    (--te-k), lags 1, 5, 10, timed on a sample of ordered pairs with the
+    pass-1 kernel --te-path chooses; two pairs per lag checked against the
+    FP64 oracle."""
     from gmx_correlation_amd import dense, ksg, synth
     from oracle import oracle as O
     spec = synth.DENSE_1000
@@ -362,12 +363,15 @@ def bench_te(args) -> int:
     npairs = args.te_pairs
     a1 = rng.integers(0, n, npairs)
     a2 = (a1 + rng.integers(1, n, npairs)) % n
-    out = {"workload": "te 1000 atoms x 10000 frames k=100", "sample_pairs": npairs, "lags": {}}
+    k = args.te_k
+    dense.te_set_path(args.te_path)
+    out = {"workload": f"te 1000 atoms x 10000 frames k={k}", "sample_pairs": npairs,
+           "pass1": {0: "automatic", 1: "heap", 2: "radix select"}[args.te_path], "lags": {}}
     for lag in (1, 5, 10):
-        te, ms = dense.te_pairs(traj, a1, a2, k=100, lag=lag)
+        te, ms = dense.te_pairs(traj, a1, a2, k=k, lag=lag)
         chk = []
         for p in range(2):
-            chk.append(abs(te[p] - O.te_pair(traj, int(a1[p]), int(a2[p]), 100, lag)))
+            chk.append(abs(te[p] - O.te_pair(traj, int(a1[p]), int(a2[p]), k, lag)))
         out["lags"][lag] = {"kernel_ms": ms, "pairs_per_s": npairs / (ms / 1e3),
                             "full_matrix_s_projected": n * (n - 1) / (npairs / (ms / 1e3)),
                             "max_abs_dte_vs_fp64_oracle": float(max(chk))}
@@ -450,6 +454,9 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=120, help="surrogate training epochs")
     ap.add_argument("--te-pairs", type=int, default=512, help="ordered pairs timed (te)")
     ap.add_argument("--lag", type=int, default=1, help="lag of the full TE matrix (te_full)")
+    ap.add_argument("--te-k", type=int, default=100, help="k of the sampled TE workload (te)")
+    ap.add_argument("--te-path", type=int, default=0, choices=(0, 1, 2),
+                    help="TE pass-1 kernel (te): 0 automatic, 1 heap, 2 radix select")
     ap.add_argument("--row-budget-gb", type=float, default=200.0,
                     help="distance-row bytes per memory wave (batched)")
     args = ap.parse_args()

@@ -10,7 +10,8 @@
 // One lane per query frame, 64 queries per wave.  The frame loop is uniform
 // across the wave, so frame j's nine coordinates are scalar loads.
 // Distances are FP64, computed exactly as the reference does.
-//   pass 1  (te_heap_kernel, one wave per block) the k + 1 smallest FP32
+//   pass 1  (te_heap_kernel, one wave per block, k <= 128; te_select_kernel,
+//           a radix select, for any k) the k + 1 smallest FP32
 //           keys fl32(dist), the query's own frame included, in a per-lane
 //           9-ary max-heap (rounding to FP32 is monotone, so the root is
 //           fl32(eps)): the root and its 9 children in registers, the rest
@@ -217,6 +218,79 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
     args.root[(size_t)blockIdx.x * 64 + lane] = top;
 }
 
+// Pass 1 for any k (the heap kernel's LDS stops at k = 128): per query, the
+// (k+1)-th smallest FP32 key of all Nv frames, the query's own included (the
+// heap's root), by a radix select over the key's bit pattern, one pass over
+// the frames per digit.  Keys are non-negative floats, so their bit patterns
+// order as unsigned integers, and bit 31 is clear: the digits are bits
+// 30..20 (2048 bins), 19..10 and 9..0 (1024 bins each).  Block b = one wave
+// = query block b % nqblocks of pair b / nqblocks; the wave takes its 64
+// queries one after another, its lanes striding over the frames, and
+// counts in an LDS histogram the frames whose key matches the digits chosen
+// so far.
+__global__ __launch_bounds__(64) void te_select_kernel(TeArgs args)
+{
+    __shared__ unsigned hist[2048];
+    const int lane = threadIdx.x;
+    const int nqb = args.nqblocks;
+    const long long p = blockIdx.x / nqb;
+    const int qb = blockIdx.x % nqb;
+    const int N = args.N, lag = args.lag, Nv = N - lag;
+    const double* X = args.traj + (size_t)args.a1[p] * N * 3;
+    const double* Y = args.traj + (size_t)args.a2[p] * N * 3;
+    const int nq = min(64, Nv - qb * 64);   // queries of this block (uniform)
+    float mine = INFINITY;                  // root of query qb * 64 + lane
+    for (int qi = 0; qi < nq; ++qi) {
+        double qc[9];
+        load_frame(X, Y, qb * 64 + qi, lag, qc);
+        unsigned prefix = 0u, mask = 0u;
+        unsigned want = (unsigned)args.k + 1u;   // 1-based rank among the matching keys
+#pragma unroll 1
+        for (int pass = 0; pass < 3; ++pass) {
+            const int sh = pass == 0 ? 20 : pass == 1 ? 10 : 0;
+            const int nb = pass == 0 ? 2048 : 1024;
+            for (int b = lane; b < nb; b += 64) hist[b] = 0u;
+            __syncthreads();
+            for (int j = lane; j < Nv; j += 64) {
+                double c[9];
+                load_frame(X, Y, j, lag, c);
+                double m = 0.0;
+#pragma unroll
+                for (int d = 0; d < 9; ++d) m = fmax(m, fabs(qc[d] - c[d]));
+                const unsigned u = __float_as_uint((float)m);
+                if ((u & mask) == prefix) atomicAdd(&hist[(u >> sh) & (unsigned)(nb - 1)], 1u);
+            }
+            __syncthreads();
+            // the digit whose bin holds the want-th matching key: lane l sums
+            // bins [l * per, (l + 1) * per) (read in a rotated order, fewer
+            // bank conflicts), an inclusive scan over the lanes finds the
+            // lane, which then walks its bins in order
+            const int per = nb / 64;
+            unsigned sum = 0u;
+            for (int i = 0; i < per; ++i) sum += hist[lane * per + ((i + lane) & (per - 1))];
+            unsigned inc = sum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned t = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += t;
+            }
+            const int wl = __ffsll((unsigned long long)__ballot(inc >= want)) - 1;
+            unsigned below = __shfl(inc - sum, wl, 64);
+            int digit = wl * per;
+            for (int i = 0; i < per; ++i) {   // uniform: every lane walks lane wl's bins
+                const unsigned h = hist[wl * per + i];
+                if (below + h >= want) { digit = wl * per + i; break; }
+                below += h;
+            }
+            want -= below;
+            prefix |= (unsigned)digit << sh;
+            mask |= (unsigned)(nb - 1) << sh;
+            __syncthreads();   // the histogram is cleared next
+        }
+        if (lane == qi) mine = __uint_as_float(prefix);
+    }
+    args.root[(size_t)blockIdx.x * 64 + lane] = mine;
+}
+
 // Pass 2 and the TE sum.  Wave w of block b runs item 4b + w = query block
 // item % nqblocks of pair item / nqblocks; its partial sum goes to
 // partial[item].
@@ -407,7 +481,9 @@ long long te_chunk_pairs(int nqblocks)
 
 hipError_t launch_te(const TeArgs& a, hipStream_t s)
 {
-    if (a.k < 1 || a.k > 128 || !a.root) return hipErrorInvalidValue;
+    if (a.k < 1 || !a.root) return hipErrorInvalidValue;
+    const bool heap = a.path == 1 || (a.path == 0 && a.k <= kTeHeapMaxK);
+    if (heap && a.k > kTeHeapMaxK) return hipErrorInvalidValue;
     const long long chunk = te_chunk_pairs(a.nqblocks);
     // heap positions kTeHeapRegs .. k in LDS (one slot when all fit in registers)
     const int slots = a.k + 1 > kTeHeapRegs ? a.k + 1 - kTeHeapRegs : 1;
@@ -419,7 +495,10 @@ hipError_t launch_te(const TeArgs& a, hipStream_t s)
         c.partial = a.partial + p0 * a.nqblocks;
         c.npairs = a.npairs - p0 < chunk ? a.npairs - p0 : chunk;
         const long long items = c.npairs * a.nqblocks;
-        hipLaunchKernelGGL(te_heap_kernel, dim3((unsigned)items), dim3(64), lds, s, c);
+        if (heap)
+            hipLaunchKernelGGL(te_heap_kernel, dim3((unsigned)items), dim3(64), lds, s, c);
+        else
+            hipLaunchKernelGGL(te_select_kernel, dim3((unsigned)items), dim3(64), 0, s, c);
         hipLaunchKernelGGL(te_count_kernel, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, s, c);
     }
     return hipGetLastError();

@@ -1,6 +1,7 @@
 // Host side of the transfer-entropy matrix (gmxcorr_te_matrix / _pairs).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -32,6 +33,9 @@ struct DevBuf {
     hipError_t alloc(size_t n) { return hipMalloc(&p, n * sizeof(T)); }
 };
 
+// Pass-1 kernel of later calls (gmxcorr_te_set_path): 0 automatic, 1 heap, 2 select.
+int g_path = 0;
+
 // mat != nullptr: full matrix over all ordered pairs (diagonal 0);
 // otherwise te_out[p] for the given list.
 int te_run(const double* traj, int natoms, int nframes, int k, int lag, long long npairs,
@@ -49,7 +53,10 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
     } else if (Nv <= k) {
         return gmxcorr::set_error(GMXCORR_EINVAL, "te: need nframes - lag > k");
     }
-    if (k > 128) return gmxcorr::set_error(GMXCORR_EUNSUP, "te: k above 128");
+    if (g_path == 1 && k > gmxcorr::kTeHeapMaxK)
+        return gmxcorr::set_error(GMXCORR_EUNSUP, "te: the heap kernel takes k <= 128");
+    for (size_t t = 0; t < (size_t)natoms * nframes * 3; ++t)
+        if (!std::isfinite(traj[t])) return gmxcorr::set_error(GMXCORR_EINVAL, "te: non-finite coordinate");
     if (!gmxcorr::device_ok()) return gmxcorr::set_error(GMXCORR_ENODEV, "no gfx950 device");
     std::vector<int> h1, h2;
     if (mat) {
@@ -89,7 +96,7 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
     HIPCHK(hipMemcpyAsync(dPsi.p, psi.data(), psi.size() * sizeof(double), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d1.p, a1, npairs * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d2.p, a2, npairs * sizeof(int), hipMemcpyHostToDevice, s));
-    gmxcorr::TeArgs ta{dTraj.p, nframes, lag, k, d1.p, d2.p, npairs, nqb, dPsi.p, dPart.p, dRoot.p};
+    gmxcorr::TeArgs ta{dTraj.p, nframes, lag, k, g_path, d1.p, d2.p, npairs, nqb, dPsi.p, dPart.p, dRoot.p};
     HIPCHK(hipEventRecord(e0, s));
     HIPCHK(gmxcorr::launch_te(ta, s));
     HIPCHK(hipEventRecord(e1, s));
@@ -135,6 +142,13 @@ int gmxcorr_te_pairs(const double* traj, int natoms, int nframes, int k, int lag
 {
     if (!te || !a1 || !a2 || npairs < 1) return gmxcorr::set_error(GMXCORR_EINVAL, "te_pairs: bad arguments");
     return te_run(traj, natoms, nframes, k, lag, npairs, a1, a2, nullptr, te, ms);
+}
+
+int gmxcorr_te_set_path(int path)
+{
+    if (path < 0 || path > 2) return gmxcorr::set_error(GMXCORR_EINVAL, "te_set_path: path must be 0, 1 or 2");
+    g_path = path;
+    return GMXCORR_OK;
 }
 
 }  // extern "C"

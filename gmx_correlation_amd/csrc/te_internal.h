@@ -8,7 +8,8 @@ struct TeArgs {
     const double* traj;   // [n][N][3] mean-centred coordinates
     int N;
     int lag;
-    int k;                // <= 128
+    int k;                // >= 1; the heap kernel takes k <= kTeHeapMaxK
+    int path;             // pass 1: 0 heap for k <= kTeHeapMaxK, else select; 1 heap; 2 select
     const int* a1;        // target atom of ordered pair p (TE(a2 -> a1))
     const int* a2;        // source atom
     long long npairs;
@@ -18,6 +19,8 @@ struct TeArgs {
     float* root;          // [te_chunk_pairs(nqblocks) * nqblocks * 64] per-query fl32(eps)
 };
 
+// Largest k of the heap kernel: its LDS holds (k - 9) * 256 bytes per wave.
+constexpr int kTeHeapMaxK = 128;
 // Heap positions per lane held in registers by the heap kernel (the root
 // and its 9 children); positions beyond them are in LDS.
 constexpr int kTeHeapRegs = 10;

@@ -86,3 +86,9 @@ def te_pairs(traj, a1, a2, k: int = 100, lag: int = 1):
                                             _lib.ptr(a1, _lib._ip), _lib.ptr(a2, _lib._ip),
                                             _lib.ptr(out), ctypes.byref(ms)))
     return out, ms.value
+
+
+def te_set_path(path: int) -> None:
+    """Pass-1 kernel of later TE calls in this process: 0 automatic (heap for
+    k <= 128, radix select above), 1 heap (k <= 128), 2 radix select."""
+    _lib.check(_lib.load().gmxcorr_te_set_path(int(path)))

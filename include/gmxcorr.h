@@ -200,12 +200,18 @@ int gmxcorr_gcmi_matrix(const double* traj, int natoms, int nframes, double* mat
  * CPU path uses them).  mat[a1*n+a2] = TE(a2 -> a1) at `lag`, diagonal 0,
  * all zero (with a warning) when nframes - lag <= k
  * (lib/transfer_entropy.cpp:63-221; the reference's GPU signature is
- * te_corrmatrix_metal, lib/te_metal.h).  k <= 128.  ms (optional): kernel
- * time.  te_pairs: the same for an explicit list of ordered pairs. */
+ * te_corrmatrix_metal, lib/te_metal.h).  Any 1 <= k < nframes - lag;
+ * non-finite coordinates are rejected.  ms (optional): kernel time.
+ * te_pairs: the same for an explicit list of ordered pairs. */
 int gmxcorr_te_matrix(const double* traj, int natoms, int nframes, int k, int lag, double* mat,
                       double* ms);
 int gmxcorr_te_pairs(const double* traj, int natoms, int nframes, int k, int lag,
                      long long npairs, const int* a1, const int* a2, double* te, double* ms);
+/* Kernel that finds each query's k-th neighbour distance in later TE calls
+ * of this process (no counterpart in the reference): 0 = automatic (a
+ * per-lane heap for k <= 128, a radix select above), 1 = the heap (k <= 128
+ * only), 2 = the radix select (any k).  Results are identical. */
+int gmxcorr_te_set_path(int path);
 
 /* ---- Batched uneven trajectories (no counterpart in the reference) --------
  * Many independent trajectories, each with its own atom count, frame count

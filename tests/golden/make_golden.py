@@ -63,6 +63,14 @@ def main() -> None:
     for lag in (1, 5, 10):
         save(f"te16_k5_lag{lag}.npy", O.ref_te_matrix(cte, 5, lag))
 
+    # Transfer entropy at k above 128 (the GPU's radix-select pass), on a
+    # 6-atom, 400-frame slice, and on the same slice quantised to 0.02 nm
+    # (exact ties at the k-th distance).
+    cte6 = np.ascontiguousarray(c0[:400, 16:22])
+    save("te6_k200_lag3.npy", O.ref_te_matrix(cte6, 200, 3))
+    cq = np.round(cte6 / 0.02).astype(np.float32) * np.float32(0.02)
+    save("te6q_k150_lag1.npy", O.ref_te_matrix(cq, 150, 1))
+
     # Quantised coordinates: exact ties in the neighbour search.
     ct = tie_spec()
     meta["ties_sha256"] = synth.coords_sha256(ct)

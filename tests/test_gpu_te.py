@@ -1,5 +1,6 @@
 """GPU transfer entropy against the reference's own output (golden, 16 atoms
-x 300 frames, k = 5, lags 1/5/10) and the FP64 oracle.  Distances, the k-th
+x 300 frames, k = 5, lags 1/5/10; 6 atoms x 400 frames at k = 200 and, with
+ties, k = 150) and the FP64 oracle.  Distances, the k-th
 neighbour distance and the strict counts are FP64 exactly as the reference
 computes them, so only the summation order differs: |dTE| <= 1e-12."""
 import numpy as np
@@ -76,3 +77,60 @@ def test_te_matrix_over_two_launch_chunks(gpu, oracle_lib):
     for a1, a2 in ((0, 1), (1000, 5), (1997, 2098), (1998, 300), (2050, 0), (2099, 2098)):
         ref = oracle_lib.te_pair(traj, a1, a2, 5, 1)
         assert abs(mat[a1, a2] - ref) <= 1e-12, (a1, a2)
+
+
+def test_te_k_above_128_matches_reference(gpu):
+    """k above the heap kernel's 128 goes through the radix-select pass:
+    the reference's own output at k = 200 (lag 3) and, on coordinates
+    quantised to 0.02 nm (exact ties at the k-th distance), k = 150."""
+    c = synth.generate(synth.SMALL_CPU)
+    c6 = np.ascontiguousarray(c[:400, 16:22])
+    mat = dense.te_matrix(ksg.center(c6), k=200, lag=3)
+    assert np.max(np.abs(mat - golden("te6_k200_lag3.npy"))) <= 1e-12
+    cq = np.round(c6 / 0.02).astype(np.float32) * np.float32(0.02)
+    mat = dense.te_matrix(ksg.center(cq), k=150, lag=1)
+    assert np.max(np.abs(mat - golden("te6q_k150_lag1.npy"))) <= 1e-12
+
+
+@pytest.mark.parametrize("path", [1, 2])
+def test_te_largest_k_vs_oracle(gpu, oracle_lib, path):
+    """k = N - lag - 1, the largest the estimator takes (the k-th neighbour
+    is the farthest frame), through each pass-1 kernel; and k = 399 of 401
+    valid frames through the radix select."""
+    traj = ksg.center(synth.generate(synth.TrajSpec(natoms=3, nframes=130, nmodes=2, seed=9)))
+    dense.te_set_path(path)
+    try:
+        mat = dense.te_matrix(traj, k=127, lag=2)
+        assert np.max(np.abs(mat - oracle_lib.te_matrix(traj, 127, 2))) <= 1e-12
+        mat = dense.te_matrix(traj, k=128, lag=1)
+        assert np.max(np.abs(mat - oracle_lib.te_matrix(traj, 128, 1))) <= 1e-12
+    finally:
+        dense.te_set_path(0)
+    traj = ksg.center(synth.generate(synth.TrajSpec(natoms=3, nframes=402, nmodes=2, seed=10)))
+    mat = dense.te_matrix(traj, k=400, lag=1)
+    assert np.max(np.abs(mat - oracle_lib.te_matrix(traj, 400, 1))) <= 1e-12
+
+
+@pytest.mark.parametrize("case", ["continuous", "quantised", "duplicates"])
+def test_te_select_pass_equals_heap_pass(gpu, case):
+    """Both pass-1 kernels find the same k-th neighbour key, so the TE
+    values are identical, bit for bit, at k <= 128 where both run."""
+    rng = np.random.default_rng(31)
+    if case == "duplicates":
+        c = np.zeros((300, 5, 3), np.float32)
+        c[:, :, 0] = rng.integers(0, 2, (300, 5))
+        c[:, 4, :] += rng.standard_normal((300, 3)).astype(np.float32)
+    else:
+        c = synth.generate(synth.TrajSpec(natoms=5, nframes=700, nmodes=3, seed=12))
+        if case == "quantised":
+            c = np.round(c / 0.02).astype(np.float32) * np.float32(0.02)
+    traj = ksg.center(c)
+    try:
+        for k, lag in ((1, 1), (5, 2), (100, 1), (128, 4)):
+            dense.te_set_path(1)
+            heap = dense.te_matrix(traj, k=k, lag=lag)
+            dense.te_set_path(2)
+            sel = dense.te_matrix(traj, k=k, lag=lag)
+            assert np.array_equal(heap, sel), (case, k, lag)
+    finally:
+        dense.te_set_path(0)

@@ -213,3 +213,23 @@ def test_plan_layout_memory_sizing_without_gpu(hiplib):
     assert ksg.plan_layout(10, 600, 10, 10**6) == {"slots": 0, "wave_atoms": 0, "stores": 0}
     with pytest.raises(_lib.GmxcorrError):
         ksg.plan_layout(1, 600, 10, 10**9)
+
+
+def test_te_argument_errors_before_the_device(hiplib):
+    """The TE entry points reject a bad kernel choice, k above the heap
+    kernel's 128 when the heap is forced, and non-finite coordinates before
+    they look for a device (so these hold without one)."""
+    from gmx_correlation_amd import dense
+    traj = ksg.center(synth.generate(synth.TrajSpec(natoms=2, nframes=300, nmodes=2, seed=1)))
+    with pytest.raises(_lib.GmxcorrError):
+        dense.te_set_path(3)
+    dense.te_set_path(1)
+    try:
+        with pytest.raises(_lib.GmxcorrError, match="128"):
+            dense.te_matrix(traj, k=129, lag=1)
+    finally:
+        dense.te_set_path(0)
+    bad = traj.copy()
+    bad[1, 7, 2] = np.nan
+    with pytest.raises(_lib.GmxcorrError, match="non-finite"):
+        dense.te_matrix(bad, k=5, lag=1)

@@ -216,6 +216,18 @@ def test_te_matches_reference(small, oracle_lib, lag):
                        rtol=1e-12, atol=1e-12)
 
 
+def test_te_k_above_128_matches_reference(small, oracle_lib):
+    """k = 200 at lag 3, and k = 150 on the same slice quantised to 0.02 nm
+    (exact ties at the k-th distance), 6 atoms x 400 frames."""
+    c, _, _, _ = small
+    c6 = np.ascontiguousarray(c[:400, 16:22])
+    assert np.allclose(oracle_lib.te_matrix(oracle_lib.center(c6), 200, 3),
+                       golden("te6_k200_lag3.npy"), rtol=1e-12, atol=1e-12)
+    cq = np.round(c6 / 0.02).astype(np.float32) * np.float32(0.02)
+    assert np.allclose(oracle_lib.te_matrix(oracle_lib.center(cq), 150, 1),
+                       golden("te6q_k150_lag1.npy"), rtol=1e-12, atol=1e-12)
+
+
 def test_te_short_series_is_zero(oracle_lib):
     traj = np.random.default_rng(0).standard_normal((3, 8, 3))
     assert not oracle_lib.te_matrix(traj, 5, 4).any()
