@@ -217,7 +217,8 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
         std::vector<float*> dD(ts.size(), nullptr);
         gmxcorr::TrajDesc* dDesc = nullptr;
         int *dItemTraj = nullptr, *dA1 = nullptr, *dA2 = nullptr, *dQuads = nullptr;
-        double *dPartial = nullptr, *dFixsum = nullptr, *dMI = nullptr;
+        double *dPartial = nullptr, *dMI = nullptr;
+        gmxcorr::FixSum* dFixsum = nullptr;
         gmxcorr::FixEntry *dFix = nullptr, *dLong = nullptr;
         unsigned* dCnt = nullptr;
         unsigned long long* dQueue = nullptr;
@@ -320,7 +321,7 @@ int gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nbl
             HIPCHK(hipMalloc(&dA1, nitems * sizeof(int)));
             HIPCHK(hipMalloc(&dA2, nitems * sizeof(int)));
             HIPCHK(hipMalloc(&dPartial, nitems * sizeof(double)));
-            HIPCHK(hipMalloc(&dFixsum, nitems * sizeof(double)));
+            HIPCHK(hipMalloc(&dFixsum, nitems * sizeof(gmxcorr::FixSum)));
             HIPCHK(hipMalloc(&dMI, nitems * sizeof(double)));
             HIPCHK(hipMalloc(&dCnt, 4 * sizeof(unsigned)));
             HIPCHK(hipMalloc(&dQueue, 8 * sizeof(unsigned long long)));

@@ -1888,7 +1888,7 @@ __global__ __launch_bounds__(256) void ksg_pair_long_kernel(KsgArgs args)
         o.rc_flag = nullptr;
     }
     const double acc = rows_long<C>(lr, N, T, i0, i1, 1, p, lds[w], lane, o);
-    if (lane == 0) atomicAdd(&args.fixsum[p], acc);
+    if (lane == 0) fix_add(&args.fixsum[p], acc);
 }
 
 // Rows the long-trajectory kernel queued to longq (the (k+1)-th neighbour
@@ -1907,7 +1907,7 @@ __global__ __launch_bounds__(256) void ksg_long_rows_coop_kernel(KsgArgs args, u
     const float* Db = args.D + (size_t)args.a2[p] * N * args.P;
     const double acc = row_coop<kMaxFrames / 1024>(Da, Db, args.P, N, args.K + 1, fe.row, p, sh,
                                                    lane, w, row_out(args));
-    if (threadIdx.x == 0) atomicAdd(&args.fixsum[p], acc);
+    if (threadIdx.x == 0) fix_add(&args.fixsum[p], acc);
     }
 }
 
@@ -1933,7 +1933,7 @@ __global__ __launch_bounds__(256) void ksg_long_rows_kernel(KsgArgs args, unsign
     const double acc = rows_single<kSortedMaxFrames / 256>(
         args.D + (size_t)a1 * N * args.P, args.D + (size_t)a2 * N * args.P, sr, args.P, N,
         args.K + 1, fe.row, fe.row + 1, 1, p, lds[w], lane, row_out(args));
-    if (lane == 0) atomicAdd(&args.fixsum[p], acc);
+    if (lane == 0) fix_add(&args.fixsum[p], acc);
     }
 }
 
@@ -2016,7 +2016,7 @@ __device__ __forceinline__ void batch_long(const TrajDesc& d, const KsgBatchArgs
     const double acc = d.s4 == 2   ? rows_long<2>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o)
                        : d.s4 == 3 ? rows_long<3>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o)
                                    : rows_long<4>(lr, d.N, d.K + 1, i0, i1, 1, gp, hist, lane, o);
-    if (lane == 0) atomicAdd(&b.fixsum[gp], acc);
+    if (lane == 0) fix_add(&b.fixsum[gp], acc);
 }
 
 template <int S4>
@@ -2151,7 +2151,7 @@ __global__ __launch_bounds__(256) void ksg_batch_kernel(KsgBatchArgs b)
 // Long rows of the batch (kinds 2 and 3): one block per entry runs the
 // cooperative streaming row of its trajectory, the psi sum added to
 // fixsum[item].  The register tile S4 covers the trajectory's frame count.
-__global__ __launch_bounds__(256) void ksg_batch_long_rows_kernel(KsgBatchArgs b, double* fixsum,
+__global__ __launch_bounds__(256) void ksg_batch_long_rows_kernel(KsgBatchArgs b, FixSum* fixsum,
                                                                   unsigned nent)
 {
     __shared__ CoopLds sh;
@@ -2183,7 +2183,7 @@ __global__ __launch_bounds__(256) void ksg_batch_long_rows_kernel(KsgBatchArgs b
         acc = row_coop<6>(Da, Db, d.P, d.N, T, fe.row, item, sh, lane, w, o);
     else
         acc = row_coop<8>(Da, Db, d.P, d.N, T, fe.row, item, sh, lane, w, o);
-    if (threadIdx.x == 0) atomicAdd(&fixsum[item], acc);
+    if (threadIdx.x == 0) fix_add(&fixsum[item], acc);
     }
 }
 
@@ -2352,7 +2352,7 @@ __global__ __launch_bounds__(256) void ksg_fixup_kernel(FixupArgs args, unsigned
         const double* xb = d.X + (size_t)(3 * args.a2[p]) * N;
         const Fp64Row o = fp64_row(xa, xb, N, d.K, i, fe.eps_bits, hist, lane);
         if (lane == 0) {
-            atomicAdd(&args.fixsum[p], d.psi[o.nx] + d.psi[o.ny]);
+            fix_add(&args.fixsum[p], d.psi[o.nx] + d.psi[o.ny]);
             if (o.tie) atomicAdd(&args.counters[1], 1u);
             if (args.rc_nx) {
                 const size_t off = (size_t)p * N + i;
@@ -2409,11 +2409,11 @@ __global__ __launch_bounds__(256) void ksg_coord_kernel(KsgArgs args)
         __syncthreads();
     }
 }
-__global__ void ksg_clear_kernel(double* fixsum, long long npairs, unsigned* counters,
+__global__ void ksg_clear_kernel(FixSum* fixsum, long long npairs, unsigned* counters,
                                  unsigned long long* queue, double* partial, int nqueue)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < npairs) fixsum[t] = 0.0;
+    if (t < npairs) fixsum[t] = 0ull;
     if (partial && t < npairs) partial[t] = 0.0;
     if (t < 3) counters[t] = 0u;
     if (queue && t < nqueue) queue[t] = 0ull;
@@ -2428,7 +2428,7 @@ __global__ void ksg_finalize_kernel(FixupArgs args, long long nitems, const doub
     const TrajDesc d = args.traj[args.item_traj ? args.item_traj[p] : 0];
     double sum = 0.0;
     for (int r = 0; r < nsplit; ++r) sum += partial[p * nsplit + r];
-    mi[p] = d.psiN_phiK - (sum + args.fixsum[p]) / d.N;
+    mi[p] = d.psiN_phiK - (sum + fix_value(args.fixsum[p])) / d.N;
 }
 
 // MI of the device-pointer coordinate path: psi[N] + (psi[K] - 1/K) - sum/N
@@ -2515,7 +2515,7 @@ hipError_t launch_build_prefix(const float* D, int natoms, int N, long long P, f
     return hipGetLastError();
 }
 
-hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
+hipError_t launch_ksg_clear(FixSum* fixsum, long long n, unsigned* counters,
                             unsigned long long* queue, hipStream_t s, double* partial, int nqueue)
 {
     if (nqueue < 0 || nqueue > 256) return hipErrorInvalidValue;
@@ -2711,7 +2711,7 @@ bool batch_variant_ok(int kind, int s4, int N, int K)
     return false;
 }
 
-hipError_t launch_ksg_batch_long_rows(const KsgBatchArgs& b, double* fixsum, unsigned nlong,
+hipError_t launch_ksg_batch_long_rows(const KsgBatchArgs& b, FixSum* fixsum, unsigned nlong,
                                       hipStream_t s)
 {
     if (nlong == 0) return hipSuccess;

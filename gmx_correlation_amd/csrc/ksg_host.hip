@@ -195,7 +195,7 @@ struct gmxcorr_ksg_plan {
     long long pair_cap = 0;
     double* dPartial = nullptr;
     long long partial_cap = 0;   // doubles
-    double* dFixsum = nullptr;
+    gmxcorr::FixSum* dFixsum = nullptr;
     double* dMI = nullptr;
     FixEntry* dFix = nullptr;
     unsigned fixcap = 0;
@@ -255,7 +255,7 @@ int ensure_pairs(gmxcorr_ksg_plan* pl, long long npairs, int slots_per_pair)
         pl->pair_cap = 0;   // a failed allocation below leaves no stale capacity
         HIPCHK(hipMalloc(&pl->dA1, npairs * sizeof(int)));
         HIPCHK(hipMalloc(&pl->dA2, npairs * sizeof(int)));
-        HIPCHK(hipMalloc(&pl->dFixsum, npairs * sizeof(double)));
+        HIPCHK(hipMalloc(&pl->dFixsum, npairs * sizeof(gmxcorr::FixSum)));
         HIPCHK(hipMalloc(&pl->dMI, npairs * sizeof(double)));
         pl->pair_cap = npairs;
         pl->all_pairs = false;

@@ -7,6 +7,22 @@
 
 namespace gmxcorr {
 
+// Sums that blocks and recount waves add into in any order (kind 3's row
+// chunks, recounted rows): each term is rounded to a multiple of 2^-44 and
+// added as a 64-bit two's-complement integer, so the total does not depend
+// on the order of the adds and MI is reproducible bit for bit.  A pair's
+// total stays below 2^19 in magnitude (at most 8192 rows of psi[nx] +
+// psi[ny] < 19 each), inside the 2^63 / 2^44 range; the rounding moves MI
+// by less than 3e-14.
+using FixSum = unsigned long long;
+constexpr double kFixScale = 17592186044416.0;   // 2^44
+__device__ inline void fix_add(FixSum* s, double v)
+{
+    atomicAdd(s, (FixSum)__double2ll_rn(v * kFixScale));
+}
+__host__ __device__ inline double fix_value(FixSum s) { return (double)(long long)s / kFixScale; }
+
+
 // Largest accepted max - min of one normalised coordinate row.  z-scores of
 // N frames lie within sqrt(N) of zero, far inside it; the pair kernels rely
 // on every distance being below 2^11 (padding 3072, keys below 2^12).
@@ -37,7 +53,7 @@ struct KsgArgs {
     const int*    a1;     // [npairs]
     const int*    a2;     // [npairs]
     double*       partial;   // [npairs][nsplit] sums of psi[nx]+psi[ny] over unflagged rows
-    double*       fixsum;    // [npairs] same over FP64-recounted rows
+    FixSum*       fixsum;    // [npairs] same over FP64-recounted rows (fix_add)
     double*       mi;        // [npairs]
     FixEntry*     fixq;      // [fixcap]
     unsigned*     counters;  // [0] queued rows, [1] exact ties, [2] long rows
@@ -96,7 +112,7 @@ struct FixupArgs {
     const int* a1;
     const int* a2;
     const FixEntry* fixq;
-    double* fixsum;
+    FixSum* fixsum;
     unsigned* counters;
     int* rc_nx;
     int* rc_ny;
@@ -117,7 +133,7 @@ struct KsgBatchArgs {
     unsigned long long* queue;   // [8] work counters, one per XCD
     FixEntry* longq;   // rows whose (k+1)-th neighbour lies beyond the prefix
     unsigned longcap;
-    double* fixsum;    // [pairs] kind 3 adds its row sums here
+    FixSum* fixsum;    // [pairs] kind 3 adds its row sums here (fix_add)
     int ntraj;         // traj[] in work order (decreasing frame count)
 };
 
@@ -196,7 +212,7 @@ std::vector<int> pair_quads(int n);
 
 // Zeroes fixsum[0..n), partial[0..n) (if given), the counters and nqueue
 // work counters.
-hipError_t launch_ksg_clear(double* fixsum, long long n, unsigned* counters,
+hipError_t launch_ksg_clear(FixSum* fixsum, long long n, unsigned* counters,
                             unsigned long long* queue, hipStream_t s, double* partial = nullptr,
                             int nqueue = 1);
 // kind 0: one wave per frame row streaming both distance rows, N <= 256*s4
@@ -262,7 +278,7 @@ inline int long_prefix_chunks(int N, int K)
 hipError_t launch_ksg_batch(const KsgBatchArgs& b, int nblocks, int kinds, hipStream_t s);
 // The batch's long rows (b.longq, nlong entries): cooperative streaming row,
 // psi sum added to fixsum[item]; tie rows go to b.fixq.
-hipError_t launch_ksg_batch_long_rows(const KsgBatchArgs& b, double* fixsum, unsigned nlong,
+hipError_t launch_ksg_batch_long_rows(const KsgBatchArgs& b, FixSum* fixsum, unsigned nlong,
                                       hipStream_t s);
 hipError_t launch_selftest(int* out, hipStream_t s);
 

@@ -211,3 +211,36 @@ def test_xtc_file_frames_bitexact_vs_fp64_recount(gpu, oracle_lib, tmp_path, nfr
     print(f"domain: XTC frames: {st['fixup_rows']} rows recounted, {st['ties']} exact ties",
           flush=True)
     _check_counts(mi, d, _brute_all(oracle_lib, rows, psi, a1, a2, k))
+
+
+@pytest.mark.parametrize("nframes,k,kind,quantum", [(2000, 10, 2, 0.001), (3000, 20, 3, 0.001),
+                                                    (3000, 20, 3, 0.0)])
+def test_repeated_runs_are_bit_identical(gpu, nframes, k, kind, quantum):
+    """Sums that blocks and recount waves add in any order (the long kind's
+    row chunks, FP64-recounted rows) are kept in 64-bit fixed point, so
+    every run of a plan gives the same bits; coordinates on the 0.001 nm
+    grid make many recounted rows.  The same holds for the batched launch,
+    whose persistent blocks take work in a different order every run."""
+    c = synth.generate(synth.TrajSpec(natoms=24, nframes=nframes, nmodes=4, seed=nframes + k))
+    if quantum:
+        c = np.round(c / quantum).astype(np.float32) * np.float32(quantum)
+    rows, psi = ksg.kraskov_prepare(ksg.center(c))
+    with ksg.KsgPlan(24, nframes, k) as plan:
+        plan.load(rows, psi)
+        plan.set_variant(kind, 2 if kind == 3 else 0)
+        runs = [plan.run_all() for _ in range(4)]
+        st = plan.stats()
+    assert st["fixup_rows"] > 0 or kind == 3
+    for r in runs[1:]:
+        assert np.array_equal(r, runs[0])
+    from gmx_correlation_amd import batch
+    outs = []
+    for _ in range(3):
+        b = batch.KsgBatch()
+        b.add(rows, psi, k)
+        b.add(rows[:30], psi, k)
+        b.run(max_row_bytes=20 * 10**9, nblocks=512)
+        outs.append((b.result(0)[0].copy(), b.result(1)[0].copy()))
+        b.close()
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])

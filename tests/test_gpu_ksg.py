@@ -392,9 +392,9 @@ def test_xtc_precision_coordinates(gpu, oracle_lib, N, k):
 
 
 def test_all_pairs_over_two_launches(gpu, oracle_lib):
-    """1500 atoms x 64 frames: 1,124,250 pairs in 281,063 2x2 tiles times 16
-    row ranges, 4.5 M blocks, more than one dispatch carries (2^22 blocks of
-    256 work-items), so the sorted-prefix kernel goes out in two launches;
+    """1500 atoms x 64 frames: 1,124,250 pairs in 281,063 2x2 tiles times 128
+    row ranges, 36.0 M blocks, more than one dispatch carries (2^22 blocks of
+    256 work-items), so the sorted-prefix kernel goes out in nine launches;
     3000 pairs spread over the matrix equal the oracle."""
     c = synth.generate(synth.TrajSpec(natoms=1500, nframes=64, nmodes=6, seed=64))
     rows, psi = ksg.kraskov_prepare(ksg.center(c))
@@ -411,9 +411,9 @@ def test_all_pairs_over_two_launches(gpu, oracle_lib):
 
 def test_long_trajectory_all_pairs_over_two_launches(gpu, oracle_lib):
     """720 atoms x 2100 frames (prefix-store path, 512-frame prefix at k =
-    10): 64,710 2x2 tiles times 66 chunks of 32 rows, 4.27 M blocks, so the
-    kernel goes out in two launches (nearly every tile has a chunk in the
-    second).  600 pairs spread over the matrix equal the oracle's grid
+    10): 64,710 2x2 tiles times 132 chunks of 16 rows, 8.54 M blocks, so the
+    kernel goes out in three launches (every tile has chunks in more than
+    one).  600 pairs spread over the matrix equal the oracle's grid
     restatement, except pairs with an exact FP64 tie at the k-th distance,
     where the GPU counts every tied frame: those equal the FP64 brute-force
     recount, which follows the same rule."""
