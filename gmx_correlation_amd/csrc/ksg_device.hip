@@ -411,63 +411,93 @@ __device__ __forceinline__ uint32_t lane_xor(uint32_t v, int lane, int m)
     return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ m) << 2, (int)v);
 }
 
-// In-thread bitonic step of stride J (J < E): pairs (e, e + J) with bit J of
-// e clear; ascending where bit k of the global index is clear.
-template <int E, int J>
-__device__ __forceinline__ void sort_step_inthread(uint32_t (&r)[E], int base, int k)
+// One pass (K, J) of the bitonic network over the M = 256*E values held E
+// per thread (thread t owns positions E*t .. E*t+E-1): position x is
+// compared with x ^ J, the smaller kept below where bit K of x is clear (an
+// ascending run), above where it is set.  K and J are compile-time, so the
+// direction is a constant per register (K < E) or one flag per thread.
+//   J < E        pairs inside the thread's registers;
+//   J / E < 64   the partner thread in the wave: DPP quad permutes for
+//                thread distance 1 and 2, ds_bpermute beyond;
+//   otherwise    through LDS (lds: M words), between block barriers.
+template <int E, int K, int J>
+__device__ __forceinline__ void bitonic_pass(uint32_t (&r)[E], uint32_t* lds, int t)
 {
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        if (e & J) continue;
-        const uint32_t a = r[e], b = r[e + J];
-        const uint32_t lo = min(a, b), hi = max(a, b);
-        const bool asc = ((base + e) & k) == 0;
-        r[e] = asc ? lo : hi;
-        r[e + J] = asc ? hi : lo;
-    }
-}
-
-// Ascending sort of the M = 256*E values held E per thread (thread t owns
-// positions E*t .. E*t+E-1).  lds: M words, used for strides >= 64E.
-template <int E>
-__device__ __forceinline__ void sort_u32_blocked(uint32_t (&r)[E], uint32_t* lds, int t)
-{
-    constexpr int M = 256 * E;
-    const int lane = t & (kWave - 1);
     const int base = E * t;
-    for (int k = 2; k <= M; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j < E) {
-                if constexpr (E > 4) { if (j == 4) { sort_step_inthread<E, (E > 4 ? 4 : 1)>(r, base, k); continue; } }
-                if constexpr (E > 2) { if (j == 2) { sort_step_inthread<E, (E > 2 ? 2 : 1)>(r, base, k); continue; } }
-                if constexpr (E > 1) sort_step_inthread<E, 1>(r, base, k);
-                continue;
-            }
-            const int m = j / E;   // thread distance of the partner
-            if (m < kWave) {
-                const bool upper = (lane & m) != 0;
+    if constexpr (J < E) {
 #pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const uint32_t o = lane_xor(r[e], lane, m);
-                    const bool asc = ((base + e) & k) == 0;
-                    r[e] = (asc != upper) ? min(r[e], o) : max(r[e], o);
-                }
+        for (int e = 0; e < E; ++e) {
+            if (e & J) continue;
+            const uint32_t a = r[e], b = r[e + J];
+            const uint32_t lo = min(a, b), hi = max(a, b);
+            if constexpr (K < E) {
+                const bool asc = (e & K) == 0;
+                r[e] = asc ? lo : hi;
+                r[e + J] = asc ? hi : lo;
             } else {
-                __syncthreads();   // earlier readers of lds are done
+                const bool asc = (base & K) == 0;
+                r[e] = asc ? lo : hi;
+                r[e + J] = asc ? hi : lo;
+            }
+        }
+    } else {
+        constexpr int m = J / E;   // thread distance of the partner
+        const bool asc = (base & K) == 0;   // K >= 2J > E: the same for all e
+        if constexpr (m < kWave) {
+            const int lane = t & (kWave - 1);
+            const bool takemin = asc != ((lane & m) != 0);
 #pragma unroll
-                for (int e = 0; e < E; ++e) lds[base + e] = r[e];
-                __syncthreads();
-                const bool upper = (t & m) != 0;
-                const int pb = E * (t ^ m);
+            for (int e = 0; e < E; ++e) {
+                uint32_t o;
+                if constexpr (m == 1)
+                    o = (uint32_t)dpp<0xB1>((int)r[e]);   // quad_perm [1,0,3,2]
+                else if constexpr (m == 2)
+                    o = (uint32_t)dpp<0x4E>((int)r[e]);   // quad_perm [2,3,0,1]
+                else
+                    o = (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ m) << 2, (int)r[e]);
+                r[e] = takemin ? min(r[e], o) : max(r[e], o);
+            }
+        } else {
+            __syncthreads();   // earlier readers of lds are done
 #pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const uint32_t o = lds[pb + e];
-                    const bool asc = ((base + e) & k) == 0;
-                    r[e] = (asc != upper) ? min(r[e], o) : max(r[e], o);
-                }
+            for (int e = 0; e < E; ++e) lds[base + e] = r[e];
+            __syncthreads();
+            const bool takemin = asc != ((t & m) != 0);
+            const int pb = E * (t ^ m);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t o = lds[pb + e];
+                r[e] = takemin ? min(r[e], o) : max(r[e], o);
             }
         }
     }
+}
+
+template <int E, int K, int J>
+struct BitonicMerge {
+    static __device__ __forceinline__ void run(uint32_t (&r)[E], uint32_t* lds, int t)
+    {
+        bitonic_pass<E, K, J>(r, lds, t);
+        if constexpr (J > 1) BitonicMerge<E, K, J / 2>::run(r, lds, t);
+    }
+};
+
+template <int E, int K>
+struct BitonicSort {
+    static __device__ __forceinline__ void run(uint32_t (&r)[E], uint32_t* lds, int t)
+    {
+        BitonicMerge<E, K, K / 2>::run(r, lds, t);
+        if constexpr (K < 256 * E) BitonicSort<E, 2 * K>::run(r, lds, t);
+    }
+};
+
+// Ascending sort of the M = 256*E values held E per thread, the whole
+// network unrolled at compile time.  lds: M words, used for thread
+// distances >= 64.
+template <int E>
+__device__ __forceinline__ void sort_u32_blocked(uint32_t (&r)[E], uint32_t* lds, int t)
+{
+    BitonicSort<E, 2>::run(r, lds, t);
 }
 
 // Exclusive prefix sum over the block's 256 threads (4 waves); total out.
