@@ -411,93 +411,101 @@ __device__ __forceinline__ uint32_t lane_xor(uint32_t v, int lane, int m)
     return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ m) << 2, (int)v);
 }
 
-// One pass (K, J) of the bitonic network over the M = 256*E values held E
-// per thread (thread t owns positions E*t .. E*t+E-1): position x is
-// compared with x ^ J, the smaller kept below where bit K of x is clear (an
-// ascending run), above where it is set.  K and J are compile-time, so the
-// direction is a constant per register (K < E) or one flag per thread.
-//   J < E        pairs inside the thread's registers;
-//   J / E < 64   the partner thread in the wave: DPP quad permutes for
-//                thread distance 1 and 2, ds_bpermute beyond;
-//   otherwise    through LDS (lds: M words), between block barriers.
-template <int E, int K, int J>
-__device__ __forceinline__ void bitonic_pass(uint32_t (&r)[E], uint32_t* lds, int t)
+// Lane exchange for the sort: the value of lane lane ^ MT, by DPP where a
+// permute covers the distance (quad permutes, half-row and row mirrors),
+// else by ds_bpermute; 64-bit keys as two words.
+template <int MT>
+__device__ __forceinline__ uint32_t sort_xchg(uint32_t v, int lane)
 {
-    const int base = E * t;
-    if constexpr (J < E) {
+    if constexpr (MT == 1) return (uint32_t)dpp<0xB1>((int)v);    // quad_perm [1,0,3,2]
+    else if constexpr (MT == 2) return (uint32_t)dpp<0x4E>((int)v);   // quad_perm [2,3,0,1]
+    else if constexpr (MT == 3) return (uint32_t)dpp<0x1B>((int)v);   // quad_perm [3,2,1,0]
+    else if constexpr (MT == 7) return (uint32_t)dpp<0x141>((int)v);  // row_half_mirror
+    else if constexpr (MT == 15) return (uint32_t)dpp<0x140>((int)v); // row_mirror
+    else return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ MT) << 2, (int)v);
+}
+
+template <int MT>
+__device__ __forceinline__ uint64_t sort_xchg(uint64_t v, int lane)
+{
+    const uint32_t lo = sort_xchg<MT>((uint32_t)v, lane);
+    const uint32_t hi = sort_xchg<MT>((uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// One pass of the bitonic sort over the M = 256*E keys held E per thread
+// (thread t owns positions E*t .. E*t+E-1).  Merge step K starts with the
+// mirror pass (position x against x ^ (K - 1) in its K-block, Flip), then
+// passes J = K/4 .. 1 (x against x ^ J); the lower position always keeps
+// the smaller key, so every run is ascending and no pass has a direction
+// (in-thread pairs are a min and a max into fixed registers).
+//   partner in the thread's registers   bit H of the position below E;
+//   partner thread in the wave          sort_xchg;
+//   partner thread in another wave      through LDS (lds: M keys),
+//                                       between block barriers.
+template <typename T, int E, int K, int J, bool Flip>
+__device__ __forceinline__ void sort_pass(T (&r)[E], T* lds, int t)
+{
+    constexpr int H = Flip ? K / 2 : J;   // the position bit of the upper partner
+    if constexpr (H < E) {
+        constexpr int X = Flip ? K - 1 : J;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            if (e & J) continue;
-            const uint32_t a = r[e], b = r[e + J];
-            const uint32_t lo = min(a, b), hi = max(a, b);
-            if constexpr (K < E) {
-                const bool asc = (e & K) == 0;
-                r[e] = asc ? lo : hi;
-                r[e + J] = asc ? hi : lo;
-            } else {
-                const bool asc = (base & K) == 0;
-                r[e] = asc ? lo : hi;
-                r[e + J] = asc ? hi : lo;
-            }
+            if (e & H) continue;
+            const T a = r[e], b = r[e ^ X];
+            const bool lt = a < b;
+            r[e] = lt ? a : b;
+            r[e ^ X] = lt ? b : a;
         }
     } else {
-        constexpr int m = J / E;   // thread distance of the partner
-        const bool asc = (base & K) == 0;   // K >= 2J > E: the same for all e
-        if constexpr (m < kWave) {
+        // the partner thread, and its element: e ^ (E - 1) for the mirror
+        constexpr int MT = Flip ? K / E - 1 : J / E;
+        constexpr int HB = H / E;
+        const bool lower = (t & HB) == 0;
+        T o[E];
+        if constexpr (MT < kWave) {
             const int lane = t & (kWave - 1);
-            const bool takemin = asc != ((lane & m) != 0);
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                uint32_t o;
-                if constexpr (m == 1)
-                    o = (uint32_t)dpp<0xB1>((int)r[e]);   // quad_perm [1,0,3,2]
-                else if constexpr (m == 2)
-                    o = (uint32_t)dpp<0x4E>((int)r[e]);   // quad_perm [2,3,0,1]
-                else
-                    o = (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ m) << 2, (int)r[e]);
-                r[e] = takemin ? min(r[e], o) : max(r[e], o);
-            }
+            for (int e = 0; e < E; ++e) o[e] = sort_xchg<MT>(r[Flip ? E - 1 - e : e], lane);
         } else {
             __syncthreads();   // earlier readers of lds are done
 #pragma unroll
-            for (int e = 0; e < E; ++e) lds[base + e] = r[e];
+            for (int e = 0; e < E; ++e) lds[E * t + e] = r[e];
             __syncthreads();
-            const bool takemin = asc != ((t & m) != 0);
-            const int pb = E * (t ^ m);
+            const int pb = E * (t ^ MT);
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const uint32_t o = lds[pb + e];
-                r[e] = takemin ? min(r[e], o) : max(r[e], o);
-            }
+            for (int e = 0; e < E; ++e) o[e] = lds[pb + (Flip ? E - 1 - e : e)];
         }
+#pragma unroll
+        for (int e = 0; e < E; ++e) r[e] = lower == (o[e] < r[e]) ? o[e] : r[e];
     }
 }
 
-template <int E, int K, int J>
-struct BitonicMerge {
-    static __device__ __forceinline__ void run(uint32_t (&r)[E], uint32_t* lds, int t)
+template <typename T, int E, int K, int J>
+struct SortHalves {
+    static __device__ __forceinline__ void run(T (&r)[E], T* lds, int t)
     {
-        bitonic_pass<E, K, J>(r, lds, t);
-        if constexpr (J > 1) BitonicMerge<E, K, J / 2>::run(r, lds, t);
+        sort_pass<T, E, K, J, false>(r, lds, t);
+        if constexpr (J > 1) SortHalves<T, E, K, J / 2>::run(r, lds, t);
     }
 };
 
-template <int E, int K>
-struct BitonicSort {
-    static __device__ __forceinline__ void run(uint32_t (&r)[E], uint32_t* lds, int t)
+template <typename T, int E, int K>
+struct SortMerges {
+    static __device__ __forceinline__ void run(T (&r)[E], T* lds, int t)
     {
-        BitonicMerge<E, K, K / 2>::run(r, lds, t);
-        if constexpr (K < 256 * E) BitonicSort<E, 2 * K>::run(r, lds, t);
+        sort_pass<T, E, K, K / 2, true>(r, lds, t);
+        if constexpr (K >= 4) SortHalves<T, E, K, K / 4>::run(r, lds, t);
+        if constexpr (K < 256 * E) SortMerges<T, E, 2 * K>::run(r, lds, t);
     }
 };
 
-// Ascending sort of the M = 256*E values held E per thread, the whole
-// network unrolled at compile time.  lds: M words, used for thread
-// distances >= 64.
-template <int E>
-__device__ __forceinline__ void sort_u32_blocked(uint32_t (&r)[E], uint32_t* lds, int t)
+// Ascending sort of the M = 256*E keys held E per thread, the whole network
+// unrolled at compile time.  lds: M keys, used for thread distances >= 64.
+template <typename T, int E>
+__device__ __forceinline__ void sort_blocked(T (&r)[E], T* lds, int t)
 {
-    BitonicSort<E, 2>::run(r, lds, t);
+    SortMerges<T, E, 2>::run(r, lds, t);
 }
 
 // Exclusive prefix sum over the block's 256 threads (4 waves); total out.
@@ -566,7 +574,7 @@ __global__ __launch_bounds__(256) void ksg_build_sorted_kernel(const double* __r
             for (int e = 0; e < E; ++e)
                 if (base + e < P) drow[base + e] = raw[e];
         }
-        sort_u32_blocked<E>(r, lds, t);
+        sort_blocked<uint32_t, E>(r, lds, t);
         // sorted row: 64 zeros, the N values, padding to the pitch
         uint32_t* srow = reinterpret_cast<uint32_t*>(SD) + row * ps;
         if (t < 64) srow[t] = 0u;
@@ -617,27 +625,14 @@ __global__ __launch_bounds__(256) void ksg_build_sorted_kernel(const double* __r
         }
         for (int q = nP + t; q < kPrefix; q += 256) sp[q] = ~0ull;
         __syncthreads();
-        // sort the kPrefix keys (one exchange per thread per pass); passes
-        // with j <= 64 stay within the wave's 128 keys
-        int pj = 0;
-        for (int k = 2; k <= kPrefix; k <<= 1)
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                if (j >= 128 || pj >= 128) {
-                    __syncthreads();
-                } else {
-                    lds_order();
-                }
-                const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
-                const uint64_t x = sp[lo], y = sp[hi];
-                if ((x > y) == ((lo & k) == 0)) {
-                    sp[lo] = y;
-                    sp[hi] = x;
-                }
-                pj = j;
-            }
-        __syncthreads();
-        for (int q = t; q < kPrefix; q += 256)
-            SI[row * kPrefix + q] = q < N ? (uint16_t)(sp[q] & 0xffffu) : (uint16_t)0;
+        // sort the kPrefix keys, two per thread, in registers
+        static_assert(kPrefix == 512, "two prefix keys per thread");
+        uint64_t pk[2] = {sp[2 * t], sp[2 * t + 1]};
+        sort_blocked<uint64_t, 2>(pk, sp, t);
+        uint16_t* si = SI + row * kPrefix;
+        const uint32_t f0 = 2 * t < N ? (uint32_t)(pk[0] & 0xffffu) : 0u;
+        const uint32_t f1 = 2 * t + 1 < N ? (uint32_t)(pk[1] & 0xffffu) : 0u;
+        reinterpret_cast<uint32_t*>(si)[t] = f0 | (f1 << 16);
         __syncthreads();   // sp, lds and s_thr are rewritten by the next row
     }
 }
