@@ -181,14 +181,17 @@ class Tuner:
         c = self.cache[key]
         return c["kind"], c["s4"]
 
-    def choose_chunk(self, rows, psi, k, kind, s4) -> int:
-        """Frame rows per kind-3 work item of the batch (0 for other kinds)."""
+    def choose_chunk(self, rows, psi, k, kind, s4, fraction: float = 1 / 32, reps: int = 1) -> int:
+        """Frame rows per kind-3 work item of the batch (0 for other kinds):
+        each candidate timed by the probe over `fraction` of the all-pairs
+        work items, best of `reps` after a warm-up (a whole trajectory's
+        plan is built for it, so the probe is kept short)."""
         if kind != 3:
             return 0
         key = f"{rows.shape[1]}:{k}:{kind}:{s4}:chunk"
         if key not in self.cache:
             shapes = [{"nsplit": 0, "chunk": c, "tile": 1} for c in SHAPE_CHUNKS]
-            t = time_shapes(rows, psi, k, kind, s4, shapes, fraction=0.125)
+            t = time_shapes(rows, psi, k, kind, s4, shapes, reps=reps, fraction=fraction)
             best = min(shapes, key=lambda sh: t[shape_key(sh)])
             self.cache[key] = {"chunk": best["chunk"], "ms": t}
             self._save()
