@@ -7,9 +7,11 @@ device code.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
+import tempfile
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -24,12 +26,11 @@ SOURCES = [
     "te_device.hip", "te_host.hip", "xtc_host.hip",
 ]
 
-HIPCC_FLAGS = [
+COMMON_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     "-I" + INCLUDE,
@@ -44,13 +45,27 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, *srcs, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compile every source to an object in parallel (each file launches only
+    its own kernels, so no relocatable device code is needed), then link the
+    shared library.  Objects live in a temporary directory."""
+    jobs = jobs or min(len(SOURCES), max(1, os.cpu_count() or 1), 16)
+    with tempfile.TemporaryDirectory(prefix="gmxcorr_build_") as tmpdir:
+        def compile_one(src: str) -> str:
+            obj = os.path.join(tmpdir, os.path.splitext(src)[0] + ".o")
+            cmd = [hipcc(), *COMMON_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+            return obj
+
+        with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+            objs = list(ex.map(compile_one, SOURCES))
+        tmp = LIB_PATH + ".tmp"
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include "dense_internal.h"
+#include "sort_network.h"
 
 namespace gmxcorr {
 
@@ -342,6 +343,7 @@ __global__ void dense_pair_mi_kernel(const float* __restrict__ C, int natoms, in
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void dense_copula_kernel(const double* __restrict__ V, int N,
                                                            long long ldv,
+                                                           const double* __restrict__ ztab,
                                                            double* __restrict__ Z,
                                                            long long ldz, int stage)
 {
@@ -362,12 +364,20 @@ __global__ __launch_bounds__(256) void dense_copula_kernel(const double* __restr
             eq += u == v;
         }
         double z = 0.0;
-        for (int t = lt; t < lt + eq; ++t) {
-            const double pr = (t + 0.5) / N;
-            z += sqrt(2.0) * erfinv_acklam(2.0 * pr - 1.0);
-        }
+        for (int t = lt; t < lt + eq; ++t) z += ztab[t];
         Z[c * ldz + f] = z / eq;
     }
+}
+
+// Normal score of rank t, the same for every component row:
+// ztab[t] = sqrt(2) erfinv(2 (t + 0.5) / N - 1), the term lib/gcmi.cpp:130-135
+// adds per rank (a product, then the sum: same rounding as the reference).
+__global__ void dense_score_table_kernel(int N, double* __restrict__ ztab)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const double pr = (t + 0.5) / N;
+    ztab[t] = sqrt(2.0) * erfinv_acklam(2.0 * pr - 1.0);
 }
 
 // Order-preserving 64-bit key of a finite double; v + 0.0 maps -0.0 to +0.0,
@@ -379,68 +389,60 @@ __device__ __forceinline__ uint64_t order_key(double v)
 }
 
 // ---------------------------------------------------------------------------
-// Copula scores by sorting (N <= kCopulaSortFrames): block per component row,
-// the row's keys bitonic-sorted in LDS (M = power of two >= N, padding keys
-// all-ones sort last), then per frame lt = #{g: v_g < v_f} and
-// eq = #{g: v_g == v_f} by binary search.  Same scores as
-// dense_copula_kernel, in O(N log N) per row.
+// Copula scores by sorting (N <= kCopulaSortFrames): block of 1024 threads per
+// component row.  The row's M = 1024 E >= N order keys (padding keys all-ones
+// sort last) are bitonic-sorted in registers, E per thread (sort_network.h:
+// thread distances of 64 and more through LDS), and stored to LDS in the
+// network's conflict-free layout (position p at sortnet::slot(p)); then per
+// frame lt = #{g: v_g < v_f} and eq = #{g: v_g == v_f} by binary
+// search.  Same scores as dense_copula_kernel, in O(N log^2 N) compare-
+// exchanges per row, most of them in registers.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void dense_copula_sort_kernel(const double* __restrict__ V,
-                                                                 int N, long long ldv,
-                                                                 double* __restrict__ Z,
-                                                                 long long ldz)
+constexpr int kCopulaThreads = 1024;
+
+template <int E>
+__global__ __launch_bounds__(kCopulaThreads) void dense_copula_sort_kernel(
+    const double* __restrict__ V, int N, long long ldv, const double* __restrict__ ztab,
+    double* __restrict__ Z, long long ldz)
 {
-    extern __shared__ uint64_t keys[];
+    constexpr int M = kCopulaThreads * E;
+    extern __shared__ uint64_t keys[];   // M keys
     const long long c = blockIdx.x;
     const double* row = V + c * ldv;
-    int M = 64;
-    while (M < N) M <<= 1;
-    for (int f = threadIdx.x; f < M; f += blockDim.x) keys[f] = f < N ? order_key(row[f]) : ~0ull;
+    const int t = threadIdx.x;
+    uint64_t r[E];
+    // the network sorts any arrangement: frames e * 1024 + t (coalesced loads)
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int f = e * kCopulaThreads + t;
+        r[e] = f < N ? order_key(row[f]) : ~0ull;
+    }
+    sortnet::sort_blocked<uint64_t, E, kCopulaThreads>(r, keys, t);
+    __syncthreads();   // the network's last LDS readers are done
+#pragma unroll
+    for (int e = 0; e < E; ++e) keys[e * kCopulaThreads + t] = r[e];   // position E t + e
     __syncthreads();
-    // Exchange t pairs keys lo = 2t - (t & (j - 1)) and lo + j; for j <= 64
-    // both lie among 128 keys owned by t's wave (blockDim a multiple of 64),
-    // so those passes need the wave's own in-order LDS accesses and no
-    // block barrier.
-    int pj = 0;
-    for (int k = 2; k <= M; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 128 || pj >= 128) {
-                __syncthreads();
-            } else {
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                __builtin_amdgcn_wave_barrier();
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            }
-            for (int t = threadIdx.x; t < M / 2; t += blockDim.x) {
-                const int lo = 2 * t - (t & (j - 1)), hi = lo + j;
-                const uint64_t x = keys[lo], y = keys[hi];
-                if ((x > y) == ((lo & k) == 0)) {
-                    keys[lo] = y;
-                    keys[hi] = x;
-                }
-            }
-            pj = j;
-        }
-    __syncthreads();
-    for (int f = threadIdx.x; f < N; f += blockDim.x) {
+    for (int f = t; f < N; f += kCopulaThreads) {
         const uint64_t key = order_key(row[f]);
         int lo = 0, hi = N;
         while (lo < hi) {   // first position with keys >= key
             const int mid = (lo + hi) >> 1;
-            if (keys[mid] < key) lo = mid + 1; else hi = mid;
+            if (keys[sortnet::slot<E, kCopulaThreads>(mid)] < key) lo = mid + 1; else hi = mid;
         }
         const int lt = lo;
-        hi = N;
-        while (lo < hi) {   // first position with keys > key
-            const int mid = (lo + hi) >> 1;
-            if (keys[mid] <= key) lo = mid + 1; else hi = mid;
+        // position lt holds the frame's own key; most values are unique, so
+        // look at the next position before searching for the first key above
+        lo = lt + 1;
+        if (lo < N && keys[sortnet::slot<E, kCopulaThreads>(lo)] <= key) {
+            hi = N;
+            while (lo < hi) {   // first position with keys > key
+                const int mid = (lo + hi) >> 1;
+                if (keys[sortnet::slot<E, kCopulaThreads>(mid)] <= key) lo = mid + 1; else hi = mid;
+            }
         }
         const int eq = lo - lt;
         double z = 0.0;
-        for (int t = lt; t < lt + eq; ++t) {
-            const double pr = (t + 0.5) / N;
-            z += sqrt(2.0) * erfinv_acklam(2.0 * pr - 1.0);
-        }
+        for (int t = lt; t < lt + eq; ++t) z += ztab[t];
         Z[c * ldz + f] = z / eq;
     }
 }
@@ -515,30 +517,41 @@ hipError_t launch_dense_pairs(const float* C, int natoms, int mode, double* mat,
 }
 
 hipError_t launch_dense_copula(const double* V, int M, int N, long long ldv, double* Z,
-                               long long ldz, hipStream_t s)
+                               long long ldz, double* ztab, hipStream_t s)
 {
+    hipLaunchKernelGGL(dense_score_table_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                       s, N, ztab);
     if (N > kCopulaMaxFrames) {
         // the row does not fit in LDS: the counting kernel reads it from
         // global memory (L2), O(N^2) per row
-        hipLaunchKernelGGL(dense_copula_kernel, dim3(M), dim3(256), 0, s, V, N, ldv, Z, ldz, 0);
+        hipLaunchKernelGGL(dense_copula_kernel, dim3(M), dim3(256), 0, s, V, N, ldv,
+                           (const double*)ztab, Z, ldz, 0);
         return hipGetLastError();
     }
     if (N <= kCopulaSortFrames) {
-        int Mp = 64;
-        while (Mp < N) Mp <<= 1;
-        const size_t kbytes = (size_t)Mp * sizeof(uint64_t);
-        hipError_t e = hipFuncSetAttribute((const void*)dense_copula_sort_kernel,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kbytes);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(dense_copula_sort_kernel, dim3(M), dim3(1024), kbytes, s, V, N, ldv, Z,
-                           ldz);
-        return hipGetLastError();
+        static_assert(kCopulaSortFrames == 16 * kCopulaThreads, "E = 16 covers the sort range");
+        auto run = [&](auto kernel, int E) -> hipError_t {
+            const size_t kbytes = (size_t)kCopulaThreads * E * sizeof(uint64_t);
+            hipError_t e = hipFuncSetAttribute((const void*)kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)kbytes);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(kernel, dim3(M), dim3(kCopulaThreads), kbytes, s, V, N, ldv,
+                               (const double*)ztab, Z, ldz);
+            return hipGetLastError();
+        };
+        if (N <= kCopulaThreads) return run(dense_copula_sort_kernel<1>, 1);
+        if (N <= 2 * kCopulaThreads) return run(dense_copula_sort_kernel<2>, 2);
+        if (N <= 4 * kCopulaThreads) return run(dense_copula_sort_kernel<4>, 4);
+        if (N <= 8 * kCopulaThreads) return run(dense_copula_sort_kernel<8>, 8);
+        return run(dense_copula_sort_kernel<16>, 16);
     }
     const size_t bytes = (size_t)N * sizeof(double);
     hipError_t e = hipFuncSetAttribute((const void*)dense_copula_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(dense_copula_kernel, dim3(M), dim3(256), bytes, s, V, N, ldv, Z, ldz, 1);
+    hipLaunchKernelGGL(dense_copula_kernel, dim3(M), dim3(256), bytes, s, V, N, ldv,
+                       (const double*)ztab, Z, ldz, 1);
     return hipGetLastError();
 }
 

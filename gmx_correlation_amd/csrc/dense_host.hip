@@ -27,6 +27,7 @@ struct DenseBuffers {
     double* traj = nullptr;
     double* rows = nullptr;
     double* z = nullptr;
+    double* ztab = nullptr;   // copula: normal score of each rank
     _Float16* hi = nullptr;
     _Float16* lo = nullptr;
     float* C = nullptr;
@@ -37,8 +38,8 @@ struct DenseBuffers {
     ~DenseBuffers()
     {
         if (s) (void)hipStreamSynchronize(s);
-        for (void* p : {(void*)traj, (void*)rows, (void*)z, (void*)hi, (void*)lo, (void*)C,
-                        (void*)Cp, (void*)mat})
+        for (void* p : {(void*)traj, (void*)rows, (void*)z, (void*)ztab, (void*)hi, (void*)lo,
+                        (void*)C, (void*)Cp, (void*)mat})
             if (p) (void)hipFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -83,7 +84,10 @@ int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* 
     for (auto& e : b.ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipMalloc(&b.traj, tbytes));
     HIPCHK(hipMalloc(&b.rows, tbytes));
-    if (mode == 1) HIPCHK(hipMalloc(&b.z, tbytes));
+    if (mode == 1) {
+        HIPCHK(hipMalloc(&b.z, tbytes));
+        HIPCHK(hipMalloc(&b.ztab, (size_t)N * sizeof(double)));
+    }
     HIPCHK(hipMalloc(&b.hi, (size_t)M * Kp * sizeof(_Float16)));
     HIPCHK(hipMalloc(&b.lo, (size_t)M * Kp * sizeof(_Float16)));
     HIPCHK(hipMalloc(&b.C, (size_t)M * M * sizeof(float)));
@@ -109,7 +113,7 @@ int dense_matrix(int mode, const double* traj, int natoms, int nframes, double* 
     HIPCHK(hipEventRecord(b.ev[1], b.s));
     const double* V = b.rows;
     if (mode == 1) {
-        HIPCHK(gmxcorr::launch_dense_copula(b.rows, M, N, N, b.z, N, b.s));
+        HIPCHK(gmxcorr::launch_dense_copula(b.rows, M, N, N, b.z, N, b.ztab, b.s));
         V = b.z;
     }
     HIPCHK(hipEventRecord(b.ev[2], b.s));

@@ -31,7 +31,9 @@ hipError_t launch_dense_cov(const _Float16* hi, const _Float16* lo, int M, int K
 // C = sum of nparts partial M x M matrices at Cp.
 hipError_t launch_dense_reduce(const float* Cp, int M, int nparts, float* C, hipStream_t s);
 hipError_t launch_dense_pairs(const float* C, int natoms, int mode, double* mat, hipStream_t s);
+// Copula scores Z of the M component rows of V; ztab: N doubles of
+// workspace (the normal score of each rank, shared by all rows).
 hipError_t launch_dense_copula(const double* V, int M, int N, long long ldv, double* Z,
-                               long long ldz, hipStream_t s);
+                               long long ldz, double* ztab, hipStream_t s);
 
 }  // namespace gmxcorr

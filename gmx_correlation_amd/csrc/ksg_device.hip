@@ -33,6 +33,7 @@
 #include <stdint.h>
 
 #include "ksg_internal.h"
+#include "sort_network.h"
 
 namespace gmxcorr {
 
@@ -405,109 +406,6 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
 // min/max work; only the prefix needs its frames.
 // ---------------------------------------------------------------------------
 
-// Exchange partner's value across lanes lane ^ m (m < 64).
-__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int lane, int m)
-{
-    return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ m) << 2, (int)v);
-}
-
-// Lane exchange for the sort: the value of lane lane ^ MT, by DPP where a
-// permute covers the distance (quad permutes, half-row and row mirrors),
-// else by ds_bpermute; 64-bit keys as two words.
-template <int MT>
-__device__ __forceinline__ uint32_t sort_xchg(uint32_t v, int lane)
-{
-    if constexpr (MT == 1) return (uint32_t)dpp<0xB1>((int)v);    // quad_perm [1,0,3,2]
-    else if constexpr (MT == 2) return (uint32_t)dpp<0x4E>((int)v);   // quad_perm [2,3,0,1]
-    else if constexpr (MT == 3) return (uint32_t)dpp<0x1B>((int)v);   // quad_perm [3,2,1,0]
-    else if constexpr (MT == 7) return (uint32_t)dpp<0x141>((int)v);  // row_half_mirror
-    else if constexpr (MT == 15) return (uint32_t)dpp<0x140>((int)v); // row_mirror
-    else return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ MT) << 2, (int)v);
-}
-
-template <int MT>
-__device__ __forceinline__ uint64_t sort_xchg(uint64_t v, int lane)
-{
-    const uint32_t lo = sort_xchg<MT>((uint32_t)v, lane);
-    const uint32_t hi = sort_xchg<MT>((uint32_t)(v >> 32), lane);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// One pass of the bitonic sort over the M = 256*E keys held E per thread
-// (thread t owns positions E*t .. E*t+E-1).  Merge step K starts with the
-// mirror pass (position x against x ^ (K - 1) in its K-block, Flip), then
-// passes J = K/4 .. 1 (x against x ^ J); the lower position always keeps
-// the smaller key, so every run is ascending and no pass has a direction
-// (in-thread pairs are a min and a max into fixed registers).
-//   partner in the thread's registers   bit H of the position below E;
-//   partner thread in the wave          sort_xchg;
-//   partner thread in another wave      through LDS (lds: M keys),
-//                                       between block barriers.
-template <typename T, int E, int K, int J, bool Flip>
-__device__ __forceinline__ void sort_pass(T (&r)[E], T* lds, int t)
-{
-    constexpr int H = Flip ? K / 2 : J;   // the position bit of the upper partner
-    if constexpr (H < E) {
-        constexpr int X = Flip ? K - 1 : J;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (e & H) continue;
-            const T a = r[e], b = r[e ^ X];
-            const bool lt = a < b;
-            r[e] = lt ? a : b;
-            r[e ^ X] = lt ? b : a;
-        }
-    } else {
-        // the partner thread, and its element: e ^ (E - 1) for the mirror
-        constexpr int MT = Flip ? K / E - 1 : J / E;
-        constexpr int HB = H / E;
-        const bool lower = (t & HB) == 0;
-        T o[E];
-        if constexpr (MT < kWave) {
-            const int lane = t & (kWave - 1);
-#pragma unroll
-            for (int e = 0; e < E; ++e) o[e] = sort_xchg<MT>(r[Flip ? E - 1 - e : e], lane);
-        } else {
-            __syncthreads();   // earlier readers of lds are done
-#pragma unroll
-            for (int e = 0; e < E; ++e) lds[E * t + e] = r[e];
-            __syncthreads();
-            const int pb = E * (t ^ MT);
-#pragma unroll
-            for (int e = 0; e < E; ++e) o[e] = lds[pb + (Flip ? E - 1 - e : e)];
-        }
-#pragma unroll
-        for (int e = 0; e < E; ++e) r[e] = lower == (o[e] < r[e]) ? o[e] : r[e];
-    }
-}
-
-template <typename T, int E, int K, int J>
-struct SortHalves {
-    static __device__ __forceinline__ void run(T (&r)[E], T* lds, int t)
-    {
-        sort_pass<T, E, K, J, false>(r, lds, t);
-        if constexpr (J > 1) SortHalves<T, E, K, J / 2>::run(r, lds, t);
-    }
-};
-
-template <typename T, int E, int K>
-struct SortMerges {
-    static __device__ __forceinline__ void run(T (&r)[E], T* lds, int t)
-    {
-        sort_pass<T, E, K, K / 2, true>(r, lds, t);
-        if constexpr (K >= 4) SortHalves<T, E, K, K / 4>::run(r, lds, t);
-        if constexpr (K < 256 * E) SortMerges<T, E, 2 * K>::run(r, lds, t);
-    }
-};
-
-// Ascending sort of the M = 256*E keys held E per thread, the whole network
-// unrolled at compile time.  lds: M keys, used for thread distances >= 64.
-template <typename T, int E>
-__device__ __forceinline__ void sort_blocked(T (&r)[E], T* lds, int t)
-{
-    SortMerges<T, E, 2>::run(r, lds, t);
-}
-
 // Exclusive prefix sum over the block's 256 threads (4 waves); total out.
 __device__ __forceinline__ int block_excl_scan(int v, int* wtot, int t, int& total)
 {
@@ -574,7 +472,7 @@ __global__ __launch_bounds__(256) void ksg_build_sorted_kernel(const double* __r
             for (int e = 0; e < E; ++e)
                 if (base + e < P) drow[base + e] = raw[e];
         }
-        sort_blocked<uint32_t, E>(r, lds, t);
+        sortnet::sort_blocked<uint32_t, E, 256>(r, lds, t);
         // sorted row: 64 zeros, the N values, padding to the pitch
         uint32_t* srow = reinterpret_cast<uint32_t*>(SD) + row * ps;
         if (t < 64) srow[t] = 0u;
@@ -628,7 +526,7 @@ __global__ __launch_bounds__(256) void ksg_build_sorted_kernel(const double* __r
         // sort the kPrefix keys, two per thread, in registers
         static_assert(kPrefix == 512, "two prefix keys per thread");
         uint64_t pk[2] = {sp[2 * t], sp[2 * t + 1]};
-        sort_blocked<uint64_t, 2>(pk, sp, t);
+        sortnet::sort_blocked<uint64_t, 2, 256>(pk, sp, t);
         uint16_t* si = SI + row * kPrefix;
         const uint32_t f0 = 2 * t < N ? (uint32_t)(pk[0] & 0xffffu) : 0u;
         const uint32_t f1 = 2 * t + 1 < N ? (uint32_t)(pk[1] & 0xffffu) : 0u;
@@ -647,7 +545,7 @@ __global__ __launch_bounds__(256) void ksg_build_sorted_kernel(const double* __r
 // first pass, and their LDS atomics would serialise.)  The frames below v*,
 // and the first frames at v* in frame order, are the prefix (the 1024
 // smallest in (value, frame) order).  They are bitonic-sorted by (value,
-// frame) in LDS, then
+// frame) in registers, then
 //   LV[a][i][0..64) = 0, LV[a][i][64 + t] = t-th smallest value, t < kLongPrefix
 //   LI[a][i][t]     = its frame
 //   LS[a][i][m]     = LV value 64m + 63
@@ -772,26 +670,15 @@ __global__ __launch_bounds__(512) void ksg_build_prefix_kernel(const float* __re
         }
     }
     __syncthreads();
-    // ---- bitonic sort of the kLongPrefix selected (value, frame) keys,
-    //      thread t exchanging keys lo = 2t - (t & (j - 1)) and lo + j.  For
-    //      j <= 64 both lie among the 128 keys of t's wave, so those passes
-    //      need the wave's own (in-order) LDS accesses, not a block barrier.
-    static_assert(kLongPrefix == 2 * 512, "one exchange per thread of the 512-thread block");
-    int pj = 0;
-    for (int k = 2; k <= kLongPrefix; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 2 * kWave || pj >= 2 * kWave)
-                __syncthreads();
-            else
-                lds_order();
-            const int lo = 2 * tid - (tid & (j - 1)), hi = lo + j;
-            const uint64_t x = sk[lo], y = sk[hi];
-            if ((x > y) == ((lo & k) == 0)) {
-                sk[lo] = y;
-                sk[hi] = x;
-            }
-            pj = j;
-        }
+    // ---- sort of the kLongPrefix selected (value, frame) keys (distinct:
+    //      one per frame), two per thread, in registers (sort_network.h),
+    //      written back in order
+    static_assert(kLongPrefix == 2 * 512, "two keys per thread of the 512-thread block");
+    uint64_t pk[2] = {sk[2 * tid], sk[2 * tid + 1]};
+    sortnet::sort_blocked<uint64_t, 2, 512>(pk, sk, tid);
+    __syncthreads();   // the network's last LDS readers are done
+    sk[2 * tid] = pk[0];
+    sk[2 * tid + 1] = pk[1];
     __syncthreads();
     uint32_t* lv = reinterpret_cast<uint32_t*>(LV) + row * kLongPitch;
     for (int t = tid; t < kLongPitch; t += blockDim.x)
@@ -1085,24 +972,31 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
 #pragma unroll
     for (int m = 0; m < 8; ++m) nq[m] = nv0;
     uint32_t nsm = 0u, nL = 0u;
+    // Each load takes the atom's base pointer (selected by the side, held in
+    // scalar registers) and a 32-bit byte offset holding the row: one vector
+    // add per load instead of 64-bit scalar pointer arithmetic per row (the
+    // offsets stay below 2^32: rows < 2048, pitches < 2300 words).
     auto issue = [&](int t) {
-        const int row = i0 + t * step;   // < N: 32-bit offsets below
+        const uint32_t row = (uint32_t)(i0 + t * step);   // < N
         const bool sa = side_of(t);
-        const uint32_t* base = (sa ? pr.SDa : pr.SDb) + (uint32_t)row * (uint32_t)pr.PS + 64;
-        nv0 = ld_lane<uint4>(base, lane4 * 4u);
-        nv1 = ld_lane<uint4>(base, lane4 * 4u + 1024u);
-        const uint16_t* ib = (sa ? pr.SIa : pr.SIb) + (uint32_t)row * (uint32_t)kPrefix;
-        const uint2 i01 = ld_lane<uint2>(ib, lane4 * 2u);
-        const uint2 i23 = ld_lane<uint2>(ib, lane4 * 2u + 512u);
+        const uint32_t* sd = sa ? pr.SDa : pr.SDb;
+        const uint32_t so = (row * (uint32_t)pr.PS + 64u) * 4u;
+        nv0 = ld_lane<uint4>(sd, so + lane4 * 4u);
+        nv1 = ld_lane<uint4>(sd, so + lane4 * 4u + 1024u);
+        const uint16_t* si = sa ? pr.SIa : pr.SIb;
+        const uint32_t io = row * (uint32_t)kPrefix * 2u;
+        const uint2 i01 = ld_lane<uint2>(si, io + lane4 * 2u);
+        const uint2 i23 = ld_lane<uint2>(si, io + lane4 * 2u + 512u);
         ni = make_uint4(i01.x, i01.y, i23.x, i23.y);
-        if (lane == 0) nL = ld_lane<uint32_t>(base, (uint32_t)(T - 1) * 4u);
-        nsm = ld_lane<uint32_t>((sa ? pr.SMb : pr.SMa) + (uint32_t)row * (uint32_t)kSumPitch, lane4);
-        const uint32_t* rq = (sa ? pr.Db : pr.Da) + (uint32_t)row * (uint32_t)pr.P;
+        if (lane == 0) nL = ld_lane<uint32_t>(sd, so + (uint32_t)(T - 1) * 4u);
+        nsm = ld_lane<uint32_t>(sa ? pr.SMb : pr.SMa, row * (uint32_t)kSumPitch * 4u + lane4);
+        const uint32_t* rq = sa ? pr.Db : pr.Da;
+        const uint32_t qo = row * (uint32_t)pr.P * 4u + lane4 * 4u;
         // all 8 chunks: past the row's last chunk the last one again (a
         // row of P < 2048 floats), staged to LDS slots no gather reads
 #pragma unroll
         for (int m = 0; m < 8; ++m)
-            nq[m] = ld_lane<uint4>(rq, lane4 * 4u + (uint32_t)min(m, nblk - 1) * 1024u);
+            nq[m] = ld_lane<uint4>(rq, qo + (uint32_t)min(m, nblk - 1) * 1024u);
     };
     double acc = 0.0, px = 0.0, py = 0.0;
     // the previous row, waiting for its Q block
@@ -1278,12 +1172,11 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         pnp = c;
         // ex is the sorted prefix's value at position c - 1; a tie at it
         // shows as an equal value at position c - 2 (compared one row later)
-        ptv = ld_lane<uint32_t>((sideA ? pr.SDa : pr.SDb) + (uint32_t)row * (uint32_t)pr.PS + 64,
-                                (uint32_t)max(c - 2, 0) * 4u);
+        const uint32_t ro = (uint32_t)row * (uint32_t)pr.PS * 4u;   // byte offset of the row
+        ptv = ld_lane<uint32_t>(sideA ? pr.SDa : pr.SDb, ro + 256u + (uint32_t)max(c - 2, 0) * 4u);
         pexp = ex;
         pc1 = __popcll(__ballot(sm <= ey));
-        const uint32_t* sq = (sideA ? pr.SDb : pr.SDa) + (uint32_t)row * (uint32_t)pr.PS + 64 * pc1;
-        pblk = ld_lane<uint2>(sq, lane4 * 2u);
+        pblk = ld_lane<uint2>(sideA ? pr.SDb : pr.SDa, ro + (uint32_t)pc1 * 256u + lane4 * 2u);
         prow = row;
         pside = sideA;
         pexq = ey;

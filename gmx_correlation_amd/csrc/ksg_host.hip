@@ -764,6 +764,35 @@ int gmxcorr_ksg_plan_load(gmxcorr_ksg_plan* pl, const double* xrows, const doubl
     return GMXCORR_OK;
 }
 
+// Pair buffers sized for the current path and shape, the all-pairs list
+// (a1 > a2, row by row) and its 2x2 atom tiles on the device, once per
+// plan until another pair list replaces them.
+static int setup_all_pairs(gmxcorr_ksg_plan* pl)
+{
+    const long long n = pl->natoms;
+    const long long npairs = n * (n - 1) / 2;
+    int kind, s4;
+    plan_kind(pl, kind, s4);
+    if (int rc = ensure_pairs(pl, npairs, gmxcorr::variant_slots(kind, plan_split(pl, kind, npairs))))
+        return rc;
+    if (!pl->all_pairs) {
+        std::vector<int> h1(npairs), h2(npairs);
+        long long idx = 0;
+        for (int a1 = 1; a1 < n; ++a1)
+            for (int a2 = 0; a2 < a1; ++a2, ++idx) { h1[idx] = a1; h2[idx] = a2; }
+        HIPCHK(hipMemcpy(pl->dA1, h1.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(pl->dA2, h2.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
+        if (!pl->dQuads && npairs < (1LL << 31)) {
+            const std::vector<int> q = gmxcorr::pair_quads((int)n);
+            HIPCHK(hipMalloc(&pl->dQuads, q.size() * sizeof(int)));
+            HIPCHK(hipMemcpy(pl->dQuads, q.data(), q.size() * sizeof(int), hipMemcpyHostToDevice));
+            pl->nquads = (long long)q.size() / 4;
+        }
+        pl->all_pairs = true;
+    }
+    return GMXCORR_OK;
+}
+
 int gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* pl)
 {
     if (!pl || !pl->loaded) return fail(GMXCORR_EINVAL, "plan not loaded");
@@ -785,25 +814,7 @@ int gmxcorr_ksg_plan_run_all(gmxcorr_ksg_plan* pl)
         pl->st.all_pairs = 1;
         return GMXCORR_OK;
     }
-    int kind, s4;
-    plan_kind(pl, kind, s4);
-    if (int rc = ensure_pairs(pl, npairs, gmxcorr::variant_slots(kind, plan_split(pl, kind, npairs))))
-        return rc;
-    if (!pl->all_pairs) {
-        std::vector<int> h1(npairs), h2(npairs);
-        long long idx = 0;
-        for (int a1 = 1; a1 < n; ++a1)
-            for (int a2 = 0; a2 < a1; ++a2, ++idx) { h1[idx] = a1; h2[idx] = a2; }
-        HIPCHK(hipMemcpy(pl->dA1, h1.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(pl->dA2, h2.data(), npairs * sizeof(int), hipMemcpyHostToDevice));
-        if (!pl->dQuads && npairs < (1LL << 31)) {
-            const std::vector<int> q = gmxcorr::pair_quads((int)n);
-            HIPCHK(hipMalloc(&pl->dQuads, q.size() * sizeof(int)));
-            HIPCHK(hipMemcpy(pl->dQuads, q.data(), q.size() * sizeof(int), hipMemcpyHostToDevice));
-            pl->nquads = (long long)q.size() / 4;
-        }
-        pl->all_pairs = true;
-    }
+    if (int rc = setup_all_pairs(pl)) return rc;
     pl->host_result = false;
     const int rc = run_loaded(pl, npairs, false, pl->tiles && pl->dQuads != nullptr);
     if (!rc) {
@@ -818,8 +829,8 @@ int gmxcorr_ksg_plan_probe(gmxcorr_ksg_plan* pl, double fraction, double* ms)
     if (!pl || !pl->loaded || !ms) return fail(GMXCORR_EINVAL, "plan_probe: plan not loaded");
     if (!(fraction > 0.0 && fraction <= 1.0)) return fail(GMXCORR_EINVAL, "plan_probe: fraction in (0, 1]");
     if (pl->wave_atoms > 0) return fail(GMXCORR_EUNSUP, "plan_probe: plan runs in memory waves");
-    // the all-pairs list and tiles, as run_all sets them up
-    if (int rc = gmxcorr_ksg_plan_run_all(pl)) return rc;
+    // the all-pairs list and tiles, as run_all sets them up (without a run)
+    if (int rc = setup_all_pairs(pl)) return rc;
     const long long npairs = (long long)pl->natoms * (pl->natoms - 1) / 2;
     int kind, s4;
     plan_kind(pl, kind, s4);

@@ -191,9 +191,15 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
     double cn0[9], cn1[9];
     load_frame(X, Y, 0, lag, cn0);
     load_frame(X, Y, min(1, Nv - 1), lag, cn1);
+    // The buffer is a shift register, newest key in slot 0: a lane that
+    // takes a key runs the shift under its own branch as plain moves (an
+    // indexed write would be a compare and a select per slot).  At most kW
+    // keys wait between flushes, so the shift drops none, and the flush
+    // inserts them in any order with the same heap contents.
     auto take = [&](float key) {
 #pragma unroll
-        for (int u = 0; u < kW; ++u) wk[u] = nw == u ? key : wk[u];
+        for (int u = kW - 1; u > 0; --u) wk[u] = wk[u - 1];
+        wk[0] = key;
         ++nw;
     };
     for (int j = 0; j < Nv; j += 2) {
@@ -326,7 +332,6 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
     // loop.  fl32 is monotone, so the keys are maxima of the converted 3-D
     // distances.
     int clt = 0, gcnt = 0;
-    int neq = 0;   // subspace keys equal to the root so far (frames to record)
     int nxfxp = 0, nypxp = 0, nxp = 0;
     double gmin = INFINITY, gmax = -INFINITY;
     double g1 = 0.0, g2 = 0.0, g3 = 0.0;   // subspace distances of a root-key frame
@@ -350,10 +355,10 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
         nxfxp += k1 < root;
         nypxp += k2 < root;
         nxp += fxp < root;
-        const int neq0 = neq;
-        neq += (k1 == root) + (k2 == root) + (fxp == root);
-        if (__ballot(neq != neq0)) {   // uniform, rare
-            if (neq != neq0 && key != root) {
+        // a subspace key equal to the root in a frame of another key
+        const bool amb = ((k1 == root) | (k2 == root) | (fxp == root)) & (key != root);
+        if (__ballot(amb)) {   // uniform, rare
+            if (amb) {
 #pragma unroll
                 for (int u = 0; u < kAmb; ++u) aj[u] = na == u ? j : aj[u];
                 ++na;
