@@ -224,7 +224,7 @@ class KsgBatch:
         return rc
 
     def set_chunk(self, t: int, chunk: int) -> None:
-        """Frame rows per kind-3 work item of trajectory t (0 = default 16)."""
+        """Frame rows per kind-3 work item of trajectory t (0 = default 4)."""
         _lib.check(_lib.load().gmxcorr_ksg_batch_set_chunk(self._h, t, chunk))
 
     def traj_bytes(self, t: int) -> int:

@@ -2176,19 +2176,92 @@ __device__ Fp64Row fp64_row(const double* __restrict__ xa, const double* __restr
         r = ek ? rank : rank - 1;
         whole = pop == rank;
     } else {
-        int clt = 0;
-        for (int j = lane; j < N; j += kWave) {
-            if (j == i) continue;
-            clt += key_of(j) < ek;
-        }
-        clt = wave_sum(clt);
-        r = K - clt;  // >= 1
+        r = 0;   // set below
     }
 
     double eps64 = 0.0;
     bool tie = false;
-    if (!whole) {
+    double ex = 0.0, ey = 0.0;
+    bool radii = false;   // eps64, tie and the radii found by the one-pass form
+    if (ek != kUnknownEps) {
+        // The key is known: one pass counts the keys below it (frames certainly
+        // in the set: rounding is monotone, so their FP64 distance is below
+        // every FP64 value of key ek), takes their marginal maxima, and lists
+        // the frames whose key equals ek (the group G, usually a few) with
+        // their FP64 distances in LDS.  The r-th smallest FP64 value of G is
+        // eps; its ties and G's members up to eps follow from the list.
+        double* gJ = reinterpret_cast<double*>(hist);   // kWave entries each
+        double* gA = gJ + kWave;
+        double* gB = gA + kWave;
+        int clt = 0, g = 0;
+        double sx = 0.0, sy = 0.0;
+        for (int j0 = 0; j0 < N; j0 += kWave) {   // uniform trip count
+            const int j = j0 + lane;
+            double A = 0.0, B = 0.0, J = 0.0;
+            bool eq = false;
+            if (j < N && j != i) {
+                dist(j, A, B);
+                J = fmax(A, B);
+                const uint32_t key = __float_as_uint((float)J);
+                if (key < ek) {
+                    ++clt;
+                    sx = fmax(sx, A);
+                    sy = fmax(sy, B);
+                }
+                eq = key == ek;
+            }
+            const uint64_t m = __ballot(eq);
+            if (m) {   // uniform
+                const int slot = g + mbcnt(m);
+                if (eq && slot < kWave) {
+                    gJ[slot] = J;
+                    gA[slot] = A;
+                    gB[slot] = B;
+                }
+                g += __popcll(m);
+            }
+        }
+        clt = wave_sum(clt);
+        r = K - clt;   // >= 1 for a consistent entry
+        if (g <= kWave && r >= 1 && r <= g) {
+            wave_sync();   // the list is read by every lane
+            double Jl = INFINITY;
+            int cl = 0, el = 0;   // entries below and equal to this lane's
+            if (lane < g) {
+                Jl = gJ[lane];
+                for (int u = 0; u < g; ++u) {
+                    const double v = gJ[u];
+                    cl += v < Jl;
+                    el += v == Jl;
+                }
+            }
+            // an entry with cl < r <= cl + el exists for 1 <= r <= g
+            const uint64_t pick = __ballot(lane < g && cl < r && r <= cl + el);
+            const int src = __ffsll((unsigned long long)pick) - 1;
+            eps64 = __shfl(Jl, src, kWave);
+            tie = __shfl(cl + el, src, kWave) > r;
+            double mx = sx, my = sy;
+            if (lane < g && Jl <= eps64) {
+                mx = fmax(mx, gA[lane]);
+                my = fmax(my, gB[lane]);
+            }
+            ex = wave_max_d(mx);
+            ey = wave_max_d(my);
+            radii = true;
+            wave_sync();   // the list is rewritten by the wave's next row
+        }
+    }
+    if (!radii && !whole) {
         // walk the distinct FP64 values of G upwards until r are covered
+        // (the group did not fit the list, or the key was not given)
+        if (ek != kUnknownEps) {
+            int clt = 0;
+            for (int j = lane; j < N; j += kWave) {
+                if (j == i) continue;
+                clt += key_of(j) < ek;
+            }
+            r = K - wave_sum(clt);
+        }
         double prev = -1.0;
         int cum = 0;
         for (;;) {
@@ -2219,21 +2292,23 @@ __device__ Fp64Row fp64_row(const double* __restrict__ xa, const double* __restr
         }
     }
 
-    // radii: the largest marginal distances over the set.  With `whole` the
-    // set is {key <= ek}, which equals {J <= max_G J} as rounding is monotone.
-    double ex = 0.0, ey = 0.0;
-    for (int j = lane; j < N; j += kWave) {
-        if (j == i) continue;
-        double A, B;
-        dist(j, A, B);
-        const bool in = whole ? __float_as_uint((float)fmax(A, B)) <= ek : fmax(A, B) <= eps64;
-        if (in) {
-            ex = fmax(ex, A);
-            ey = fmax(ey, B);
+    if (!radii) {
+        // radii: the largest marginal distances over the set.  With `whole`
+        // the set is {key <= ek}, which equals {J <= max_G J} as rounding is
+        // monotone.
+        for (int j = lane; j < N; j += kWave) {
+            if (j == i) continue;
+            double A, B;
+            dist(j, A, B);
+            const bool in = whole ? __float_as_uint((float)fmax(A, B)) <= ek : fmax(A, B) <= eps64;
+            if (in) {
+                ex = fmax(ex, A);
+                ey = fmax(ey, B);
+            }
         }
+        ex = wave_max_d(ex);
+        ey = wave_max_d(ey);
     }
-    ex = wave_max_d(ex);
-    ey = wave_max_d(ey);
     int nx = 0, ny = 0;
     for (int j = lane; j < N; j += kWave) {
         if (j == i) continue;

@@ -185,8 +185,10 @@ constexpr int kLongPrefix = 1024;
 // TrajDesc::chunk; this is the default, the autotuner may choose another).
 // Short chunks keep the blocks in flight on one XCD on the same rows (as the
 // row ranges do for the sorted-prefix kernel): 16 rows ran 4-11% faster
-// than 32 on 400 atoms x 5000 frames (k = 100) and x 8000 frames (k = 10).
-constexpr int kLongChunk = 16;
+// than 32 on 400 atoms x 5000 frames (k = 100) and x 8000 frames (k = 10),
+// and the batch autotuner, timing 4 to 64 rows, picks 4 on each of the 29
+// trajectories of 94-587 atoms x 2244-7783 frames it tunes.
+constexpr int kLongChunk = 4;
 constexpr int kLongPitch = 64 + kLongPrefix;
 constexpr int kLongSum = kLongPrefix / 64;
 

@@ -147,7 +147,7 @@ class KsgPlan:
     def set_shape(self, nsplit: int = 0, chunk: int = 0, tile: int = 0) -> None:
         """Row-block and tile shape: nsplit row ranges per pair (kinds 0, 2,
         4; 0 = default), chunk frame rows per block of kind 3 (0 = default
-        16), tile 0/1 = 2x2 atom tiles in the all-pairs run, 2 = runs of four
+        4), tile 0/1 = 2x2 atom tiles in the all-pairs run, 2 = runs of four
         pairs sharing their first atom."""
         _lib.check(_lib.load().gmxcorr_ksg_plan_set_shape(self._h, nsplit, chunk, tile))
 

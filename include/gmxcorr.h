@@ -122,7 +122,7 @@ int  gmxcorr_ksg_plan_set_variant(gmxcorr_ksg_plan* plan, int kind, int s4);
  * per pair for kinds 0, 2 and 4 (1..256; 0 = default: 128 for kinds 0 and 2,
  * fewer when the partial sums would pass 8 GB, and enough to fill the device
  * for kind 4), chunk frame rows per block of kind 3 (1..1024; 0 = default
- * 16), tile: the four pairs of a block in the all-pairs run (0 or 1: a 2x2
+ * 4), tile: the four pairs of a block in the all-pairs run (0 or 1: a 2x2
  * atom tile, 2: four consecutive pairs sharing their first atom). */
 int  gmxcorr_ksg_plan_set_shape(gmxcorr_ksg_plan* plan, int nsplit, int chunk, int tile);
 int  gmxcorr_ksg_variant_fits(int kind, int s4, int nframes, int k);
@@ -241,7 +241,7 @@ int  gmxcorr_ksg_batch_add(gmxcorr_ksg_batch* b, const double* xrows, const doub
 long long gmxcorr_ksg_batch_traj_bytes(const gmxcorr_ksg_batch* b, int t);
 int  gmxcorr_ksg_batch_run(gmxcorr_ksg_batch* b, long long max_row_bytes, int nblocks);
 /* Frame rows per work item of trajectory t when it runs on kind 3 (1..1024;
- * 0 = default 16); the autotuner's shape knob for the batch. */
+ * 0 = default 4); the autotuner's shape knob for the batch. */
 int  gmxcorr_ksg_batch_set_chunk(gmxcorr_ksg_batch* b, int t, int chunk);
 /* MI of trajectory t's pairs (a1 = 1.., a2 < a1); ms_done: batch-relative
  * completion time of its wave; wave: its wave index (both optional). */
