@@ -140,8 +140,15 @@ __global__ void dense_split_kernel(const double* __restrict__ V, int N, long lon
 // ---------------------------------------------------------------------------
 constexpr int kGTile = 128;            // block tile (components)
 constexpr int kGStep = 32;             // frames per staged step
-constexpr int kGPitch = kGStep + 8;    // LDS row pitch in halves (80 B: no bank conflicts)
+constexpr int kGPitch = kGStep;        // LDS row pitch in halves (64 B, four 16-byte units)
 constexpr int kGPanel = kGTile * kGPitch;
+
+// LDS swizzle of the 16-byte unit q (frames 8q..8q+7 of a step) in panel row
+// `row`: unit q ^ gsw(row).  The staging writes (16 lanes: four rows x four
+// units) and the operand reads (16 lanes: 16 consecutive rows, one unit) then
+// fall on distinct banks for 32 or 64 banks of 4 bytes; rows 32 apart share
+// the swizzle.
+__device__ __forceinline__ int gsw(int row) { return ((row >> 1) ^ (row >> 2)) & 3; }
 
 __global__ __launch_bounds__(256, 2) void dense_cov_tile_kernel(const _Float16* __restrict__ hi,
                                                              const _Float16* __restrict__ lo,
@@ -170,7 +177,7 @@ __global__ __launch_bounds__(256, 2) void dense_cov_tile_kernel(const _Float16* 
         const int ra = min(ti * kGTile + c / 4, M - 1), rb = min(tj * kGTile + c / 4, M - 1);
         aoff[h] = (uint32_t)ra * (uint32_t)Kp + 8u * (c % 4);
         boff[h] = (uint32_t)rb * (uint32_t)Kp + 8u * (c % 4);
-        soff[h] = (c / 4) * kGPitch + 8 * (c % 4);
+        soff[h] = (c / 4) * kGPitch + 8 * ((c % 4) ^ gsw(c / 4));
     }
     const int wr = (w >> 1) * 64, wc = (w & 1) * 64;   // wave sub-tile in the block tile
     const int r = lane & 31, kq = (lane >> 5) * 8;
@@ -205,8 +212,9 @@ __global__ __launch_bounds__(256, 2) void dense_cov_tile_kernel(const _Float16* 
         const _Float16* pa = sm[buf][0];                                                 \
         _Pragma("unroll") for (int k16 = 0; k16 < kGStep; k16 += 16)                     \
         {                                                                                \
-            const int ao = (wr + r) * kGPitch + k16 + kq;                                \
-            const int bo = (wc + r) * kGPitch + k16 + kq;                                \
+            const int qu = (k16 + kq) >> 3;                                              \
+            const int ao = (wr + r) * kGPitch + 8 * (qu ^ gsw(wr + r));                  \
+            const int bo = (wc + r) * kGPitch + 8 * (qu ^ gsw(wc + r));                  \
             const half8 ah0 = *reinterpret_cast<const half8*>(pa + ao);                  \
             const half8 ah1 = *reinterpret_cast<const half8*>(pa + ao + 32 * kGPitch);   \
             const half8 al0 = *reinterpret_cast<const half8*>(pa + kGPanel + ao);        \
