@@ -457,7 +457,7 @@ def main() -> int:
     ap.add_argument("--te-k", type=int, default=100, help="k of the sampled TE workload (te)")
     ap.add_argument("--te-path", type=int, default=0, choices=(0, 1, 2),
                     help="TE pass-1 kernel (te): 0 automatic, 1 heap, 2 radix select")
-    ap.add_argument("--row-budget-gb", type=float, default=200.0,
+    ap.add_argument("--row-budget-gb", type=float, default=250.0,
                     help="distance-row bytes per memory wave (batched)")
     args = ap.parse_args()
     if args.gpus != 1:
