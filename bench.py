@@ -94,9 +94,9 @@ def cpu_baseline(coords: np.ndarray, k: int, natoms: int, budget_s: float) -> di
         "cpu_model": _cpu_model(),
         "restatement": "oracle/ksg_oracle.c oracle_ksg_pair_baseline: the reference's grid "
                        "algorithm with its row copies, cell-ordered copies, linked lists and "
-                       "per-query scratch; same MI bit for bit; measured 15-25% faster than the "
-                       "reference's own compiled code in the development container (8 threads), "
-                       "so this baseline flatters the CPU",
+                       "per-query scratch and out-of-line neighbour routines; same MI bit for "
+                       "bit; 0.95 of the reference's own compiled time at 64x500 k=10 and "
+                       "247x2000 k=100 in the development container (8 threads)",
         "_values": vals,
         "_idx": idx,
     }
