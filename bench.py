@@ -64,6 +64,7 @@ def cpu_baseline(coords: np.ndarray, k: int, natoms: int, budget_s: float) -> di
     runtime offers (nthreads = 0 as -nt 0 does: omp_get_max_threads)."""
     from gmx_correlation_amd import ksg
     from oracle import oracle as O
+    O.build(with_ref=False)   # compiled here, on this host, from oracle/*.c
     rows, psi = ksg.kraskov_prepare(ksg.center(coords))
     nthreads = O.max_threads()
     a1, a2 = np.tril_indices(natoms, -1)
