@@ -95,9 +95,11 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
     // needs no LDS and the LDS part is kR slots shorter.
     const int kh = k + 1;
     float r[kR];
+    auto reset_heap = [&]() {
 #pragma unroll
-    for (int t = 0; t < kR; ++t) r[t] = t < kh ? INFINITY : -INFINITY;
-    for (int t = kR; t < kh; ++t) heap(t) = INFINITY;
+        for (int t = 0; t < kR; ++t) r[t] = t < kh ? INFINITY : -INFINITY;
+        for (int t = kR; t < kh; ++t) heap(t) = INFINITY;
+    };
     // largest of kD values: a tree of pairwise maxima, index in ix
     auto argmax = [&](float* ch, int* ix) {
 #pragma unroll
@@ -166,31 +168,53 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         heap(pos) = key;
         return r[0];
     };
+    // ---- a cap on the root from a strided sample of the frames ----------
+    // The kTeSampleRank-th smallest key of every stride-th frame.  Keys at
+    // or above it are not offered to the heap, which cuts the inserts (and
+    // the wave-wide flushes) to less than half.  The heap still ends with
+    // the k + 1 smallest keys whenever k + 1 keys lie below the cap (then
+    // all of the k + 1 smallest do); a lane where fewer do ends with an
+    // unfilled slot, an infinite root, and the wave runs the frame loop
+    // again without the cap: a failed lane costs time, not exactness.
+    const int stride = max(kTeSampleStride, kTeSampleStride * kh / 101);
+    float smp[kTeSampleRank];
+#pragma unroll
+    for (int u = 0; u < kTeSampleRank; ++u) smp[u] = INFINITY;
+    {
+        double cs[9];
+        load_frame(X, Y, 0, lag, cs);
+        for (int j = 0; j < Nv; j += stride) {
+            double c[9];
+#pragma unroll
+            for (int d = 0; d < 9; ++d) c[d] = cs[d];
+            load_frame(X, Y, min(j + stride, Nv - 1), lag, cs);
+            double a0, b0, e0;
+            dist9(c, a0, b0, e0);
+            const float key = (float)fmax(fmax(a0, b0), e0);
+            // insert into the ascending list, the largest dropped:
+            // smp[u] = min(smp[u], max(smp[u - 1], key))
+#pragma unroll
+            for (int u = kTeSampleRank - 1; u > 0; --u) smp[u] = fminf(smp[u], fmaxf(smp[u - 1], key));
+            smp[0] = fminf(smp[0], key);
+        }
+    }
     float top = INFINITY;
+    float cap = smp[kTeSampleRank - 1];
     // Keys below the root wait in kW registers per lane; when some lane is
     // about to run out of them, every lane inserts its waiting keys together
     // (each checked against the root again), so the wave runs the sift with
     // most lanes active instead of for one lane at a time.
     constexpr int kW = 8;
     float wk[kW];
-#pragma unroll
-    for (int u = 0; u < kW; ++u) wk[u] = INFINITY;
     int nw = 0;
     auto flush = [&]() {
 #pragma unroll
         for (int u = 0; u < kW; ++u) {
-            if (wk[u] < top) top = insert(wk[u]);
+            if (wk[u] < top) top = fminf(insert(wk[u]), cap);
             wk[u] = INFINITY;
         }
         nw = 0;
     };
-    // two frames per iteration (two independent FP64 chains); frames j + 2
-    // and j + 3 are loaded while j and j + 1 are processed.  A lane takes
-    // at most two keys per iteration, so the buffer is flushed once some
-    // lane holds kW - 1.
-    double cn0[9], cn1[9];
-    load_frame(X, Y, 0, lag, cn0);
-    load_frame(X, Y, min(1, Nv - 1), lag, cn1);
     // The buffer is a shift register, newest key in slot 0: a lane that
     // takes a key runs the shift under its own branch as plain moves (an
     // indexed write would be a compare and a select per slot).  At most kW
@@ -202,26 +226,46 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         wk[0] = key;
         ++nw;
     };
-    for (int j = 0; j < Nv; j += 2) {
-        double c0[9], c1[9];
+    for (int attempt = 0;; ++attempt) {
+        reset_heap();
+        top = cap;
 #pragma unroll
-        for (int d = 0; d < 9; ++d) {
-            c0[d] = cn0[d];
-            c1[d] = cn1[d];
+        for (int u = 0; u < kW; ++u) wk[u] = INFINITY;
+        nw = 0;
+        // two frames per iteration (two independent FP64 chains); frames
+        // j + 2 and j + 3 are loaded while j and j + 1 are processed.  A
+        // lane takes at most two keys per iteration, so the buffer is
+        // flushed once some lane holds kW - 1.
+        double cn0[9], cn1[9];
+        load_frame(X, Y, 0, lag, cn0);
+        load_frame(X, Y, min(1, Nv - 1), lag, cn1);
+        for (int j = 0; j < Nv; j += 2) {
+            double c0[9], c1[9];
+#pragma unroll
+            for (int d = 0; d < 9; ++d) {
+                c0[d] = cn0[d];
+                c1[d] = cn1[d];
+            }
+            load_frame(X, Y, min(j + 2, Nv - 1), lag, cn0);
+            load_frame(X, Y, min(j + 3, Nv - 1), lag, cn1);
+            double a0, b0, e0, a1, b1, e1;
+            dist9(c0, a0, b0, e0);
+            dist9(c1, a1, b1, e1);
+            const float key0 = (float)fmax(fmax(a0, b0), e0);
+            const float key1 = (float)fmax(fmax(a1, b1), e1);
+            if (key0 < top) take(key0);
+            if (j + 1 < Nv && key1 < top) take(key1);
+            if (__ballot(nw >= kW - 1)) flush();   // uniform over the wave
         }
-        load_frame(X, Y, min(j + 2, Nv - 1), lag, cn0);
-        load_frame(X, Y, min(j + 3, Nv - 1), lag, cn1);
-        double a0, b0, e0, a1, b1, e1;
-        dist9(c0, a0, b0, e0);
-        dist9(c1, a1, b1, e1);
-        const float key0 = (float)fmax(fmax(a0, b0), e0);
-        const float key1 = (float)fmax(fmax(a1, b1), e1);
-        if (key0 < top) take(key0);
-        if (j + 1 < Nv && key1 < top) take(key1);
-        if (__ballot(nw >= kW - 1)) flush();   // uniform over the wave
+        flush();
+        // an unfilled slot leaves the root infinite: fewer than k + 1 keys
+        // below a finite cap (the plain run, cap = inf, always fills it:
+        // Nv >= k + 1 finite keys)
+        const bool short_ = active && r[0] == INFINITY && cap < INFINITY;
+        if (!__ballot(short_)) break;   // uniform over the wave
+        cap = INFINITY;
     }
-    flush();
-    args.root[(size_t)blockIdx.x * 64 + lane] = top;
+    args.root[(size_t)blockIdx.x * 64 + lane] = r[0];
 }
 
 // Pass 1 for any k (the heap kernel's LDS stops at k = 128): per query, the

@@ -24,6 +24,14 @@ constexpr int kTeHeapMaxK = 128;
 // Heap positions per lane held in registers by the heap kernel (the root
 // and its 9 children); positions beyond them are in LDS.
 constexpr int kTeHeapRegs = 10;
+// The heap kernel's cap on the root: the kTeSampleRank-th smallest key of
+// every stride-th frame, stride = max(kTeSampleStride, kTeSampleStride *
+// (k + 1) / 101), so that on average at most 3.2 of the k + 1 smallest keys
+// are sampled; the cap then holds fewer than k + 1 keys (and the wave
+// repeats its frame loop without it) with probability about 5e-6 per query
+// for frames in random order.
+constexpr int kTeSampleStride = 32;
+constexpr int kTeSampleRank = 14;
 // Queries (64 per query block) per launch pair of the heap and count
 // kernels: each dispatch stays far below 2^32 work-items, and the root
 // buffer at 4 bytes per query below 2 GiB.
