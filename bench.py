@@ -353,7 +353,9 @@ def bench_te(args) -> int:
     """Transfer entropy at 1000 atoms x 10000 frames (Student-t noise), k=100
     (--te-k), lags 1, 5, 10, timed on a sample of ordered pairs with the
     pass-1 kernel --te-path chooses; two pairs per lag checked against the
-    FP64 oracle."""
+    FP64 oracle.  The sample is drawn as the full matrix is launched,
+    target-major: --te-pairs / 4 random target atoms with 4 random sources
+    each (the count kernel takes the pairs of one target four at a time)."""
     from gmx_correlation_amd import dense, ksg, synth
     from oracle import oracle as O
     spec = synth.DENSE_1000
@@ -361,12 +363,15 @@ def bench_te(args) -> int:
     traj = ksg.center(synth.generate(spec))
     n = spec.natoms
     rng = np.random.default_rng(0)
-    npairs = args.te_pairs
-    a1 = rng.integers(0, n, npairs)
-    a2 = (a1 + rng.integers(1, n, npairs)) % n
+    npairs = args.te_pairs // 4 * 4
+    a1 = np.repeat(rng.choice(n, npairs // 4, replace=False), 4)
+    a2 = (a1 + np.concatenate([rng.choice(np.arange(1, n), 4, replace=False)
+                               for _ in range(npairs // 4)])) % n
     k = args.te_k
     dense.te_set_path(args.te_path)
     out = {"workload": f"te 1000 atoms x 10000 frames k={k}", "sample_pairs": npairs,
+           "sample": f"{npairs // 4} random target atoms x 4 random sources (target-major, as "
+                     "the full matrix is launched)",
            "pass1": {0: "automatic", 1: "heap", 2: "radix select"}[args.te_path], "lags": {}}
     for lag in (1, 5, 10):
         te, ms = dense.te_pairs(traj, a1, a2, k=k, lag=lag)

@@ -44,6 +44,16 @@ struct Frame9 {
     double f[9];
 };
 
+// Orders a wave's accesses to its own LDS region: the LDS unit executes a
+// wave's LDS instructions in issue order, so only compiler reordering has to
+// be prevented.
+__device__ __forceinline__ void lds_order()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Nine coordinates of frame j: X_a1(j+lag), X_a1(j), X_a2(j).
 __device__ __forceinline__ void load_frame(const double* __restrict__ X,
                                            const double* __restrict__ Y, int j, int lag,
@@ -226,36 +236,83 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         wk[0] = key;
         ++nw;
     };
+    // LDS stage of one chunk of frames after the heap: stream s (0: X at
+    // j + lag, 1: X at j, 2: Y at j), frame f, coordinate d at
+    // stage[s * 3 kTeStage + 3 f + d]; lane l holds elements l and l + 64
+    // of each stream of the next chunk (zero past the last frame)
+    constexpr int kE = 3 * kTeStage;        // doubles per stream and chunk
+    constexpr int kPer = (kE + 63) / 64;    // per lane
+    double* stage = reinterpret_cast<double*>(heap_lds + (size_t)(kh > kR ? kh - kR : 1) * 64);
+    double pre[3][kPer];
+    auto fetch = [&](int c0) {
+        const int n = min(kTeStage, Nv - c0) * 3;
+        const double* src[3] = {X + (size_t)(c0 + lag) * 3, X + (size_t)c0 * 3, Y + (size_t)c0 * 3};
+#pragma unroll
+        for (int st = 0; st < 3; ++st)
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int e = lane + 64 * u;
+                pre[st][u] = e < n ? src[st][e] : 0.0;
+            }
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int st = 0; st < 3; ++st)
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int e = lane + 64 * u;
+                if (e < kE) stage[st * kE + e] = pre[st][u];
+            }
+    };
+    // frames f and f + 1 of the chunk (f even): three 16-byte reads per stream
+    auto frames = [&](int f, double* c0v, double* c1v) {
+#pragma unroll
+        for (int st = 0; st < 3; ++st) {
+            const double2* p = reinterpret_cast<const double2*>(stage + st * kE) + (f / 2) * 3;
+            const double2 v0 = p[0], v1 = p[1], v2 = p[2];
+            c0v[3 * st] = v0.x;
+            c0v[3 * st + 1] = v0.y;
+            c0v[3 * st + 2] = v1.x;
+            c1v[3 * st] = v1.y;
+            c1v[3 * st + 1] = v2.x;
+            c1v[3 * st + 2] = v2.y;
+        }
+    };
     for (int attempt = 0;; ++attempt) {
         reset_heap();
         top = cap;
 #pragma unroll
         for (int u = 0; u < kW; ++u) wk[u] = INFINITY;
         nw = 0;
-        // two frames per iteration (two independent FP64 chains); frames
-        // j + 2 and j + 3 are loaded while j and j + 1 are processed.  A
-        // lane takes at most two keys per iteration, so the buffer is
-        // flushed once some lane holds kW - 1.
-        double cn0[9], cn1[9];
-        load_frame(X, Y, 0, lag, cn0);
-        load_frame(X, Y, min(1, Nv - 1), lag, cn1);
-        for (int j = 0; j < Nv; j += 2) {
-            double c0[9], c1[9];
-#pragma unroll
-            for (int d = 0; d < 9; ++d) {
-                c0[d] = cn0[d];
-                c1[d] = cn1[d];
+        // The frames pass through LDS in chunks of kTeStage: the wave loads
+        // a chunk's three coordinate streams (X at j + lag, X at j, Y at j)
+        // with coalesced vector loads into registers while it processes the
+        // previous chunk, stores them to its LDS stage, and every lane then
+        // reads each frame from the same LDS address (a broadcast).  Read
+        // frame by frame through the scalar cache instead, the streams of
+        // the waves sharing a scalar cache missed it on most lines and kept
+        // it busy 90% of the time.  Two frames per iteration (two
+        // independent FP64 chains); a lane takes at most two keys per
+        // iteration, so the buffer is flushed once some lane holds kW - 1.
+        fetch(0);
+        for (int c0 = 0; c0 < Nv; c0 += kTeStage) {
+            lds_order();
+            put();
+            lds_order();
+            if (c0 + kTeStage < Nv) fetch(c0 + kTeStage);   // uniform
+            const int nf = min(kTeStage, Nv - c0);
+            for (int f = 0; f < nf; f += 2) {
+                double c0v[9], c1v[9];
+                frames(f, c0v, c1v);
+                double a0, b0, e0, a1, b1, e1;
+                dist9(c0v, a0, b0, e0);
+                dist9(c1v, a1, b1, e1);
+                const float key0 = (float)fmax(fmax(a0, b0), e0);
+                const float key1 = (float)fmax(fmax(a1, b1), e1);
+                if (key0 < top) take(key0);
+                if (f + 1 < nf && key1 < top) take(key1);
+                if (__ballot(nw >= kW - 1)) flush();   // uniform over the wave
             }
-            load_frame(X, Y, min(j + 2, Nv - 1), lag, cn0);
-            load_frame(X, Y, min(j + 3, Nv - 1), lag, cn1);
-            double a0, b0, e0, a1, b1, e1;
-            dist9(c0, a0, b0, e0);
-            dist9(c1, a1, b1, e1);
-            const float key0 = (float)fmax(fmax(a0, b0), e0);
-            const float key1 = (float)fmax(fmax(a1, b1), e1);
-            if (key0 < top) take(key0);
-            if (j + 1 < Nv && key1 < top) take(key1);
-            if (__ballot(nw >= kW - 1)) flush();   // uniform over the wave
         }
         flush();
         // an unfilled slot leaves the root infinite: fewer than k + 1 keys
@@ -341,133 +398,191 @@ __global__ __launch_bounds__(64) void te_select_kernel(TeArgs args)
     args.root[(size_t)blockIdx.x * 64 + lane] = mine;
 }
 
-// Pass 2 and the TE sum.  Wave w of block b runs item 4b + w = query block
-// item % nqblocks of pair item / nqblocks; its partial sum goes to
-// partial[item].
-__global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
+// Pass 2 and the TE sum for G ordered pairs with one target atom a1
+// (TE(a2_u -> a1), u < G): the target's two 3-D distances of a frame (a1's
+// future and present) are computed once for the G pairs, the source's
+// present distance once per pair.  Lane = query frame, the frame loop
+// uniform over the wave (scalar loads of the frame's coordinates).
+//
+// Every frame, the query's own included (removed after the loop), adds its
+// strict FP32 comparisons with the pair's root.  A frame with a key equal
+// to the root, or with a subspace key equal to it, is handled under a
+// wave-uniform branch (rarely taken): frames with the root key are
+// gathered, frames of another key with a subspace key equal to the root
+// recorded (the first kAmb per lane), and both resolved in FP64 after the
+// loop.  fl32 is monotone, so the keys are maxima of the converted 3-D
+// distances.  The partial sum of pair p0 + u goes to
+// partial[(p0 + u) * nqblocks + qb].
+template <int G>
+__device__ __forceinline__ void te_count_group(const TeArgs& args, long long p0, int qb, int lane)
 {
-    const int lane = threadIdx.x & 63;
-    const long long item = (long long)blockIdx.x * 4 +
-                           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int nqb = args.nqblocks;
-    if (item >= args.npairs * nqb) return;
-    const long long p = item / nqb;
-    const int qb = (int)(item - p * nqb);
+    // per pair and lane: the last root-key frame (its distances are
+    // recomputed after the loop) and the first kAmb frames of another key
+    // with a subspace key equal to the root; more of either sends the lane
+    // to the FP64 pass
+    constexpr int kAmb = 2;
     const int N = args.N, lag = args.lag, k = args.k, Nv = N - lag;
-    const double* X = args.traj + (size_t)args.a1[p] * N * 3;
-    const double* Y = args.traj + (size_t)args.a2[p] * N * 3;
+    const int nqb = args.nqblocks;
+    const double* X = args.traj + (size_t)args.a1[p0] * N * 3;
+    const double* Y[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) Y[u] = args.traj + (size_t)args.a2[p0 + u] * N * 3;
     const int q = qb * 64 + lane;
     const bool active = q < Nv;
-    double qc[9];
-    load_frame(X, Y, active ? q : 0, lag, qc);
-    auto dist9 = [&](const double* c, double& dxf, double& dxp, double& dyp) {
-        dxf = fmax(fmax(fabs(qc[0] - c[0]), fabs(qc[1] - c[1])), fabs(qc[2] - c[2]));
-        dxp = fmax(fmax(fabs(qc[3] - c[3]), fabs(qc[4] - c[4])), fabs(qc[5] - c[5]));
-        dyp = fmax(fmax(fabs(qc[6] - c[6]), fabs(qc[7] - c[7])), fabs(qc[8] - c[8]));
+    const int qs = active ? q : 0;
+    double qx[6], qy[G][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        qx[d] = X[(size_t)(qs + lag) * 3 + d];
+        qx[3 + d] = X[(size_t)qs * 3 + d];
+#pragma unroll
+        for (int u = 0; u < G; ++u) qy[u][d] = Y[u][(size_t)qs * 3 + d];
+    }
+    float root[G];
+    int nxfxp[G], nypxp[G], nxp[G], gcnt[G], gj[G], na[G];
+    int aj[G][kAmb];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        root[u] = args.root[((size_t)(p0 + u) * nqb + qb) * 64 + lane];
+        nxfxp[u] = nypxp[u] = nxp[u] = gcnt[u] = gj[u] = na[u] = 0;
+#pragma unroll
+        for (int v = 0; v < kAmb; ++v) aj[u][v] = 0;
+    }
+    // frame j: a1 at j + lag and j, each source at j, read as 32-bit byte
+    // offsets from the atoms' base pointers (rows of 24 bytes; the host
+    // keeps N * 24 below 2^32); the next frame is loaded while this one is
+    // processed
+    auto at = [](const double* base, uint32_t off) {
+        return reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
     };
-    const float root = args.root[(size_t)item * 64 + lane];
-    double cn[9];
-
-    // ---- pass 2: eps and the strict counts from FP32 keys ----------------
-    // Every frame, the self frame included (removed after the loop), adds
-    // its strict FP32 comparisons; frames with the root key, and frames of
-    // another key with a subspace key equal to the root, are gathered under
-    // wave-uniform branches (rarely taken) and resolved in FP64 after the
-    // loop.  fl32 is monotone, so the keys are maxima of the converted 3-D
-    // distances.
-    int clt = 0, gcnt = 0;
-    int nxfxp = 0, nypxp = 0, nxp = 0;
-    double gmin = INFINITY, gmax = -INFINITY;
-    double g1 = 0.0, g2 = 0.0, g3 = 0.0;   // subspace distances of a root-key frame
-    // frames of another key with a subspace key equal to the root: the first
-    // kAmb recorded per lane (na counts them all)
-    constexpr int kAmb = 4;
-    int aj[kAmb] = {0, 0, 0, 0};
-    int na = 0;
-    load_frame(X, Y, 0, lag, cn);
+    auto load = [&](int j, double* f, double (*y)[3]) {
+        const uint32_t of = (uint32_t)(j + lag) * 24u, op = (uint32_t)j * 24u;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            f[d] = at(X, of)[d];
+            f[3 + d] = at(X, op)[d];
+#pragma unroll
+            for (int u = 0; u < G; ++u) y[u][d] = at(Y[u], op)[d];
+        }
+    };
+    auto maxnorm3 = [](const double* a, const double* b) {
+        return fmax(fmax(fabs(a[0] - b[0]), fabs(a[1] - b[1])), fabs(a[2] - b[2]));
+    };
+    double nf[6], ny[G][3];
+    load(0, nf, ny);
     for (int j = 0; j < Nv; ++j) {
-        double c[9];
+        double f[6], y[G][3];
 #pragma unroll
-        for (int d = 0; d < 9; ++d) c[d] = cn[d];
-        load_frame(X, Y, min(j + 1, Nv - 1), lag, cn);
-        double dxf, dxp, dyp;
-        dist9(c, dxf, dxp, dyp);
-        const float fxf = (float)dxf, fxp = (float)dxp, fyp = (float)dyp;
-        const float k1 = fmaxf(fxf, fxp), k2 = fmaxf(fyp, fxp);
-        const float key = fmaxf(k1, fyp);
-        clt += key < root;
-        nxfxp += k1 < root;
-        nypxp += k2 < root;
-        nxp += fxp < root;
-        // a subspace key equal to the root in a frame of another key
-        const bool amb = ((k1 == root) | (k2 == root) | (fxp == root)) & (key != root);
-        if (__ballot(amb)) {   // uniform, rare
-            if (amb) {
+        for (int d = 0; d < 6; ++d) f[d] = nf[d];
 #pragma unroll
-                for (int u = 0; u < kAmb; ++u) aj[u] = na == u ? j : aj[u];
-                ++na;
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) y[u][d] = ny[u][d];
+        load(min(j + 1, Nv - 1), nf, ny);
+        const float fxf = (float)maxnorm3(qx, f), fxp = (float)maxnorm3(qx + 3, f + 3);
+        const float k1 = fmaxf(fxf, fxp);
+        float fyp[G];
+        bool odd[G];
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            fyp[u] = (float)maxnorm3(qy[u], y[u]);
+            const float k2 = fmaxf(fyp[u], fxp);
+            const float rt = root[u];
+            nxfxp[u] += k1 < rt;
+            nypxp[u] += k2 < rt;
+            nxp[u] += fxp < rt;
+            // the key max(k1, fyp) or a subspace key equal to the root
+            odd[u] = (k1 == rt) | (k2 == rt) | (fxp == rt) | (fyp[u] == rt);
+            any |= odd[u];
+        }
+        if (__builtin_amdgcn_ballot_w64(any)) {   // uniform, rare
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (odd[u]) {
+                    const float rt = root[u];
+                    if (fmaxf(k1, fyp[u]) == rt) {
+                        if (j != q) {
+                            ++gcnt[u];
+                            gj[u] = j;
+                        }
+                    } else if ((k1 == rt) | (fmaxf(fyp[u], fxp) == rt) | (fxp == rt)) {
+                        // a subspace key equal to the root in a frame of another key
+#pragma unroll
+                        for (int v = 0; v < kAmb; ++v) aj[u][v] = na[u] == v ? j : aj[u][v];
+                        ++na[u];
+                    }
+                }
             }
         }
-        if (__ballot(key == root)) {   // uniform
-            if (key == root && j != q) {
-                ++gcnt;
-                const double dd = fmax(fmax(dxf, dxp), dyp);
-                gmin = fmin(gmin, dd);
-                gmax = fmax(gmax, dd);
-                g1 = fmax(dxf, dxp);
-                g2 = fmax(dyp, dxp);
-                g3 = dxp;
-            }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const float rt = root[u];
+        double qc[9];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            qc[d] = qx[d];
+            qc[3 + d] = qx[3 + d];
+            qc[6 + d] = qy[u][d];
         }
-    }
-    // the self frame (all distances 0) added one to each count below a
-    // positive root (at a zero root its key equals the root: not counted,
-    // not recorded)
-    if (root > 0.0f) {
-        --clt;
-        --nxfxp;
-        --nypxp;
-        --nxp;
-    }
-    // a root-key frame added its subspace keys below the root to the
-    // counts; the fast path below adds them back as FP64 comparisons with
-    // eps
-    if (gcnt == 1) {
-        const float r1 = (float)g1, r2 = (float)g2, r3 = (float)g3;
-        nxfxp -= r1 < root;
-        nypxp -= r2 < root;
-        nxp -= r3 < root;
-    }
-    if (!active) {
-        // no query: nothing to count
-    } else if (gcnt == 1 && na <= kAmb) {
-        // the one root-key frame is the k-th: eps = its distance, and its
-        // subspace distances (<= eps) count when below it; so do the
-        // recorded frames' subspace distances whose keys equal the root
-        // (their keys below the root are counted already)
-        const double eps = gmin;
-        nxfxp += g1 < eps;
-        nypxp += g2 < eps;
-        nxp += g3 < eps;
-        for (int u = 0; u < kAmb; ++u) {
-            if (u < na) {
+        auto dist9 = [&](const double* c, double& dxf, double& dxp, double& dyp) {
+            dxf = maxnorm3(qc, c);
+            dxp = maxnorm3(qc + 3, c + 3);
+            dyp = maxnorm3(qc + 6, c + 6);
+        };
+        // the self frame (all distances 0) added one to each count below a
+        // positive root (at a zero root its key equals the root: not
+        // counted, not recorded)
+        int cx = nxfxp[u], cy = nypxp[u], cp = nxp[u];
+        if (rt > 0.0f) {
+            --cx;
+            --cy;
+            --cp;
+        }
+        if (!active) {
+            // no query: nothing to count
+        } else if (gcnt[u] == 1 && na[u] <= kAmb) {
+            // the one root-key frame is the k-th: eps = its FP64 distance.
+            // Its subspace keys below the root were counted in the loop; its
+            // subspace distances are compared with eps instead, and so are
+            // the recorded frames' subspace distances whose keys equal the
+            // root (their keys below the root are counted already)
+            double c[9];
+            load_frame(X, Y[u], gj[u], lag, c);
+            double dxf, dxp, dyp;
+            dist9(c, dxf, dxp, dyp);
+            const double eps = fmax(fmax(dxf, dxp), dyp);
+            const double s1 = fmax(dxf, dxp), s2 = fmax(dyp, dxp);
+            cx += (s1 < eps) - ((float)s1 < rt);
+            cy += (s2 < eps) - ((float)s2 < rt);
+            cp += (dxp < eps) - ((float)dxp < rt);
+            for (int v = 0; v < kAmb; ++v) {
+                if (v < na[u]) {
+                    load_frame(X, Y[u], aj[u][v], lag, c);
+                    dist9(c, dxf, dxp, dyp);
+                    const double t1 = fmax(dxf, dxp), t2 = fmax(dyp, dxp);
+                    cx += (float)t1 == rt && t1 < eps;
+                    cy += (float)t2 == rt && t2 < eps;
+                    cp += (float)dxp == rt && dxp < eps;
+                }
+            }
+        } else {
+            // FP32 ties at the root: eps in FP64, then the counts in FP64.
+            // clt = #{j != q: fl32(dist) < root}; eps is the r-th smallest
+            // FP64 distance among the root-key frames, r = k - clt
+            // (1 <= r <= gcnt), found by walking their distinct values
+            // upwards
+            int clt = 0;
+            for (int j = 0; j < Nv; ++j) {
                 double c[9];
-                load_frame(X, Y, aj[u], lag, c);
+                load_frame(X, Y[u], j, lag, c);
                 double dxf, dxp, dyp;
                 dist9(c, dxf, dxp, dyp);
-                const double s1 = fmax(dxf, dxp), s2 = fmax(dyp, dxp);
-                nxfxp += (float)s1 == root && s1 < eps;
-                nypxp += (float)s2 == root && s2 < eps;
-                nxp += (float)dxp == root && dxp < eps;
+                clt += j != q && (float)fmax(fmax(dxf, dxp), dyp) < rt;
             }
-        }
-    } else {
-        // FP32 ties at the root: eps in FP64, then the counts in FP64
-        const int r = k - clt;   // 1 <= r <= gcnt
-        double eps = gmin;
-        if (gcnt > 1 && r > 1 && !(r == gcnt)) {
-            // walk the group's distinct FP64 values upwards
-            double prev = -1.0;
+            const int r = k - clt;
+            double eps = 0.0, prev = -1.0;
             int cum = 0;
             for (;;) {
                 double vmin = INFINITY;
@@ -475,11 +590,11 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
                 for (int j = 0; j < Nv; ++j) {
                     if (j == q) continue;
                     double c[9];
-                    load_frame(X, Y, j, lag, c);
+                    load_frame(X, Y[u], j, lag, c);
                     double dxf, dxp, dyp;
                     dist9(c, dxf, dxp, dyp);
                     const double dd = fmax(fmax(dxf, dxp), dyp);
-                    if ((float)dd == root && dd > prev) {
+                    if ((float)dd == rt && dd > prev) {
                         if (dd < vmin) { vmin = dd; m = 1; }
                         else if (dd == vmin) ++m;
                     }
@@ -488,29 +603,58 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
                 cum += m;
                 prev = vmin;
             }
-        } else if (r == gcnt) {
-            eps = gmax;
+            cx = cy = cp = 0;
+            for (int j = 0; j < Nv; ++j) {
+                double c[9];
+                load_frame(X, Y[u], j, lag, c);
+                double dxf, dxp, dyp;
+                dist9(c, dxf, dxp, dyp);
+                if (j == q) continue;
+                cx += fmax(dxf, dxp) < eps;
+                cy += fmax(dyp, dxp) < eps;
+                cp += dxp < eps;
+            }
         }
-        nxfxp = nypxp = nxp = 0;
-        for (int j = 0; j < Nv; ++j) {
-            double c[9];
-            load_frame(X, Y, j, lag, c);
-            double dxf, dxp, dyp;
-            dist9(c, dxf, dxp, dyp);
-            if (j == q) continue;
-            nxfxp += fmax(dxf, dxp) < eps;
-            nypxp += fmax(dyp, dxp) < eps;
-            nxp += dxp < eps;
+        double te = 0.0;
+        if (active) {
+            const double* psi = args.psi;
+            te = psi[k] + psi[min(cp, Nv) + 1] - psi[min(cx, Nv) + 1] - psi[min(cy, Nv) + 1];
         }
+        // wave sum, fixed order
+        for (int o = 32; o >= 1; o >>= 1) te += __shfl_xor(te, o, 64);
+        if (lane == 0) args.partial[(size_t)(p0 + u) * nqb + qb] = te;
     }
-    double te = 0.0;
-    if (active) {
-        const double* psi = args.psi;
-        te = psi[k] + psi[min(nxp, Nv) + 1] - psi[min(nxfxp, Nv) + 1] - psi[min(nypxp, Nv) + 1];
+}
+
+// Wave w of block b runs item 4b + w = query block item % nqblocks of
+// pair group item / nqblocks, the launch's pairs kTeCountGroup at a time:
+// a group whose pairs share their target atom runs together, any other
+// (the last, short group of a launch, or pairs of different targets) one
+// pair at a time.
+__global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
+{
+    constexpr int G = kTeCountGroup;
+    const int lane = threadIdx.x & 63;
+    const long long item = (long long)blockIdx.x * 4 +
+                           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nqb = args.nqblocks;
+    const long long ngroups = (args.npairs + G - 1) / G;
+    if (item >= ngroups * nqb) return;
+    const long long g = item / nqb;
+    const int qb = (int)(item - g * nqb);
+    const long long p0 = g * G;
+    bool shared = p0 + G <= args.npairs;
+    if (shared) {
+        const int t = args.a1[p0];
+#pragma unroll
+        for (int u = 1; u < G; ++u) shared = shared && args.a1[p0 + u] == t;
     }
-    // wave sum, fixed order
-    for (int o = 32; o >= 1; o >>= 1) te += __shfl_xor(te, o, 64);
-    if (lane == 0) args.partial[item] = te;
+    if (shared) {
+        te_count_group<G>(args, p0, qb, lane);
+    } else {
+        const long long pe = p0 + G < args.npairs ? p0 + G : args.npairs;
+        for (long long p = p0; p < pe; ++p) te_count_group<1>(args, p, qb, lane);
+    }
 }
 
 __global__ void te_finalize_kernel(TeArgs args, double* mat, int natoms)
@@ -536,7 +680,7 @@ hipError_t launch_te(const TeArgs& a, hipStream_t s)
     const long long chunk = te_chunk_pairs(a.nqblocks);
     // heap positions kTeHeapRegs .. k in LDS (one slot when all fit in registers)
     const int slots = a.k + 1 > kTeHeapRegs ? a.k + 1 - kTeHeapRegs : 1;
-    const size_t lds = (size_t)slots * 64 * sizeof(float);
+    const size_t lds = (size_t)slots * 64 * sizeof(float) + 3 * 3 * kTeStage * sizeof(double);
     for (long long p0 = 0; p0 < a.npairs; p0 += chunk) {
         TeArgs c = a;
         c.a1 = a.a1 + p0;
@@ -548,7 +692,8 @@ hipError_t launch_te(const TeArgs& a, hipStream_t s)
             hipLaunchKernelGGL(te_heap_kernel, dim3((unsigned)items), dim3(64), lds, s, c);
         else
             hipLaunchKernelGGL(te_select_kernel, dim3((unsigned)items), dim3(64), 0, s, c);
-        hipLaunchKernelGGL(te_count_kernel, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, s, c);
+        const long long citems = (c.npairs + kTeCountGroup - 1) / kTeCountGroup * a.nqblocks;
+        hipLaunchKernelGGL(te_count_kernel, dim3((unsigned)((citems + 3) / 4)), dim3(256), 0, s, c);
     }
     return hipGetLastError();
 }

@@ -1,6 +1,7 @@
 // Host side of the transfer-entropy matrix (gmxcorr_te_matrix / _pairs).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -43,6 +44,9 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
 {
     if (!traj || natoms < 1 || nframes < 1 || k < 1 || lag < 0)
         return gmxcorr::set_error(GMXCORR_EINVAL, "te: bad arguments");
+    // the count kernel addresses an atom's frames by 32-bit byte offsets
+    if ((long long)nframes * 24 >= (1LL << 32))
+        return gmxcorr::set_error(GMXCORR_EUNSUP, "te: more than 178,956,970 frames");
     const int Nv = nframes - lag;
     if (mat) {
         for (size_t t = 0; t < (size_t)natoms * natoms; ++t) mat[t] = 0.0;
@@ -70,6 +74,25 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
     for (long long p = 0; p < npairs; ++p)
         if (a1[p] < 0 || a2[p] < 0 || a1[p] >= natoms || a2[p] >= natoms || a1[p] == a2[p])
             return gmxcorr::set_error(GMXCORR_EINVAL, "te: pair index out of range or a1 == a2");
+    // a list goes to the device in target-major order (stable), so that
+    // pairs with one target atom sit next to each other and the count
+    // kernel takes them together; order[i] is the caller's index of the
+    // i-th pair launched
+    std::vector<long long> order;
+    if (!mat) {
+        order.resize((size_t)npairs);
+        for (long long p = 0; p < npairs; ++p) order[(size_t)p] = p;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](long long x, long long y) { return a1[x] < a1[y]; });
+        h1.resize((size_t)npairs);
+        h2.resize((size_t)npairs);
+        for (long long i = 0; i < npairs; ++i) {
+            h1[(size_t)i] = a1[order[(size_t)i]];
+            h2[(size_t)i] = a2[order[(size_t)i]];
+        }
+        a1 = h1.data();
+        a2 = h2.data();
+    }
     // digamma table of size Nv+1 as lib/transfer_entropy.cpp:41-48
     std::vector<double> psi(Nv + 1, 0.0);
     psi[1] = -0.57721566490153;
@@ -114,7 +137,7 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
         for (long long p = 0; p < npairs; ++p) {
             double acc = 0.0;
             for (int b = 0; b < nqb; ++b) acc += part[(size_t)p * nqb + b];
-            te_out[p] = acc / Nv;
+            te_out[order[(size_t)p]] = acc / Nv;
         }
     }
     float t = 0.f;

@@ -31,6 +31,12 @@ constexpr int kTeHeapRegs = 10;
 // repeats its frame loop without it) with probability about 5e-6 per query
 // for frames in random order.
 constexpr int kTeSampleStride = 32;
+// Frames per LDS stage chunk of the heap kernel (3 * 3 * 8 bytes each, after
+// the heap's slots; even).
+constexpr int kTeStage = 32;
+// Ordered pairs per wave of the count kernel when they share their target
+// atom (consecutive pairs of a launch; the full matrix is target-major).
+constexpr int kTeCountGroup = 4;
 constexpr int kTeSampleRank = 14;
 // Queries (64 per query block) per launch pair of the heap and count
 // kernels: each dispatch stays far below 2^32 work-items, and the root

@@ -134,3 +134,67 @@ def test_te_select_pass_equals_heap_pass(gpu, case):
             assert np.array_equal(heap, sel), (case, k, lag)
     finally:
         dense.te_set_path(0)
+
+
+def _cap_short_queries(traj, a1, a2, k, lag, stride=32, rank=14):
+    """Queries of TE(a2 -> a1) whose heap cap (the rank-th smallest FP32 key
+    of every stride-th frame) has fewer than k + 1 keys below it."""
+    X = np.asarray(traj[a1], np.float64)   # centred trajectories are [atom][frame][3]
+    Y = np.asarray(traj[a2], np.float64)
+    Nv = len(X) - lag
+    P = np.concatenate([X[lag:lag + Nv], X[:Nv], Y[:Nv]], axis=1)
+    short = 0
+    for q in range(Nv):
+        keys = np.abs(P - P[q]).max(axis=1).astype(np.float32)
+        s = np.sort(keys[::stride])
+        short += len(s) >= rank and int((keys < s[rank - 1]).sum()) < k + 1
+    return short
+
+
+def test_te_heap_cap_and_its_fallback(gpu, oracle_lib):
+    """From 448 frames on, the heap pass offers only keys below the 14th
+    smallest key of every 32nd frame.  A random trajectory takes that path
+    throughout.  In the second, every 32nd 9-D point of TE(1 -> 0) is the
+    same point (atom 0 at frames j and j + 1 and atom 1 at frame j, j a
+    multiple of 32), the others spread over a cube around it: the cap of
+    most queries holds fewer than k + 1 keys (checked here in numpy), and
+    their waves repeat the frame loop without it.  Both equal the FP64
+    oracle."""
+    c = synth.generate(synth.TrajSpec(natoms=3, nframes=1000, nmodes=3, seed=17))
+    traj = ksg.center(c)
+    for k, lag in ((100, 1), (10, 5)):
+        assert _cap_short_queries(traj, 0, 1, k, lag) == 0
+        mat = dense.te_matrix(traj, k=k, lag=lag)
+        ref = oracle_lib.te_matrix(traj, k, lag)
+        assert np.max(np.abs(mat - ref)) <= 1e-12, (k, lag)
+    rng = np.random.default_rng(77)
+    c = rng.uniform(-10.0, 10.0, (1001, 3, 3)).astype(np.float32)
+    for j in range(0, 1000, 32):
+        c[j, 0] = c[j + 1, 0] = c[j, 1] = 0.0
+    traj = ksg.center(c)
+    short = _cap_short_queries(traj, 0, 1, 100, 1)
+    assert short > 100, short
+    mat = dense.te_matrix(traj, k=100, lag=1)
+    ref = oracle_lib.te_matrix(traj, 100, 1)
+    assert np.max(np.abs(mat - ref)) <= 1e-12
+
+
+def test_te_grouped_count_pass_equals_single(gpu, oracle_lib):
+    """The count kernel takes four consecutive ordered pairs with one target
+    atom together (the matrix is launched target-major, a pair list in
+    stable target-major order) and any other pair alone.  A scrambled list
+    with two or six pairs per target equals the matrix entry for entry, bit
+    for bit, and the oracle on a few pairs."""
+    traj = ksg.center(synth.generate(synth.TrajSpec(natoms=7, nframes=600, nmodes=3, seed=23)))
+    k, lag = 20, 2
+    mat = dense.te_matrix(traj, k=k, lag=lag)
+    rng = np.random.default_rng(5)
+    pairs = [(t, s) for t in range(7) for s in range(7) if s != t]
+    keep = [p for p in pairs if not (p[0] == 2 and p[1] > 2) and not (p[0] == 5 and p[1] > 1)]
+    pick = rng.permutation(len(keep))
+    a1 = np.array([keep[i][0] for i in pick], np.int32)
+    a2 = np.array([keep[i][1] for i in pick], np.int32)
+    te, _ = dense.te_pairs(traj, a1, a2, k=k, lag=lag)
+    assert np.array_equal(te, mat[a1, a2])
+    for p in range(3):
+        assert abs(te[p] - oracle_lib.te_pair(traj, int(a1[p]), int(a2[p]), k, lag)) <= 1e-12
