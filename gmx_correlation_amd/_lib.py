@@ -199,3 +199,31 @@ def ptr(a: np.ndarray, typ=_dp):
 
 def probe() -> bool:
     return bool(load().gmxcorr_probe())
+
+
+class TKraskov(ctypes.Structure):
+    """t_kraskov as lib/correlation_core.h:45-52 lays it out."""
+    _fields_ = [("x", ctypes.POINTER(ctypes.POINTER(ctypes.c_double))),
+                ("psi", ctypes.POINTER(ctypes.c_double)),
+                ("scal", ctypes.POINTER(ctypes.c_double)),
+                ("N", ctypes.c_int), ("dim", ctypes.c_int)]
+
+
+def kraskov_corrmatrix_gpu(rows: np.ndarray, psi: np.ndarray, natoms: int, k: int) -> np.ndarray:
+    """Call the exported C++ kraskov_corrmatrix_gpu(const t_kraskov*, int,
+    double*, int) (lib/kraskov_gpu.h:32) on prepared rows [3 natoms][N] and
+    psi [N + 1], as the reference's dispatcher does after kraskov_prepare
+    (lib/correlation_core.cpp:337-348).  Returns the MI matrix."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    nrows, N = rows.shape
+    dp = ctypes.POINTER(ctypes.c_double)
+    ptrs = (dp * nrows)(*[rows[c].ctypes.data_as(dp) for c in range(nrows)])
+    scal = np.ones(nrows)
+    kr = TKraskov(ptrs, psi.ctypes.data_as(dp), scal.ctypes.data_as(dp), N, nrows)
+    mat = np.zeros((natoms, natoms))
+    fn = getattr(load(), CXX_SYMBOLS[1])
+    fn.argtypes = [ctypes.POINTER(TKraskov), ctypes.c_int, dp, ctypes.c_int]
+    fn.restype = None
+    fn(ctypes.byref(kr), natoms, mat.ctypes.data_as(dp), k)
+    return mat
