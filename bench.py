@@ -354,8 +354,9 @@ def bench_te(args) -> int:
     (--te-k), lags 1, 5, 10, timed on a sample of ordered pairs with the
     pass-1 kernel --te-path chooses; two pairs per lag checked against the
     FP64 oracle.  The sample is drawn as the full matrix is launched,
-    target-major: --te-pairs / 4 random target atoms with 4 random sources
-    each (the count kernel takes the pairs of one target four at a time)."""
+    target-major: --te-pairs / 64 random target atoms with 64 random sources
+    each (the count kernel takes the pairs of one target four at a time, and
+    the prefix stores of a target serve all its pairs)."""
     from gmx_correlation_amd import dense, ksg, synth
     from oracle import oracle as O
     spec = synth.DENSE_1000
@@ -363,16 +364,18 @@ def bench_te(args) -> int:
     traj = ksg.center(synth.generate(spec))
     n = spec.natoms
     rng = np.random.default_rng(0)
-    npairs = args.te_pairs // 4 * 4
-    a1 = np.repeat(rng.choice(n, npairs // 4, replace=False), 4)
-    a2 = (a1 + np.concatenate([rng.choice(np.arange(1, n), 4, replace=False)
-                               for _ in range(npairs // 4)])) % n
+    per = 64
+    npairs = max(args.te_pairs // per, 1) * per
+    a1 = np.repeat(rng.choice(n, npairs // per, replace=False), per)
+    a2 = (a1 + np.concatenate([rng.choice(np.arange(1, n), per, replace=False)
+                               for _ in range(npairs // per)])) % n
     k = args.te_k
     dense.te_set_path(args.te_path)
     out = {"workload": f"te 1000 atoms x 10000 frames k={k}", "sample_pairs": npairs,
-           "sample": f"{npairs // 4} random target atoms x 4 random sources (target-major, as "
-                     "the full matrix is launched)",
-           "pass1": {0: "automatic", 1: "heap", 2: "radix select"}[args.te_path], "lags": {}}
+           "sample": f"{npairs // per} random target atoms x {per} random sources (target-major, "
+                     "as the full matrix is launched)",
+           "pass1": {0: "automatic", 1: "heap", 2: "radix select", 3: "prefix stores"}[args.te_path],
+           "lags": {}}
     for lag in (1, 5, 10):
         te, ms = dense.te_pairs(traj, a1, a2, k=k, lag=lag)
         chk = []
@@ -380,7 +383,8 @@ def bench_te(args) -> int:
             chk.append(abs(te[p] - O.te_pair(traj, int(a1[p]), int(a2[p]), k, lag)))
         out["lags"][lag] = {"kernel_ms": ms, "pairs_per_s": npairs / (ms / 1e3),
                             "full_matrix_s_projected": n * (n - 1) / (npairs / (ms / 1e3)),
-                            "max_abs_dte_vs_fp64_oracle": float(max(chk))}
+                            "max_abs_dte_vs_fp64_oracle": float(max(chk)),
+                            "counts": dense.te_stats()}
         log(f"bench: te lag {lag}: {out['lags'][lag]}")
     print(json.dumps(out), flush=True)
     return 0
@@ -406,10 +410,15 @@ def bench_te_full(args) -> int:
     chunk = 50000
     kern_ms = 0.0
     t0 = time.perf_counter()
+    counts = {"queries": 0, "prefix_resolved": 0, "heap_listed": 0, "prefix_path_calls": 0}
     for c0 in range(0, len(a1), chunk):
         v, ms = dense.te_pairs(traj, a1[c0:c0 + chunk], a2[c0:c0 + chunk], k=k, lag=lag)
         te[c0:c0 + chunk] = v
         kern_ms += ms
+        st = dense.te_stats()
+        for key in ("queries", "prefix_resolved", "heap_listed"):
+            counts[key] += st[key]
+        counts["prefix_path_calls"] += int(st["prefix_path"])
         log(f"bench: te lag {lag}: {min(c0 + chunk, len(a1))} / {len(a1)} ordered pairs, "
             f"{time.perf_counter() - t0:.1f} s")
     wall = time.perf_counter() - t0
@@ -422,7 +431,7 @@ def bench_te_full(args) -> int:
            "ordered_pairs": int(len(a1)), "wall_s": wall, "kernel_s": kern_ms / 1e3,
            "ordered_pairs_per_s": len(a1) / wall,
            "sampled_pairs_vs_fp64_oracle": 24, "max_abs_dte": d,
-           "te_mean": float(te.mean()), "te_max": float(te.max())}
+           "te_mean": float(te.mean()), "te_max": float(te.max()), "counts": counts}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -461,8 +470,9 @@ def main() -> int:
     ap.add_argument("--te-pairs", type=int, default=512, help="ordered pairs timed (te)")
     ap.add_argument("--lag", type=int, default=1, help="lag of the full TE matrix (te_full)")
     ap.add_argument("--te-k", type=int, default=100, help="k of the sampled TE workload (te)")
-    ap.add_argument("--te-path", type=int, default=0, choices=(0, 1, 2),
-                    help="TE pass-1 kernel (te): 0 automatic, 1 heap, 2 radix select")
+    ap.add_argument("--te-path", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="TE pass-1 kernel (te): 0 automatic, 1 heap, 2 radix select, "
+                         "3 prefix stores")
     ap.add_argument("--row-budget-gb", type=float, default=250.0,
                     help="distance-row bytes per memory wave (batched)")
     args = ap.parse_args()

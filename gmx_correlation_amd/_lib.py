@@ -23,6 +23,7 @@ _u8p = ctypes.POINTER(ctypes.c_ubyte)
 _fp = ctypes.POINTER(ctypes.c_float)
 _i = ctypes.c_int
 _ll = ctypes.c_longlong
+_llp = ctypes.POINTER(ctypes.c_longlong)
 
 
 class KsgStats(ctypes.Structure):
@@ -104,7 +105,7 @@ C_SYMBOLS = [
     "gmxcorr_ksg_plan_set_shape", "gmxcorr_ksg_pairs_device",
     "gmxcorr_ksg_pairs_device_noalloc",
     "gmxcorr_gauss_matrix", "gmxcorr_gcmi_matrix", "gmxcorr_dense_set_slab_frames",
-    "gmxcorr_te_matrix", "gmxcorr_te_pairs", "gmxcorr_te_set_path",
+    "gmxcorr_te_matrix", "gmxcorr_te_pairs", "gmxcorr_te_set_path", "gmxcorr_te_stats",
     "gmxcorr_ksg_batch_create", "gmxcorr_ksg_batch_add", "gmxcorr_ksg_batch_run",
     "gmxcorr_ksg_batch_traj_bytes", "gmxcorr_ksg_batch_set_chunk",
     "gmxcorr_ksg_batch_result", "gmxcorr_ksg_batch_stats", "gmxcorr_ksg_batch_destroy",
@@ -169,6 +170,7 @@ def load() -> ctypes.CDLL:
     L.gmxcorr_te_matrix.argtypes = [_dp, _i, _i, _i, _i, _dp, _dp]
     L.gmxcorr_te_pairs.argtypes = [_dp, _i, _i, _i, _i, _ll, _ip, _ip, _dp, _dp]
     L.gmxcorr_te_set_path.argtypes = [_i]
+    L.gmxcorr_te_stats.argtypes = [_llp]
     L.gmxcorr_ksg_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
     L.gmxcorr_ksg_batch_add.argtypes = [_vp, _dp, _dp, _i, _i, _i, _i, _i]
     L.gmxcorr_ksg_batch_traj_bytes.argtypes = [_vp, _i]

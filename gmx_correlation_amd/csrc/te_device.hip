@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sort_network.h"
 #include "te_internal.h"
 
 namespace gmxcorr {
@@ -67,6 +68,43 @@ __device__ __forceinline__ void load_frame(const double* __restrict__ X,
     }
 }
 
+// Inclusive prefix sum over the wave's 64 lanes.
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int up = __shfl_up(v, o, 64);
+        if ((int)(threadIdx.x & 63) >= o) v += up;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
+// Number of set bits of m below this lane.
+__device__ __forceinline__ int mbcnt(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, int l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
 }  // namespace
 
 // Pass 1.  Block b = one wave = query block b % nqblocks of pair b /
@@ -80,13 +118,25 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
     auto heap = [&](int t) -> float& { return heap_lds[(t - kR) * 64 + threadIdx.x]; };
     const int lane = threadIdx.x;
     const int nqb = args.nqblocks;
-    const long long p = blockIdx.x / nqb;
-    const int qb = blockIdx.x % nqb;
     const int N = args.N, lag = args.lag, k = args.k, Nv = N - lag;
+    // list mode: the queries of pair p listed from position first on
+    const bool listed = args.qblocks != nullptr;   // uniform
+    long long p;
+    int q;
+    bool active;
+    if (listed) {
+        const int2 qbk = args.qblocks[blockIdx.x];
+        p = qbk.x;
+        const int idx = qbk.y + lane;
+        active = idx < (int)args.qcount[p];
+        q = active ? (int)args.qlist[(size_t)p * Nv + idx] : 0;
+    } else {
+        p = blockIdx.x / nqb;
+        q = (int)(blockIdx.x % nqb) * 64 + lane;
+        active = q < Nv;
+    }
     const double* X = args.traj + (size_t)args.a1[p] * N * 3;
     const double* Y = args.traj + (size_t)args.a2[p] * N * 3;
-    const int q = qb * 64 + lane;
-    const bool active = q < Nv;
     double qc[9];
     load_frame(X, Y, active ? q : 0, lag, qc);
     auto dist9 = [&](const double* c, double& dxf, double& dxp, double& dyp) {
@@ -322,7 +372,12 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         if (!__ballot(short_)) break;   // uniform over the wave
         cap = INFINITY;
     }
-    args.root[(size_t)blockIdx.x * 64 + lane] = r[0];
+    // the root of query q of pair p at root[(p * nqb) * 64 + q]
+    if (listed) {
+        if (active) args.root[(size_t)p * nqb * 64 + q] = r[0];
+    } else {
+        args.root[(size_t)blockIdx.x * 64 + lane] = r[0];
+    }
 }
 
 // Pass 1 for any k (the heap kernel's LDS stops at k = 128): per query, the
@@ -657,6 +712,334 @@ __global__ __launch_bounds__(256) void te_count_kernel(TeArgs args)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Prefix path, build: one block of 512 threads per row (target slot t,
+// query q), thread `tid` holding frames tid + 512 e (e < E) in registers
+// (coalesced coordinate loads).  Keys are fl32 of the FP64 target-only
+// distance M(q, j) = max(|X(q+L) - X(j+L)|, |X(q) - X(j)|).  The query's own
+// frame (key 0) is always stored; of the other frames, those below the
+// value v* of rank kTePrefix - 2 (a radix select over v - lo, 8 bits per
+// pass from the top set bit of hi - lo) and as many frames at v* as that
+// rank leaves.  Which of several frames at v* are kept does not matter:
+// every frame left out has a key >= v* = H, the bound the select checks
+// against.  With Nv <= kTePrefix every frame is stored and the rest padded
+// with +inf keys (frame 0).  The kTePrefix (key, frame) pairs are sorted
+// two per thread by the register network of sort_network.h.
+// ---------------------------------------------------------------------------
+template <int E>
+__global__ __launch_bounds__(512) void te_prefix_build_kernel(TeArgs args, TePrefixArgs pa)
+{
+    constexpr int BT = 512;
+    constexpr uint32_t kNone = 0xffffffffu;   // the own frame and frames past Nv
+    static_assert(kTePrefix == 2 * BT, "two stored keys per thread");
+    __shared__ uint64_t sk[kTePrefix];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t wlo[BT / 64], whi[BT / 64];
+    __shared__ uint32_t sel[4];   // chosen bin, rank left within it, slot and tie counters
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int N = args.N, lag = args.lag, Nv = N - lag;
+    const long long nrows = pa.ntargets * (long long)Nv;
+    for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {   // uniform
+        const long long t = row / Nv;
+        const int q = (int)(row - t * Nv);
+        const double* X = args.traj + (size_t)pa.targets[t] * N * 3;
+        double qf[3], qp[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            qf[d] = X[(size_t)(q + lag) * 3 + d];
+            qp[d] = X[(size_t)q * 3 + d];
+        }
+        uint32_t v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int j = tid + BT * e;
+            v[e] = kNone;
+            if (j < Nv && j != q) {
+                const double* f = X + (size_t)(j + lag) * 3;
+                const double* c = X + (size_t)j * 3;
+                const double a = fmax(fmax(fabs(qf[0] - f[0]), fabs(qf[1] - f[1])), fabs(qf[2] - f[2]));
+                const double b = fmax(fmax(fabs(qp[0] - c[0]), fabs(qp[1] - c[1])), fabs(qp[2] - c[2]));
+                v[e] = __float_as_uint((float)fmax(a, b));
+            }
+        }
+        // v* and the number of frames at v* to keep (rank + 1)
+        uint32_t vstar = kNone;
+        int rank = -1;
+        if (Nv - 1 > kTePrefix - 1) {
+            uint32_t mn = kNone, mx = 0u;
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (v[e] != kNone) {
+                    mn = min(mn, v[e]);
+                    mx = max(mx, v[e]);
+                }
+            mn = wave_min_u(mn);
+            mx = wave_max_u(mx);
+            if (lane == 0) {
+                wlo[wv] = mn;
+                whi[wv] = mx;
+            }
+            __syncthreads();
+            uint32_t lo = wlo[0], hi = whi[0];
+#pragma unroll
+            for (int u = 1; u < BT / 64; ++u) {
+                lo = min(lo, wlo[u]);
+                hi = max(hi, whi[u]);
+            }
+            // keys are finite floats (< 2^31 as bits): hi - lo < 2^31
+            const int nb = hi > lo ? 32 - __builtin_clz(hi - lo) : 0;
+            uint32_t pre = 0u;
+            uint32_t rk = kTePrefix - 2;   // 0-based among the other frames
+            for (int top = nb; top > 0;) {   // uniform
+                const int shift = max(top - 8, 0);
+                if (tid < 256) hist[tid] = 0u;
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const uint32_t dv = v[e] - lo;
+                    if (v[e] <= hi && (dv >> top) == pre) atomicAdd(&hist[(dv >> shift) & 255u], 1u);
+                }
+                __syncthreads();
+                if (tid < 64) {
+                    const uint4 h = reinterpret_cast<const uint4*>(hist)[lane];
+                    const uint32_t own = h.x + h.y + h.z + h.w;
+                    const uint32_t incl = (uint32_t)wave_incl_scan((int)own);
+                    const uint32_t excl = incl - own;
+                    if (excl <= rk && rk < incl) {
+                        uint32_t r = rk - excl, bin = 4u * lane;
+                        if (r >= h.x) { r -= h.x; ++bin;
+                            if (r >= h.y) { r -= h.y; ++bin;
+                                if (r >= h.z) { r -= h.z; ++bin; } } }
+                        sel[0] = bin;
+                        sel[1] = r;
+                    }
+                }
+                __syncthreads();
+                pre = (pre << (top - shift)) | sel[0];
+                rk = sel[1];
+                top = shift;
+                __syncthreads();   // sel and hist are rewritten by the next pass
+            }
+            vstar = pre + lo;
+            rank = (int)rk;
+        }
+        if (tid == 0) {
+            sel[2] = 0u;
+            sel[3] = 0u;
+            sk[0] = (uint64_t)(uint32_t)q;   // the own frame, key 0
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            bool take = v[e] < vstar;
+            if (v[e] == vstar && v[e] != kNone) take = (int)atomicAdd(&sel[3], 1u) <= rank;
+            if (take) {
+                const uint32_t slot = atomicAdd(&sel[2], 1u);
+                if (slot < (uint32_t)(kTePrefix - 1))
+                    sk[1 + slot] = ((uint64_t)v[e] << 32) | (uint32_t)(tid + BT * e);
+            }
+        }
+        __syncthreads();
+        const int filled = 1 + (int)min(sel[2], (uint32_t)(kTePrefix - 1));
+        for (int i = filled + tid; i < kTePrefix; i += BT) sk[i] = (uint64_t)0x7f800000u << 32;
+        __syncthreads();
+        uint64_t pk[2] = {sk[2 * tid], sk[2 * tid + 1]};
+        sortnet::sort_blocked<uint64_t, 2, BT>(pk, sk, tid);
+        const size_t base = (size_t)row * kTePrefix;
+        reinterpret_cast<uint2*>(pa.tv + base)[tid] =
+            make_uint2((uint32_t)(pk[0] >> 32), (uint32_t)(pk[1] >> 32));
+        reinterpret_cast<uint32_t*>(pa.ti + base)[tid] =
+            (uint32_t)(pk[0] & 0xffffu) | ((uint32_t)(pk[1] & 0xffffu) << 16);
+        __syncthreads();   // sk, hist and sel are rewritten by the next row
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Prefix path, select: the T-th smallest of a wave's NK * 64 joint keys
+// (key[m] at lane l is prefix position l + 64 m), T = k + 1 (the own frame
+// included), known to lie in [L, H) when it is below H:
+//   histogram  256 bins over [L, H) in the key's bit pattern, keys below L
+//              counted in bin 0 by ballots, keys >= H left out (the last bin
+//              holds a count that always reaches T);
+//   select     the target bin's keys compacted to LDS (hist, the wave's 256
+//              words), ballot radix select.
+// Returns false when the T-th key is not below H, lies in bin 0, or its bin
+// holds more than 64 keys.
+// ---------------------------------------------------------------------------
+template <int NK>
+__device__ __forceinline__ bool prefix_select(const uint32_t (&key)[NK], uint32_t L, uint32_t H,
+                                              int T, uint32_t* hist, int lane, uint32_t& eps_key)
+{
+    constexpr int kB = 256;
+    const uint32_t span1 = H > L ? H - L - 1u : 0u;
+    int sh = max(0, 24 - (int)__builtin_clz(span1 | 1u));
+    if ((span1 >> sh) > (uint32_t)(kB - 3)) ++sh;
+    const uint32_t Lm = L - min(L, 1u << sh);
+    const uint32_t B1 = Lm + (1u << sh);
+    int cb = 0;
+#pragma unroll
+    for (int m = 0; m < NK; ++m) cb += __popcll(__ballot(key[m] < B1));
+    reinterpret_cast<uint4*>(hist)[lane] =
+        make_uint4(lane == 0 ? (uint32_t)cb : 0u, 0u, 0u, lane == 63 ? 4096u : 0u);
+    lds_order();
+    const uint32_t span = H > B1 ? H - B1 : 0u;
+#pragma unroll
+    for (int m = 0; m < NK; ++m) {
+        const uint32_t dk = key[m] - B1;   // keys below B1 wrap above the span
+        if (dk < span) atomicAdd(&hist[(dk >> sh) + 1u], 1u);
+    }
+    lds_order();
+    // bin holding the T-th key: lane l owns bins 4l .. 4l + 3
+    const uint4 hv = reinterpret_cast<const uint4*>(hist)[lane];
+    const int hs = (int)((hv.x + hv.y) + (hv.z + hv.w));
+    const int incl = wave_incl_scan(hs);
+    const int excl = incl - hs;
+    const uint64_t bal = __ballot((uint32_t)(T - 1 - excl) < (uint32_t)hs);
+    const int owner = bal ? __ffsll((unsigned long long)bal) - 1 : 0;
+    const int r = T - __builtin_amdgcn_readlane(excl, owner);
+    const int h0 = __builtin_amdgcn_readlane((int)hv.x, owner);
+    const int h1 = __builtin_amdgcn_readlane((int)hv.y, owner);
+    const int h2 = __builtin_amdgcn_readlane((int)hv.z, owner);
+    const int h3 = __builtin_amdgcn_readlane((int)hv.w, owner);
+    int sub = 0, below = 0, pop = h0;
+    if (r > h0 && r <= h0 + h1) { sub = 1; below = h0; pop = h1; }
+    else if (r > h0 + h1 && r <= h0 + h1 + h2) { sub = 2; below = h0 + h1; pop = h2; }
+    else if (r > h0 + h1 + h2) { sub = 3; below = h0 + h1 + h2; pop = h3; }
+    const int b1 = owner * 4 + sub;
+    const int r0 = r - below;
+    lds_order();
+    if (b1 == 0 || b1 == kB - 1 || pop > 64) return false;
+    const uint32_t lo = Lm + ((uint32_t)b1 << sh);
+    const uint32_t lim = min(1u << sh, H > lo ? H - lo : 0u);
+    int nstored = 0;
+#pragma unroll
+    for (int m = 0; m < NK; ++m) {
+        const bool inbin = key[m] - lo < lim;   // keys below lo wrap
+        const uint64_t cm = __ballot(inbin);
+        if (cm) {
+            if (inbin) hist[nstored + mbcnt(cm)] = key[m] - lo;
+            nstored += __popcll(cm);
+        }
+    }
+    lds_order();
+    const bool act = lane < pop;
+    const uint32_t low = act ? hist[lane] : 0xffffffffu;
+    uint64_t live = __ballot(act);
+    uint32_t pre = 0u;
+    int rr = r0;
+    for (int bit = sh - 1; bit >= 0 && __popcll(live) > 1; --bit) {
+        const uint64_t z = __ballot((low >> bit) == (pre << 1));
+        const int c0 = __popcll(z);
+        if (rr > c0) {
+            rr -= c0;
+            pre = (pre << 1) | 1u;
+            live &= ~z;
+        } else {
+            pre <<= 1;
+            live = z;
+        }
+    }
+    const uint32_t sv = readlane_u(low, __ffsll((unsigned long long)live) - 1);
+    eps_key = lo + sv;
+    // the r0-th smallest of the bin: fewer than r0 below it, r0 or more at
+    // or below it
+    const int lt = __popcll(__ballot(low < sv)), le = __popcll(__ballot(low <= sv));
+    lds_order();
+    return nstored == pop && lt < r0 && r0 <= le && eps_key < H;
+}
+
+// ---------------------------------------------------------------------------
+// Prefix path, select kernel: block = query block qb of pair p (four waves,
+// 16 queries each, one after another).  Lane l holds prefix positions
+// l + 64 m: the target-only key from the store and the source's distance of
+// the same frame, FP64 from the gathered coordinates (one 16-byte and one
+// 8-byte load), joined as key = max(fl32(M), fl32(dY)).  L = the store's key
+// at position k, a lower bound of the T-th joint key.  Two stages:
+// positions 0 .. 511 with H = the store's key at 511, and, when the T-th key
+// of those is not below it, all 1024 with H = the store's last key.  With
+// every frame stored (Nv <= the stage's positions) H is one above the
+// largest joint key.  The root goes to root[p nqb 64 + q]; a query that
+// neither stage resolves is appended to its pair's list for the heap kernel.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void te_prefix_select_kernel(TeArgs args, TePrefixArgs pa)
+{
+    constexpr int kM = kTePrefix / 64;   // keys per lane, both stages
+    constexpr int kA = kM / 2;           // keys per lane of the first stage
+    typedef double dpair __attribute__((ext_vector_type(2), aligned(8)));
+    __shared__ __align__(16) uint32_t hist_all[4][256];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint32_t* hist = hist_all[w];
+    const int nqb = args.nqblocks;
+    const long long p = blockIdx.x / nqb;
+    const int qb = (int)(blockIdx.x - p * nqb);
+    const int N = args.N, lag = args.lag, k = args.k, Nv = N - lag;
+    const int T = k + 1;
+    const double* Y = args.traj + (size_t)args.a2[p] * N * 3;
+    const long long slot = pa.tslot[p];
+    for (int i = 0; i < 16; ++i) {
+        const int q = qb * 64 + w * 16 + i;
+        if (q >= Nv) break;   // uniform
+        const size_t rbase = ((size_t)slot * Nv + q) * kTePrefix;
+        const double y0 = Y[(size_t)q * 3], y1 = Y[(size_t)q * 3 + 1], y2 = Y[(size_t)q * 3 + 2];
+        uint32_t key[kM];
+        // positions m * 64 + lane for m in [m0, m1): store keys joined with
+        // the source's distances
+        auto join = [&](int m0, int m1) {
+            uint32_t fr[kM];
+#pragma unroll
+            for (int m = 0; m < kM; ++m) {
+                if (m < m0 || m >= m1) continue;
+                key[m] = pa.tv[rbase + lane + 64 * m];
+                fr[m] = pa.ti[rbase + lane + 64 * m];
+            }
+#pragma unroll
+            for (int m = 0; m < kM; ++m) {
+                if (m < m0 || m >= m1) continue;
+                const double* yj = Y + (size_t)fr[m] * 3;
+                const dpair xy = *reinterpret_cast<const dpair*>(yj);
+                const double d = fmax(fmax(fabs(y0 - xy.x), fabs(y1 - xy.y)), fabs(y2 - yj[2]));
+                key[m] = max(key[m], __float_as_uint((float)d));
+            }
+        };
+        // one above the largest joint key of the real frames below position n
+        auto above_all = [&](int nk, int n) {
+            uint32_t mx = 0u;
+#pragma unroll
+            for (int m = 0; m < kM; ++m)
+                if (m < nk && lane + 64 * m < n) mx = max(mx, key[m]);
+            return wave_max_u(mx) + 1u;
+        };
+        // L and the stages' bounds are the store's target-only keys (the
+        // joined keys are larger)
+        const int km = k >> 6;   // k <= 128: m = 0, 1 or 2
+        const uint32_t L = readlane_u(pa.tv[rbase + (size_t)(64 * km + lane)], k & 63);
+        const uint32_t storeA = readlane_u(pa.tv[rbase + (size_t)(64 * (kA - 1) + lane)], 63);
+        const uint32_t storeB = readlane_u(pa.tv[rbase + (size_t)(64 * (kM - 1) + lane)], 63);
+        uint32_t eps_key = 0u;
+        join(0, kA);
+        const uint32_t HA = Nv <= 64 * kA ? above_all(kA, Nv) : storeA;
+        // the first stage's keys as an array of their own (registers)
+        uint32_t ka[kA];
+#pragma unroll
+        for (int m = 0; m < kA; ++m) ka[m] = key[m];
+        bool ok = prefix_select<kA>(ka, L, HA, T, hist, lane, eps_key);
+        if (!ok && Nv > 64 * kA) {   // uniform
+            join(kA, kM);
+            const uint32_t HB = Nv <= kTePrefix ? above_all(kM, Nv) : storeB;
+            ok = prefix_select<kM>(key, L, HB, T, hist, lane, eps_key);
+        }
+        if (lane == 0) {
+            if (ok) {
+                args.root[(size_t)p * nqb * 64 + q] = __uint_as_float(eps_key);
+            } else {
+                const unsigned s = atomicAdd(&pa.qcount[p], 1u);
+                pa.qlist[(size_t)p * Nv + s] = (uint16_t)q;
+            }
+        }
+    }
+}
+
 __global__ void te_finalize_kernel(TeArgs args, double* mat, int natoms)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -672,15 +1055,15 @@ long long te_chunk_pairs(int nqblocks)
     return c > 0 ? c : 1;
 }
 
+static size_t heap_lds_bytes(int k);
+
 hipError_t launch_te(const TeArgs& a, hipStream_t s)
 {
     if (a.k < 1 || !a.root) return hipErrorInvalidValue;
     const bool heap = a.path == 1 || (a.path == 0 && a.k <= kTeHeapMaxK);
     if (heap && a.k > kTeHeapMaxK) return hipErrorInvalidValue;
     const long long chunk = te_chunk_pairs(a.nqblocks);
-    // heap positions kTeHeapRegs .. k in LDS (one slot when all fit in registers)
-    const int slots = a.k + 1 > kTeHeapRegs ? a.k + 1 - kTeHeapRegs : 1;
-    const size_t lds = (size_t)slots * 64 * sizeof(float) + 3 * 3 * kTeStage * sizeof(double);
+    const size_t lds = heap_lds_bytes(a.k);
     for (long long p0 = 0; p0 < a.npairs; p0 += chunk) {
         TeArgs c = a;
         c.a1 = a.a1 + p0;
@@ -692,9 +1075,58 @@ hipError_t launch_te(const TeArgs& a, hipStream_t s)
             hipLaunchKernelGGL(te_heap_kernel, dim3((unsigned)items), dim3(64), lds, s, c);
         else
             hipLaunchKernelGGL(te_select_kernel, dim3((unsigned)items), dim3(64), 0, s, c);
-        const long long citems = (c.npairs + kTeCountGroup - 1) / kTeCountGroup * a.nqblocks;
-        hipLaunchKernelGGL(te_count_kernel, dim3((unsigned)((citems + 3) / 4)), dim3(256), 0, s, c);
+        const hipError_t e = launch_te_count(c, s);
+        if (e != hipSuccess) return e;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_te_count(const TeArgs& c, hipStream_t s)
+{
+    const long long citems = (c.npairs + kTeCountGroup - 1) / kTeCountGroup * c.nqblocks;
+    hipLaunchKernelGGL(te_count_kernel, dim3((unsigned)((citems + 3) / 4)), dim3(256), 0, s, c);
+    return hipGetLastError();
+}
+
+// heap positions kTeHeapRegs .. k in LDS (one slot when all fit in
+// registers), then the frame stage
+static size_t heap_lds_bytes(int k)
+{
+    const int slots = k + 1 > kTeHeapRegs ? k + 1 - kTeHeapRegs : 1;
+    return (size_t)slots * 64 * sizeof(float) + 3 * 3 * kTeStage * sizeof(double);
+}
+
+hipError_t launch_te_heap_list(const TeArgs& a, long long nblocks, hipStream_t s)
+{
+    if (a.k < 1 || a.k > kTeHeapMaxK || !a.qblocks || !a.qcount || !a.qlist)
+        return hipErrorInvalidValue;
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(te_heap_kernel, dim3((unsigned)nblocks), dim3(64), heap_lds_bytes(a.k), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_te_prefix_build(const TeArgs& a, const TePrefixArgs& pa, hipStream_t s)
+{
+    const int Nv = a.N - a.lag;
+    if (Nv < 1 || Nv > kTePrefixMaxFrames || pa.ntargets < 1) return hipErrorInvalidValue;
+    const long long rows = pa.ntargets * (long long)Nv;
+    const unsigned grid = (unsigned)(rows < (1LL << 22) ? rows : (1LL << 22));
+    if (Nv <= 512 * 4)
+        hipLaunchKernelGGL(te_prefix_build_kernel<4>, dim3(grid), dim3(512), 0, s, a, pa);
+    else if (Nv <= 512 * 16)
+        hipLaunchKernelGGL(te_prefix_build_kernel<16>, dim3(grid), dim3(512), 0, s, a, pa);
+    else
+        hipLaunchKernelGGL(te_prefix_build_kernel<32>, dim3(grid), dim3(512), 0, s, a, pa);
+    return hipGetLastError();
+}
+
+hipError_t launch_te_prefix_select(const TeArgs& a, const TePrefixArgs& pa, hipStream_t s)
+{
+    const int Nv = a.N - a.lag;
+    if (a.k < 1 || a.k + 1 > kTePrefix || a.k > kTeHeapMaxK || Nv > kTePrefixMaxFrames)
+        return hipErrorInvalidValue;
+    const long long items = a.npairs * a.nqblocks;
+    hipLaunchKernelGGL(te_prefix_select_kernel, dim3((unsigned)items), dim3(256), 0, s, a, pa);
     return hipGetLastError();
 }
 

@@ -34,8 +34,106 @@ struct DevBuf {
     hipError_t alloc(size_t n) { return hipMalloc(&p, n * sizeof(T)); }
 };
 
-// Pass-1 kernel of later calls (gmxcorr_te_set_path): 0 automatic, 1 heap, 2 select.
+// Pass-1 kernel of later calls (gmxcorr_te_set_path): 0 automatic, 1 heap,
+// 2 select, 3 prefix stores (with the heap kernel for the queries they
+// leave).
 int g_path = 0;
+// The last call: queries, queries resolved by the prefix select, queries
+// left to the heap kernel, whether the prefix path ran.
+long long g_stats[4] = {0, 0, 0, 0};
+
+// The prefix path over pairs a.a1 / a.a2 (target-major; h1 is the host copy
+// of a.a1): per launch chunk of at most te_chunk_pairs pairs and maxT
+// distinct target atoms, the targets' prefix stores, the select, the heap
+// kernel over the listed queries, the count kernel.
+int run_prefix(const gmxcorr::TeArgs& a, const int* h1, hipStream_t s)
+{
+    using namespace gmxcorr;
+    const int Nv = a.N - a.lag;
+    const int nqb = a.nqblocks;
+    const long long npairs = a.npairs;
+    const long long chunk = te_chunk_pairs(nqb);
+    const size_t per_target = (size_t)Nv * kTePrefix * (sizeof(uint32_t) + sizeof(uint16_t));
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const size_t budget = std::min(free_b / 2, (size_t)32 << 30);
+    long long distinct = 0;
+    for (long long p = 0; p < npairs; ++p) distinct += p == 0 || h1[p] != h1[p - 1];
+    long long maxT = std::min((long long)(budget / per_target), distinct);
+    if (maxT < 1) return gmxcorr::set_error(GMXCORR_ENOMEM, "te: no room for one target's prefix stores");
+    const long long npc = std::min(npairs, chunk);
+    DevBuf<uint32_t> tv;
+    DevBuf<uint16_t> ti, qlist;
+    DevBuf<int> dTargets, dSlot;
+    DevBuf<unsigned> dCount;
+    DevBuf<int2> dBlocks;
+    HIPCHK(tv.alloc((size_t)maxT * Nv * kTePrefix));
+    HIPCHK(ti.alloc((size_t)maxT * Nv * kTePrefix));
+    HIPCHK(qlist.alloc((size_t)npc * Nv));
+    HIPCHK(dTargets.alloc((size_t)maxT));
+    HIPCHK(dSlot.alloc((size_t)npc));
+    HIPCHK(dCount.alloc((size_t)npc));
+    HIPCHK(dBlocks.alloc((size_t)npc * nqb));
+    std::vector<int> targets, tslot;
+    std::vector<unsigned> cnt;
+    std::vector<int2> blocks;
+    for (long long p0 = 0; p0 < npairs;) {
+        targets.clear();
+        tslot.clear();
+        long long p1 = p0;
+        while (p1 < npairs && p1 - p0 < chunk) {
+            if (targets.empty() || h1[p1] != targets.back()) {
+                if ((long long)targets.size() == maxT) break;
+                targets.push_back(h1[p1]);
+            }
+            tslot.push_back((int)targets.size() - 1);
+            ++p1;
+        }
+        const long long np = p1 - p0;
+        TeArgs c = a;
+        c.a1 = a.a1 + p0;
+        c.a2 = a.a2 + p0;
+        c.partial = a.partial + p0 * nqb;
+        c.npairs = np;
+        const TePrefixArgs pa{dTargets.p, (long long)targets.size(), dSlot.p, tv.p, ti.p, dCount.p,
+                              qlist.p};
+        HIPCHK(hipMemcpyAsync(dTargets.p, targets.data(), targets.size() * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dSlot.p, tslot.data(), tslot.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(dCount.p, 0, (size_t)np * sizeof(unsigned), s));
+        HIPCHK(launch_te_prefix_build(c, pa, s));
+        HIPCHK(launch_te_prefix_select(c, pa, s));
+        cnt.resize((size_t)np);
+        HIPCHK(hipMemcpyAsync(cnt.data(), dCount.p, (size_t)np * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        blocks.clear();
+        long long left = 0;
+        for (long long p = 0; p < np; ++p) {
+            if (cnt[(size_t)p] > (unsigned)Nv)
+                return gmxcorr::set_error(GMXCORR_EHIP, "te: prefix select listed more queries than a pair has");
+            left += cnt[(size_t)p];
+            for (unsigned f = 0; f < cnt[(size_t)p]; f += 64) blocks.push_back(make_int2((int)p, (int)f));
+        }
+        if (!blocks.empty()) {
+            HIPCHK(hipMemcpyAsync(dBlocks.p, blocks.data(), blocks.size() * sizeof(int2),
+                                  hipMemcpyHostToDevice, s));
+            TeArgs l = c;
+            l.qblocks = dBlocks.p;
+            l.qcount = dCount.p;
+            l.qlist = qlist.p;
+            HIPCHK(launch_te_heap_list(l, (long long)blocks.size(), s));
+        }
+        HIPCHK(launch_te_count(c, s));
+        // the host vectors are refilled for the next chunk
+        HIPCHK(hipStreamSynchronize(s));
+        g_stats[0] += np * Nv;
+        g_stats[1] += np * Nv - left;
+        g_stats[2] += left;
+        p0 = p1;
+    }
+    g_stats[3] = 1;
+    return GMXCORR_OK;
+}
 
 // mat != nullptr: full matrix over all ordered pairs (diagonal 0);
 // otherwise te_out[p] for the given list.
@@ -57,8 +155,10 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
     } else if (Nv <= k) {
         return gmxcorr::set_error(GMXCORR_EINVAL, "te: need nframes - lag > k");
     }
-    if (g_path == 1 && k > gmxcorr::kTeHeapMaxK)
-        return gmxcorr::set_error(GMXCORR_EUNSUP, "te: the heap kernel takes k <= 128");
+    if ((g_path == 1 || g_path == 3) && k > gmxcorr::kTeHeapMaxK)
+        return gmxcorr::set_error(GMXCORR_EUNSUP, "te: the heap and prefix paths take k <= 128");
+    if (g_path == 3 && Nv > gmxcorr::kTePrefixMaxFrames)
+        return gmxcorr::set_error(GMXCORR_EUNSUP, "te: the prefix path takes nframes - lag <= 16384");
     for (size_t t = 0; t < (size_t)natoms * nframes * 3; ++t)
         if (!std::isfinite(traj[t])) return gmxcorr::set_error(GMXCORR_EINVAL, "te: non-finite coordinate");
     if (!gmxcorr::device_ok()) return gmxcorr::set_error(GMXCORR_ENODEV, "no gfx950 device");
@@ -93,6 +193,16 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
         a1 = h1.data();
         a2 = h2.data();
     }
+    // pass 1 through the prefix stores: forced, or automatic for series
+    // longer than kTePrefixAutoFrames whose pairs share their targets (the
+    // stores of a target serve all its pairs)
+    bool prefix = g_path == 3;
+    if (g_path == 0 && k <= gmxcorr::kTeHeapMaxK && Nv <= gmxcorr::kTePrefixMaxFrames &&
+        Nv > gmxcorr::kTePrefixAutoFrames) {
+        long long distinct = 0;
+        for (long long p = 0; p < npairs; ++p) distinct += p == 0 || a1[p] != a1[p - 1];
+        prefix = npairs >= (long long)gmxcorr::kTePrefixMinPairs * distinct;
+    }
     // digamma table of size Nv+1 as lib/transfer_entropy.cpp:41-48
     std::vector<double> psi(Nv + 1, 0.0);
     psi[1] = -0.57721566490153;
@@ -109,19 +219,36 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
     HIPCHK(d1.alloc(npairs));
     HIPCHK(d2.alloc(npairs));
     HIPCHK(dMat.alloc(mat ? (size_t)natoms * natoms : (size_t)npairs));
-    hipStream_t s;
-    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    // the stream and events are released on every return
+    struct Handles {
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Handles()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } h;
+    HIPCHK(hipStreamCreateWithFlags(&h.s, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&h.e0));
+    HIPCHK(hipEventCreate(&h.e1));
+    const hipStream_t s = h.s;
+    const hipEvent_t e0 = h.e0, e1 = h.e1;
     HIPCHK(hipMemcpyAsync(dTraj.p, traj, (size_t)natoms * nframes * 3 * sizeof(double),
                           hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dPsi.p, psi.data(), psi.size() * sizeof(double), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d1.p, a1, npairs * sizeof(int), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d2.p, a2, npairs * sizeof(int), hipMemcpyHostToDevice, s));
     gmxcorr::TeArgs ta{dTraj.p, nframes, lag, k, g_path, d1.p, d2.p, npairs, nqb, dPsi.p, dPart.p, dRoot.p};
+    for (auto& v : g_stats) v = 0;
     HIPCHK(hipEventRecord(e0, s));
-    HIPCHK(gmxcorr::launch_te(ta, s));
+    if (prefix) {
+        if (int rc = run_prefix(ta, a1, s)) return rc;
+    } else {
+        HIPCHK(gmxcorr::launch_te(ta, s));
+        g_stats[0] = npairs * Nv;
+    }
     HIPCHK(hipEventRecord(e1, s));
     if (mat) {
         std::vector<double> zero((size_t)natoms * natoms, 0.0);
@@ -143,9 +270,6 @@ int te_run(const double* traj, int natoms, int nframes, int k, int lag, long lon
     float t = 0.f;
     (void)hipEventElapsedTime(&t, e0, e1);
     if (ms) *ms = t;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
     return GMXCORR_OK;
 }
 
@@ -169,8 +293,16 @@ int gmxcorr_te_pairs(const double* traj, int natoms, int nframes, int k, int lag
 
 int gmxcorr_te_set_path(int path)
 {
-    if (path < 0 || path > 2) return gmxcorr::set_error(GMXCORR_EINVAL, "te_set_path: path must be 0, 1 or 2");
+    if (path < 0 || path > 3)
+        return gmxcorr::set_error(GMXCORR_EINVAL, "te_set_path: path must be 0, 1, 2 or 3");
     g_path = path;
+    return GMXCORR_OK;
+}
+
+int gmxcorr_te_stats(long long* out)
+{
+    if (!out) return gmxcorr::set_error(GMXCORR_EINVAL, "te_stats: null output");
+    for (int i = 0; i < 4; ++i) out[i] = g_stats[i];
     return GMXCORR_OK;
 }
 

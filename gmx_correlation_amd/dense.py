@@ -89,6 +89,18 @@ def te_pairs(traj, a1, a2, k: int = 100, lag: int = 1):
 
 
 def te_set_path(path: int) -> None:
-    """Pass-1 kernel of later TE calls in this process: 0 automatic (heap for
-    k <= 128, radix select above), 1 heap (k <= 128), 2 radix select."""
+    """Pass-1 kernel of later TE calls in this process: 0 automatic (prefix
+    stores for k <= 128 and 2048 < N - lag <= 16384 when the pairs average 8
+    or more per target, else the heap for k <= 128, the radix select above),
+    1 heap (k <= 128), 2 radix select, 3 prefix stores (k <= 128,
+    N - lag <= 16384)."""
     _lib.check(_lib.load().gmxcorr_te_set_path(int(path)))
+
+
+def te_stats() -> dict:
+    """Counts of the last TE call: queries, those resolved from the prefix
+    stores, those left to the heap kernel, and whether the prefix path ran."""
+    out = (ctypes.c_longlong * 4)()
+    _lib.check(_lib.load().gmxcorr_te_stats(out))
+    return {"queries": out[0], "prefix_resolved": out[1], "heap_listed": out[2],
+            "prefix_path": bool(out[3])}

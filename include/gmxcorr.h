@@ -208,10 +208,19 @@ int gmxcorr_te_matrix(const double* traj, int natoms, int nframes, int k, int la
 int gmxcorr_te_pairs(const double* traj, int natoms, int nframes, int k, int lag,
                      long long npairs, const int* a1, const int* a2, double* te, double* ms);
 /* Kernel that finds each query's k-th neighbour distance in later TE calls
- * of this process (no counterpart in the reference): 0 = automatic (a
- * per-lane heap for k <= 128, a radix select above), 1 = the heap (k <= 128
- * only), 2 = the radix select (any k).  Results are identical. */
+ * of this process (no counterpart in the reference): 0 = automatic (the
+ * prefix stores for k <= 128 and 2048 < nframes - lag <= 16384 when the
+ * pairs average 8 or more per target atom, else a per-lane heap for
+ * k <= 128, a radix select above), 1 = the heap (k <= 128 only), 2 = the
+ * radix select (any k), 3 = the prefix stores (k <= 128, nframes - lag <=
+ * 16384): per target atom and query the 1024 frames nearest in the target's
+ * own two 3-D distances, shared by the target's pairs, with the heap for
+ * the queries whose neighbour lies beyond them.  Results are identical. */
 int gmxcorr_te_set_path(int path);
+/* Counts of the last TE call (no counterpart in the reference): out[0]
+ * queries, out[1] resolved from the prefix stores, out[2] left to the heap,
+ * out[3] 1 when the prefix path ran. */
+int gmxcorr_te_stats(long long* out);
 
 /* ---- Batched uneven trajectories (no counterpart in the reference) --------
  * Many independent trajectories, each with its own atom count, frame count

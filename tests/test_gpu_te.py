@@ -198,3 +198,108 @@ def test_te_grouped_count_pass_equals_single(gpu, oracle_lib):
     assert np.array_equal(te, mat[a1, a2])
     for p in range(3):
         assert abs(te[p] - oracle_lib.te_pair(traj, int(a1[p]), int(a2[p]), k, lag)) <= 1e-12
+
+
+def _te_both(traj, k, lag, a1=None, a2=None):
+    """TE through the heap pass (path 1) and the prefix stores (path 3), and
+    the prefix run's counts."""
+    try:
+        dense.te_set_path(1)
+        heap = (dense.te_matrix(traj, k=k, lag=lag) if a1 is None
+                else dense.te_pairs(traj, a1, a2, k=k, lag=lag)[0])
+        dense.te_set_path(3)
+        pre = (dense.te_matrix(traj, k=k, lag=lag) if a1 is None
+               else dense.te_pairs(traj, a1, a2, k=k, lag=lag)[0])
+        st = dense.te_stats()
+    finally:
+        dense.te_set_path(0)
+    return heap, pre, st
+
+
+@pytest.mark.parametrize("case", ["continuous", "quantised", "duplicates"])
+def test_te_prefix_path_equals_heap_path_whole_rows(gpu, case):
+    """Up to 1024 valid frames the stores hold every frame of a query, so the
+    select resolves the root from them alone (or lists the query when the
+    root's bin is crowded); TE equals the heap pass's bit for bit."""
+    rng = np.random.default_rng(31)
+    if case == "duplicates":
+        c = np.zeros((300, 5, 3), np.float32)
+        c[:, :, 0] = rng.integers(0, 2, (300, 5))
+        c[:, 4, :] += rng.standard_normal((300, 3)).astype(np.float32)
+    else:
+        c = synth.generate(synth.TrajSpec(natoms=5, nframes=700, nmodes=3, seed=12))
+        if case == "quantised":
+            c = np.round(c / 0.02).astype(np.float32) * np.float32(0.02)
+    traj = ksg.center(c)
+    for k, lag in ((1, 1), (5, 2), (100, 1), (128, 4)):
+        heap, pre, st = _te_both(traj, k, lag)
+        assert st["prefix_path"] and st["queries"] == 20 * (traj.shape[1] - lag)
+        assert np.array_equal(heap, pre), (case, k, lag)
+
+
+def test_te_prefix_path_beyond_the_stores(gpu, oracle_lib):
+    """3000 frames: the stores hold the 1024 frames nearest in the target's
+    own distances.  On a coupled trajectory most queries resolve there and
+    the rest go to the heap kernel; with a target atom that barely moves
+    (its distances tiny beside the source's) the neighbour lies anywhere in
+    that order and most queries are listed.  Both equal the heap pass bit
+    for bit, and the FP64 oracle on sampled pairs."""
+    traj = ksg.center(synth.generate(synth.TrajSpec(natoms=4, nframes=3000, nmodes=3, seed=41)))
+    heap, pre, st = _te_both(traj, 100, 1)
+    assert np.array_equal(heap, pre)
+    assert st["prefix_resolved"] > st["heap_listed"] > 0, st
+    assert abs(pre[1, 2] - oracle_lib.te_pair(traj, 1, 2, 100, 1)) <= 1e-12
+    rng = np.random.default_rng(43)
+    c = rng.standard_normal((3000, 3, 3)).astype(np.float32)
+    c[:, 0, :] *= np.float32(1e-3)
+    traj = ksg.center(c)
+    a1 = np.array([0, 0, 1], np.int32)
+    a2 = np.array([1, 2, 0], np.int32)
+    heap, pre, st = _te_both(traj, 20, 3, a1, a2)
+    assert np.array_equal(heap, pre)
+    assert st["heap_listed"] > st["queries"] // 2, st
+    for p in range(3):
+        ref = oracle_lib.te_pair(traj, int(a1[p]), int(a2[p]), 20, 3)
+        assert abs(pre[p] - ref) <= 1e-12, p
+
+
+def test_te_prefix_path_chosen_automatically(gpu, oracle_lib):
+    """The automatic path takes the stores for more than 2048 valid frames
+    when the pairs average at least 8 per target atom (10 atoms: 9), not
+    for a list of one pair per target, nor at 2048 valid frames."""
+    traj = ksg.center(synth.generate(synth.TrajSpec(natoms=10, nframes=2100, nmodes=4, seed=45)))
+    mat = dense.te_matrix(traj, k=50, lag=2)
+    assert dense.te_stats()["prefix_path"]
+    try:
+        dense.te_set_path(1)
+        assert np.array_equal(mat, dense.te_matrix(traj, k=50, lag=2))
+    finally:
+        dense.te_set_path(0)
+    dense.te_pairs(traj, [0, 1], [1, 0], k=50, lag=2)
+    assert not dense.te_stats()["prefix_path"]
+    dense.te_matrix(traj[:, :2050], k=50, lag=2)
+    assert not dense.te_stats()["prefix_path"]
+    assert abs(mat[3, 7] - oracle_lib.te_pair(traj, 3, 7, 50, 2)) <= 1e-12
+
+
+def test_te_prefix_path_over_two_launch_chunks(gpu):
+    """2100 atoms x 100 frames: 4.4 M ordered pairs in two launch chunks, the
+    boundary inside one target's pairs (its stores built in both chunks);
+    the prefix path equals the heap pass on pairs on both sides of it."""
+    traj = ksg.center(synth.generate(synth.TrajSpec(natoms=2100, nframes=100, nmodes=4, seed=47)))
+    rng = np.random.default_rng(49)
+    a1 = rng.choice(2100, 300, replace=False).astype(np.int32)
+    # target 1998's pairs straddle the chunk boundary of the full matrix
+    # (pair 4,194,304 of 2100 x 2099, launched target-major)
+    a1 = np.concatenate([a1, np.full(2099, 1998, np.int32)])
+    a2 = np.array([(t + 1 + rng.integers(0, 2098)) % 2100 for t in a1[:300]] +
+                  [s for s in range(2100) if s != 1998], np.int32)
+    heap, pre, st = _te_both(traj, 10, 1, a1, a2)
+    assert np.array_equal(heap, pre)
+    full = None
+    try:
+        dense.te_set_path(3)
+        full = dense.te_matrix(traj, k=10, lag=1)
+    finally:
+        dense.te_set_path(0)
+    assert np.array_equal(full[a1, a2], heap)

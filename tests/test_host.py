@@ -216,17 +216,22 @@ def test_plan_layout_memory_sizing_without_gpu(hiplib):
 
 
 def test_te_argument_errors_before_the_device(hiplib):
-    """The TE entry points reject a bad kernel choice, k above the heap
-    kernel's 128 when the heap is forced, and non-finite coordinates before
-    they look for a device (so these hold without one)."""
+    """The TE entry points reject a bad kernel choice, k above 128 when the
+    heap or the prefix stores are forced, more than 16384 valid frames for
+    the prefix stores, and non-finite coordinates before they look for a
+    device (so these hold without one)."""
     from gmx_correlation_amd import dense
     traj = ksg.center(synth.generate(synth.TrajSpec(natoms=2, nframes=300, nmodes=2, seed=1)))
     with pytest.raises(_lib.GmxcorrError):
-        dense.te_set_path(3)
-    dense.te_set_path(1)
+        dense.te_set_path(4)
     try:
-        with pytest.raises(_lib.GmxcorrError, match="128"):
-            dense.te_matrix(traj, k=129, lag=1)
+        for path in (1, 3):
+            dense.te_set_path(path)
+            with pytest.raises(_lib.GmxcorrError, match="128"):
+                dense.te_matrix(traj, k=129, lag=1)
+        long = np.zeros((2, 16386, 3))
+        with pytest.raises(_lib.GmxcorrError, match="16384"):
+            dense.te_matrix(long, k=5, lag=1)
     finally:
         dense.te_set_path(0)
     bad = traj.copy()
