@@ -57,10 +57,25 @@ int run_prefix(const gmxcorr::TeArgs& a, const int* h1, hipStream_t s)
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     const size_t budget = std::min(free_b / 2, (size_t)32 << 30);
-    long long distinct = 0;
-    for (long long p = 0; p < npairs; ++p) distinct += p == 0 || h1[p] != h1[p - 1];
-    long long maxT = std::min((long long)(budget / per_target), distinct);
+    // chunks of at most `chunk` pairs and `maxT` distinct targets (pairs
+    // are target-major); the stores are sized for the largest chunk's targets
+    long long maxT = (long long)(budget / per_target);
     if (maxT < 1) return gmxcorr::set_error(GMXCORR_ENOMEM, "te: no room for one target's prefix stores");
+    auto chunk_end = [&](long long p0, long long& ntargets) {
+        long long p1 = p0;
+        ntargets = 0;
+        while (p1 < npairs && p1 - p0 < chunk) {
+            if (p1 == p0 || h1[p1] != h1[p1 - 1]) {
+                if (ntargets == maxT) break;
+                ++ntargets;
+            }
+            ++p1;
+        }
+        return p1;
+    };
+    long long most = 0;
+    for (long long p0 = 0, nt = 0; p0 < npairs; most = std::max(most, nt)) p0 = chunk_end(p0, nt);
+    maxT = most;
     const long long npc = std::min(npairs, chunk);
     DevBuf<uint32_t> tv;
     DevBuf<uint16_t> ti, qlist;
@@ -78,16 +93,13 @@ int run_prefix(const gmxcorr::TeArgs& a, const int* h1, hipStream_t s)
     std::vector<unsigned> cnt;
     std::vector<int2> blocks;
     for (long long p0 = 0; p0 < npairs;) {
+        long long nt = 0;
+        const long long p1 = chunk_end(p0, nt);
         targets.clear();
         tslot.clear();
-        long long p1 = p0;
-        while (p1 < npairs && p1 - p0 < chunk) {
-            if (targets.empty() || h1[p1] != targets.back()) {
-                if ((long long)targets.size() == maxT) break;
-                targets.push_back(h1[p1]);
-            }
+        for (long long p = p0; p < p1; ++p) {
+            if (p == p0 || h1[p] != h1[p - 1]) targets.push_back(h1[p]);
             tslot.push_back((int)targets.size() - 1);
-            ++p1;
         }
         const long long np = p1 - p0;
         TeArgs c = a;
