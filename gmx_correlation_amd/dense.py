@@ -90,7 +90,7 @@ def te_pairs(traj, a1, a2, k: int = 100, lag: int = 1):
 
 def te_set_path(path: int) -> None:
     """Pass-1 kernel of later TE calls in this process: 0 automatic (prefix
-    stores for k <= 128 and 2048 < N - lag <= 16384 when the pairs average 8
+    stores for k <= 128 and 256 < N - lag <= 16384 when the pairs average 8
     or more per target, else the heap for k <= 128, the radix select above),
     1 heap (k <= 128), 2 radix select, 3 prefix stores (k <= 128,
     N - lag <= 16384)."""

@@ -209,7 +209,7 @@ int gmxcorr_te_pairs(const double* traj, int natoms, int nframes, int k, int lag
                      long long npairs, const int* a1, const int* a2, double* te, double* ms);
 /* Kernel that finds each query's k-th neighbour distance in later TE calls
  * of this process (no counterpart in the reference): 0 = automatic (the
- * prefix stores for k <= 128 and 2048 < nframes - lag <= 16384 when the
+ * prefix stores for k <= 128 and 256 < nframes - lag <= 16384 when the
  * pairs average 8 or more per target atom, else a per-lane heap for
  * k <= 128, a radix select above), 1 = the heap (k <= 128 only), 2 = the
  * radix select (any k), 3 = the prefix stores (k <= 128, nframes - lag <=

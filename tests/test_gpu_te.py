@@ -264,9 +264,9 @@ def test_te_prefix_path_beyond_the_stores(gpu, oracle_lib):
 
 
 def test_te_prefix_path_chosen_automatically(gpu, oracle_lib):
-    """The automatic path takes the stores for more than 2048 valid frames
+    """The automatic path takes the stores for more than 256 valid frames
     when the pairs average at least 8 per target atom (10 atoms: 9), not
-    for a list of one pair per target, nor at 2048 valid frames."""
+    for a list of one pair per target, nor at 256 valid frames."""
     traj = ksg.center(synth.generate(synth.TrajSpec(natoms=10, nframes=2100, nmodes=4, seed=45)))
     mat = dense.te_matrix(traj, k=50, lag=2)
     assert dense.te_stats()["prefix_path"]
@@ -277,8 +277,10 @@ def test_te_prefix_path_chosen_automatically(gpu, oracle_lib):
         dense.te_set_path(0)
     dense.te_pairs(traj, [0, 1], [1, 0], k=50, lag=2)
     assert not dense.te_stats()["prefix_path"]
-    dense.te_matrix(traj[:, :2050], k=50, lag=2)
+    dense.te_matrix(traj[:, :258], k=50, lag=2)
     assert not dense.te_stats()["prefix_path"]
+    dense.te_matrix(traj[:, :259], k=50, lag=2)
+    assert dense.te_stats()["prefix_path"]
     assert abs(mat[3, 7] - oracle_lib.te_pair(traj, 3, 7, 50, 2)) <= 1e-12
 
 
