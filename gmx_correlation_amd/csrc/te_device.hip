@@ -373,11 +373,94 @@ __global__ __launch_bounds__(64) void te_heap_kernel(TeArgs args)
         cap = INFINITY;
     }
     // the root of query q of pair p at root[(p * nqb) * 64 + q]
-    if (listed) {
-        if (active) args.root[(size_t)p * nqb * 64 + q] = r[0];
-    } else {
+    if (!listed) {
         args.root[(size_t)blockIdx.x * 64 + lane] = r[0];
+        return;
     }
+    if (!args.eps64) {
+        if (active) args.root[(size_t)p * nqb * 64 + q] = r[0];
+        return;
+    }
+    // The prefix path's listed queries: the exact FP64 radius, the r-th
+    // smallest FP64 distance among the frames whose key is the root, r =
+    // k + 1 - #{key < root} (fl32 is monotone: every frame below the root key
+    // is nearer).  A second pass over the frames (the same LDS stage) counts
+    // the keys below the root and keeps the distances of the first kRec
+    // frames at it; a lane with more goes through their distinct values.
+    constexpr int kRec = 4;
+    const float rt = r[0];
+    int clt = 0, nrec = 0;
+    double rec[kRec];
+#pragma unroll
+    for (int u = 0; u < kRec; ++u) rec[u] = INFINITY;
+    fetch(0);
+    for (int c0 = 0; c0 < Nv; c0 += kTeStage) {
+        lds_order();
+        put();
+        lds_order();
+        if (c0 + kTeStage < Nv) fetch(c0 + kTeStage);   // uniform
+        const int nf = min(kTeStage, Nv - c0);
+        for (int f = 0; f < nf; f += 2) {
+            double c0v[9], c1v[9];
+            frames(f, c0v, c1v);
+            double a0, b0, e0, a1, b1, e1;
+            dist9(c0v, a0, b0, e0);
+            dist9(c1v, a1, b1, e1);
+            const double d0 = fmax(fmax(a0, b0), e0), d1 = fmax(fmax(a1, b1), e1);
+            const float key0 = (float)d0, key1 = f + 1 < nf ? (float)d1 : INFINITY;
+            clt += (key0 < rt) + (key1 < rt);
+            if (__ballot(key0 == rt || key1 == rt)) {   // uniform, rare
+#pragma unroll
+                for (int u = 0; u < kRec; ++u) {
+                    rec[u] = key0 == rt && nrec == u ? d0 : rec[u];
+                    rec[u] = key1 == rt && nrec + (key0 == rt) == u ? d1 : rec[u];
+                }
+                nrec += (key0 == rt) + (key1 == rt);
+            }
+        }
+    }
+    if (!active) return;
+    const int rk = k + 1 - clt;   // 1 <= rk <= nrec
+    double eps = INFINITY;
+    if (nrec <= kRec) {
+        // the rk-th smallest of the recorded distances
+        double v[kRec];
+#pragma unroll
+        for (int u = 0; u < kRec; ++u) v[u] = rec[u];
+#pragma unroll
+        for (int a = 0; a < kRec; ++a)
+#pragma unroll
+            for (int b = a + 1; b < kRec; ++b) {
+                const double lo = fmin(v[a], v[b]), hi = fmax(v[a], v[b]);
+                v[a] = lo;
+                v[b] = hi;
+            }
+#pragma unroll
+        for (int u = 0; u < kRec; ++u) eps = u == rk - 1 ? v[u] : eps;
+    } else {
+        // the distinct FP64 values of the root-key frames upwards
+        double prev = -1.0;
+        int cum = 0;
+        for (;;) {
+            double vmin = INFINITY;
+            int m = 0;
+            for (int j = 0; j < Nv; ++j) {
+                double c[9];
+                load_frame(X, Y, j, lag, c);
+                double a, b, e;
+                dist9(c, a, b, e);
+                const double dd = fmax(fmax(a, b), e);
+                if ((float)dd == rt && dd > prev) {
+                    if (dd < vmin) { vmin = dd; m = 1; }
+                    else if (dd == vmin) ++m;
+                }
+            }
+            if (m == 0 || cum + m >= rk) { eps = vmin; break; }
+            cum += m;
+            prev = vmin;
+        }
+    }
+    args.eps64[(size_t)p * nqb * 64 + q] = eps;
 }
 
 // Pass 1 for any k (the heap kernel's LDS stops at k = 128): per query, the
@@ -1024,19 +1107,209 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void t
 #pragma unroll
         for (int m = 0; m < kA; ++m) ka[m] = key[m];
         bool ok = prefix_select<kA>(ka, L, HA, T, hist, lane, eps_key);
+        int nk = kA;   // key arrays joined
         if (!ok && Nv > 64 * kA) {   // uniform
             join(kA, kM);
             const uint32_t HB = Nv <= kTePrefix ? above_all(kM, Nv) : storeB;
             ok = prefix_select<kM>(key, L, HB, T, hist, lane, eps_key);
+            nk = kM;
+        }
+        // The exact radius: the r-th smallest FP64 joint distance among the
+        // entries whose key is the root, r = T - #{key < root} (fl32 is
+        // monotone, so every entry below the root key is nearer).  All of
+        // them lie in the joined positions (their keys are below the bound).
+        double eps = 0.0;
+        if (ok) {
+            int clt = 0, ctie = 0;
+#pragma unroll
+            for (int m = 0; m < kM; ++m) {
+                if (m >= nk) continue;
+                clt += __popcll(__ballot(key[m] < eps_key));
+                ctie += __popcll(__ballot(key[m] == eps_key));
+            }
+            int r = T - clt;
+            ok = ctie <= 64 && r >= 1 && r <= ctie;
+            if (ok) {
+                const double* X = args.traj + (size_t)args.a1[p] * N * 3;
+                double qc[6];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    qc[d] = X[(size_t)(q + lag) * 3 + d];
+                    qc[3 + d] = X[(size_t)q * 3 + d];
+                }
+                double* tied = reinterpret_cast<double*>(hist);   // up to 64
+                int n = 0;
+#pragma unroll
+                for (int m = 0; m < kM; ++m) {
+                    if (m >= nk) continue;
+                    const bool t = key[m] == eps_key;
+                    const uint64_t tm = __ballot(t);
+                    if (tm) {
+                        if (t) {
+                            const size_t j = pa.ti[rbase + lane + 64 * m];
+                            const double* f = X + (j + lag) * 3;
+                            const double* c = X + j * 3;
+                            const double* yj = Y + j * 3;
+                            double d = 0.0;
+#pragma unroll
+                            for (int e = 0; e < 3; ++e) {
+                                d = fmax(d, fabs(qc[e] - f[e]));
+                                d = fmax(d, fabs(qc[3 + e] - c[e]));
+                            }
+                            d = fmax(d, fmax(fmax(fabs(y0 - yj[0]), fabs(y1 - yj[1])), fabs(y2 - yj[2])));
+                            tied[n + mbcnt(tm)] = d;
+                        }
+                        n += __popcll(tm);
+                    }
+                }
+                lds_order();
+                double v = lane < n ? tied[lane] : INFINITY;
+                lds_order();
+                // the r-th smallest, ties counted
+                for (;;) {
+                    double mn = v;
+#pragma unroll
+                    for (int o = 32; o >= 1; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o, 64));
+                    const int c = __popcll(__ballot(v == mn));
+                    if (r <= c) {
+                        eps = mn;
+                        break;
+                    }
+                    r -= c;
+                    v = v == mn ? INFINITY : v;
+                }
+            }
         }
         if (lane == 0) {
             if (ok) {
-                args.root[(size_t)p * nqb * 64 + q] = __uint_as_float(eps_key);
+                args.eps64[(size_t)p * nqb * 64 + q] = eps;
             } else {
+                args.eps64[(size_t)p * nqb * 64 + q] = __builtin_nan("");
                 const unsigned s = atomicAdd(&pa.qcount[p], 1u);
                 pa.qlist[(size_t)p * Nv + s] = (uint16_t)q;
             }
         }
+    }
+}
+
+// Counts of the prefix path from exact FP64 radii: G ordered pairs of one
+// target atom, lane = query q of block qb; the strict counts
+// #{j != q: max(dXf, dXp) < eps}, #{max(dYp, dXp) < eps} and #{dXp < eps}
+// in FP64, the comparisons of lib/transfer_entropy.cpp:170-195 (the own
+// frame, all distances 0, is removed after the loop).  No FP32 keys and no
+// tie handling: eps is the exact radius, from the select kernel or, for the
+// queries it listed, the heap kernel's list mode.  Wave sums in the order of
+// te_count_group.
+template <int G>
+__device__ __forceinline__ void te_count64_group(const TeArgs& args, long long p0, int q,
+                                                 bool active, int qb, int lane)
+{
+    const int N = args.N, lag = args.lag, k = args.k, Nv = N - lag;
+    const int nqb = args.nqblocks;
+    const double* X = args.traj + (size_t)args.a1[p0] * N * 3;
+    const double* Y[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) Y[u] = args.traj + (size_t)args.a2[p0 + u] * N * 3;
+    const int qs = active ? q : 0;
+    double qx[6], qy[G][3], eps[G];
+    int cx[G], cy[G], cp[G];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        qx[d] = X[(size_t)(qs + lag) * 3 + d];
+        qx[3 + d] = X[(size_t)qs * 3 + d];
+#pragma unroll
+        for (int u = 0; u < G; ++u) qy[u][d] = Y[u][(size_t)qs * 3 + d];
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        eps[u] = args.eps64[(size_t)(p0 + u) * nqb * 64 + qs];
+        cx[u] = cy[u] = cp[u] = 0;
+    }
+    // The coordinates are read-only for the whole kernel: read through the
+    // constant address space, the frame's loads (uniform over the wave) are
+    // scalar loads
+    typedef const __attribute__((address_space(4))) char* cchar;
+    auto at = [](const double* base, uint32_t off) {
+        return reinterpret_cast<const __attribute__((address_space(4))) double*>(
+            (cchar)(reinterpret_cast<const char*>(base)) + off);
+    };
+    // frame j: a1 at j + lag and j, each source at j (32-bit byte offsets;
+    // the next frame is loaded while this one is processed)
+    auto load = [&](int j, double* f, double (*y)[3]) {
+        const uint32_t of = (uint32_t)(j + lag) * 24u, op = (uint32_t)j * 24u;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            f[d] = at(X, of)[d];
+            f[3 + d] = at(X, op)[d];
+#pragma unroll
+            for (int u = 0; u < G; ++u) y[u][d] = at(Y[u], op)[d];
+        }
+    };
+    auto maxnorm3 = [](const double* a, const double* b) {
+        return fmax(fmax(fabs(a[0] - b[0]), fabs(a[1] - b[1])), fabs(a[2] - b[2]));
+    };
+    double nf[6], ny[G][3];
+    load(0, nf, ny);
+    for (int j = 0; j < Nv; ++j) {
+        double f[6], y[G][3];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) f[d] = nf[d];
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) y[u][d] = ny[u][d];
+        load(min(j + 1, Nv - 1), nf, ny);
+        const double dxp = maxnorm3(qx + 3, f + 3);
+        const double m1 = fmax(maxnorm3(qx, f), dxp);
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const double m2 = fmax(maxnorm3(qy[u], y[u]), dxp);
+            cx[u] += m1 < eps[u];
+            cy[u] += m2 < eps[u];
+            cp[u] += dxp < eps[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        // the own frame counted in all three below a positive radius
+        const int own = eps[u] > 0.0 ? 1 : 0;
+        double te = 0.0;
+        if (active) {
+            const double* psi = args.psi;
+            te = psi[k] + psi[min(cp[u] - own, Nv) + 1] - psi[min(cx[u] - own, Nv) + 1] -
+                 psi[min(cy[u] - own, Nv) + 1];
+        }
+        for (int o = 32; o >= 1; o >>= 1) te += __shfl_xor(te, o, 64);
+        if (lane == 0) args.partial[(size_t)(p0 + u) * nqb + qb] = te;
+    }
+}
+
+// Wave w of block b runs item 4b + w as te_count_kernel does.
+__global__ __launch_bounds__(256) void te_count64_kernel(TeArgs args)
+{
+    constexpr int G = kTeCountGroup;
+    const int lane = threadIdx.x & 63;
+    const long long item = (long long)blockIdx.x * 4 +
+                           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nqb = args.nqblocks;
+    const long long ngroups = (args.npairs + G - 1) / G;
+    if (item >= ngroups * nqb) return;
+    const long long g = item / nqb;
+    const int qb = (int)(item - g * nqb);
+    const long long p0 = g * G;
+    bool shared = p0 + G <= args.npairs;
+    if (shared) {
+        const int t = args.a1[p0];
+#pragma unroll
+        for (int u = 1; u < G; ++u) shared = shared && args.a1[p0 + u] == t;
+    }
+    const int q = qb * 64 + lane;
+    const bool active = q < args.N - args.lag;
+    if (shared) {
+        te_count64_group<G>(args, p0, q, active, qb, lane);
+    } else {
+        const long long pe = p0 + G < args.npairs ? p0 + G : args.npairs;
+        for (long long p = p0; p < pe; ++p) te_count64_group<1>(args, p, q, active, qb, lane);
     }
 }
 
@@ -1078,6 +1351,14 @@ hipError_t launch_te(const TeArgs& a, hipStream_t s)
         const hipError_t e = launch_te_count(c, s);
         if (e != hipSuccess) return e;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_te_count64(const TeArgs& c, hipStream_t s)
+{
+    if (!c.eps64 || c.qblocks) return hipErrorInvalidValue;
+    const long long citems = (c.npairs + kTeCountGroup - 1) / kTeCountGroup * c.nqblocks;
+    hipLaunchKernelGGL(te_count64_kernel, dim3((unsigned)((citems + 3) / 4)), dim3(256), 0, s, c);
     return hipGetLastError();
 }
 

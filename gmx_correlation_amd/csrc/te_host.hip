@@ -44,8 +44,9 @@ long long g_stats[4] = {0, 0, 0, 0};
 
 // The prefix path over pairs a.a1 / a.a2 (target-major; h1 is the host copy
 // of a.a1): per launch chunk of at most te_chunk_pairs pairs and maxT
-// distinct target atoms, the targets' prefix stores, the select, the heap
-// kernel over the listed queries, the count kernel.
+// distinct target atoms, the targets' prefix stores, the select (exact FP64
+// radii), the heap kernel for the queries it lists (their roots and exact
+// radii), and the count pass in FP64 over every query.
 int run_prefix(const gmxcorr::TeArgs& a, const int* h1, hipStream_t s)
 {
     using namespace gmxcorr;
@@ -82,6 +83,8 @@ int run_prefix(const gmxcorr::TeArgs& a, const int* h1, hipStream_t s)
     DevBuf<int> dTargets, dSlot;
     DevBuf<unsigned> dCount;
     DevBuf<int2> dBlocks;
+    DevBuf<double> dEps;
+    HIPCHK(dEps.alloc((size_t)npc * nqb * 64));
     HIPCHK(tv.alloc((size_t)maxT * Nv * kTePrefix));
     HIPCHK(ti.alloc((size_t)maxT * Nv * kTePrefix));
     HIPCHK(qlist.alloc((size_t)npc * Nv));
@@ -107,6 +110,7 @@ int run_prefix(const gmxcorr::TeArgs& a, const int* h1, hipStream_t s)
         c.a2 = a.a2 + p0;
         c.partial = a.partial + p0 * nqb;
         c.npairs = np;
+        c.eps64 = dEps.p;
         const TePrefixArgs pa{dTargets.p, (long long)targets.size(), dSlot.p, tv.p, ti.p, dCount.p,
                               qlist.p};
         HIPCHK(hipMemcpyAsync(dTargets.p, targets.data(), targets.size() * sizeof(int),
@@ -133,9 +137,11 @@ int run_prefix(const gmxcorr::TeArgs& a, const int* h1, hipStream_t s)
             l.qblocks = dBlocks.p;
             l.qcount = dCount.p;
             l.qlist = qlist.p;
+            // the listed queries' roots and exact radii
             HIPCHK(launch_te_heap_list(l, (long long)blocks.size(), s));
         }
-        HIPCHK(launch_te_count(c, s));
+        // the counts from the exact FP64 radii
+        HIPCHK(launch_te_count64(c, s));
         // the host vectors are refilled for the next chunk
         HIPCHK(hipStreamSynchronize(s));
         g_stats[0] += np * Nv;

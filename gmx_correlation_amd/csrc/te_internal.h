@@ -26,6 +26,10 @@ struct TeArgs {
     const int2* qblocks = nullptr;
     const unsigned* qcount = nullptr;
     const uint16_t* qlist = nullptr;
+    // Prefix path: the exact FP64 radius of each query ([chunk pairs][nqblocks
+    // * 64]), from the select kernel or, for the queries it lists, the heap
+    // kernel's list mode
+    double* eps64 = nullptr;
 };
 
 // Prefix path of pass 1 (k <= kTeHeapMaxK, N - lag <= kTePrefixMaxFrames).
@@ -93,6 +97,8 @@ hipError_t launch_te_prefix_build(const TeArgs& a, const TePrefixArgs& pa, hipSt
 hipError_t launch_te_prefix_select(const TeArgs& a, const TePrefixArgs& pa, hipStream_t s);
 hipError_t launch_te_heap_list(const TeArgs& a, long long nblocks, hipStream_t s);
 hipError_t launch_te_count(const TeArgs& a, hipStream_t s);
+// The counts of the prefix path from the exact radii a.eps64.
+hipError_t launch_te_count64(const TeArgs& a, hipStream_t s);
 hipError_t launch_te_finalize(const TeArgs& a, double* mat, int natoms, hipStream_t s);
 
 }  // namespace gmxcorr
