@@ -349,8 +349,8 @@ __device__ __forceinline__ void clear_hist(uint32_t* hist, int lane)
 // computed in FP64 exactly as the reference does (fabs of the difference,
 // lib/kraskov.cpp:131-132, 280-281), then rounded once.  j in [N, P) = 3072.
 // One block per row (a, i); used where no sorted rows are kept (N >
-// kSortedMaxFrames).  With sorted rows, ksg_build_sorted_kernel below writes
-// the same D and also
+// kSortedMaxFrames).  With sorted rows, ksg_build_sorted_wave_kernel below
+// writes the same D and also
 //   SD[a][i][0..64)        0 (front pad)
 //   SD[a][i][64 + t]       t-th smallest value, t < N; 3072 up to the pitch
 //   SM[a][i][m]            SD value t = 64m + 63 (3072 past N), m < 64
@@ -406,49 +406,45 @@ __global__ __launch_bounds__(256) void ksg_build_rows_kernel(const double* __res
 // min/max work; only the prefix needs its frames.
 // ---------------------------------------------------------------------------
 
-// Exclusive prefix sum over the block's 256 threads (4 waves); total out.
-__device__ __forceinline__ int block_excl_scan(int v, int* wtot, int t, int& total)
+// ---------------------------------------------------------------------------
+// Sorted rows (N <= kSortedMaxFrames): D as above and SD, SM, SI, one wave
+// per row (a, i): lane l holds frames l + 64 e (e < EW, coalesced loads and
+// distance-row stores), N <= 64 EW.  The 64 EW keys are sorted by the register network with the
+// whole row inside the wave, so every pass is in registers or a lane
+// exchange (no LDS passes, no block barriers); afterwards lane l holds sorted
+// positions EW l .. EW l + EW - 1.  The kPrefix smallest (value, frame) keys
+// are the frames below the value T at position kPrefix - 1 and the first
+// frames at T in frame order (e-major, lane-minor, counted by ballots),
+// compacted to the wave's LDS and sorted as 8 keys per lane.
+// ---------------------------------------------------------------------------
+template <int EW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ksg_build_sorted_wave_kernel(const double* __restrict__ X,
+                                                                    int N, long long P,
+                                                                    long long nrows,
+                                                                    float* __restrict__ D,
+                                                                    float* __restrict__ SD,
+                                                                    float* __restrict__ SM,
+                                                                    uint16_t* __restrict__ SI)
 {
-    const int lane = t & (kWave - 1), w = t / kWave;
-    const int incl = wave_incl_scan(v);
-    if (lane == kWave - 1) wtot[w] = incl;
-    __syncthreads();
-    int off = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) off += q < w ? wtot[q] : 0;
-    total = (wtot[0] + wtot[1]) + (wtot[2] + wtot[3]);
-    __syncthreads();   // wtot is reused by the next scan
-    return off + incl - v;
-}
-
-template <int E>
-__global__ __launch_bounds__(256) void ksg_build_sorted_kernel(const double* __restrict__ X,
-                                                               int N, long long P,
-                                                               long long nrows,
-                                                               float* __restrict__ D,
-                                                               float* __restrict__ SD,
-                                                               float* __restrict__ SM,
-                                                               uint16_t* __restrict__ SI)
-{
-    constexpr int M = 256 * E;
-    __shared__ uint32_t lds[M];
-    __shared__ uint64_t sp[kPrefix];
-    __shared__ int wtot[4];
-    __shared__ uint32_t s_thr;
-    const int t = threadIdx.x;
-    const int base = E * t;
+    constexpr int M = 64 * EW;
+    static_assert(EW % 4 == 0 && kPrefix == 8 * kWave, "whole 16-byte groups; 8 prefix keys per lane");
+    __shared__ uint64_t sp_all[4][kPrefix];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint64_t* sp = sp_all[w];
     const long long ps = sorted_pitch(N);
     const int nP = N < kPrefix ? N : kPrefix;
-    for (long long row = blockIdx.x; row < nrows; row += gridDim.x) {   // uniform
+    for (long long row = (long long)blockIdx.x * 4 + w; row < nrows;
+         row += (long long)gridDim.x * 4) {   // uniform over the wave
         const int a = (int)(row / N), i = (int)(row % N);
         const double* x0 = X + (size_t)(3 * a) * N;
         const double* x1 = x0 + N;
         const double* x2 = x1 + N;
         const double q0 = x0[i], q1 = x1[i], q2 = x2[i];
-        uint32_t raw[E], r[E];
+        uint32_t raw[EW], r[EW];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int j = base + e;
+        for (int e = 0; e < EW; ++e) {
+            const int j = lane + kWave * e;
             uint32_t v = kPadBits;
             if (j < N) {
                 double d = fabs(q0 - x0[j]);
@@ -461,77 +457,67 @@ __global__ __launch_bounds__(256) void ksg_build_sorted_kernel(const double* __r
         }
         // distance row, pitch P <= M
         uint32_t* drow = reinterpret_cast<uint32_t*>(D + row * P);
-        if constexpr (E >= 4) {
 #pragma unroll
-            for (int e = 0; e < E; e += 4)
-                if (base + e < P)
-                    *reinterpret_cast<uint4*>(drow + base + e) =
-                        make_uint4(raw[e], raw[e + 1], raw[e + 2], raw[e + 3]);
-        } else {
-#pragma unroll
-            for (int e = 0; e < E; ++e)
-                if (base + e < P) drow[base + e] = raw[e];
-        }
-        sortnet::sort_blocked<uint32_t, E, 256>(r, lds, t);
+        for (int e = 0; e < EW; ++e)
+            if (lane + kWave * e < P) drow[lane + kWave * e] = raw[e];
+        sortnet::sort_blocked<uint32_t, EW, kWave>(r, reinterpret_cast<uint32_t*>(sp), lane);
         // sorted row: 64 zeros, the N values, padding to the pitch
         uint32_t* srow = reinterpret_cast<uint32_t*>(SD) + row * ps;
-        if (t < 64) srow[t] = 0u;
+        srow[lane] = 0u;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int q = base + e;
-            if (64 + q < ps) srow[64 + q] = q < N ? r[e] : kPadBits;
+        for (int e = 0; e < EW; e += 4) {
+            const int q = EW * lane + e;   // 64 + q < ps covers q .. q + 3 (ps % 64 == 0)
+            if (64 + q < ps)
+                *reinterpret_cast<uint4*>(srow + 64 + q) =
+                    make_uint4(q < N ? r[e] : kPadBits, q + 1 < N ? r[e + 1] : kPadBits,
+                               q + 2 < N ? r[e + 2] : kPadBits, q + 3 < N ? r[e + 3] : kPadBits);
         }
-        for (long long q = 64 + M + t; q < ps; q += 256) srow[q] = kPadBits;
-        // summary: value 64m + 63 (owned by the thread whose last slot it is)
+        for (long long q = 64 + M + lane; q < ps; q += kWave) srow[q] = kPadBits;
+        // summary: sorted value 64m + 63, the last element of lane l when
+        // EW l + EW - 1 = 64m + 63
         uint32_t* sm = reinterpret_cast<uint32_t*>(SM) + row * kSumPitch;
-        if ((base + E - 1) % 64 == 63) {
-            const int q = base + E - 1;
-            sm[q / 64] = q < N ? r[E - 1] : kPadBits;
+        if ((EW * lane + EW - 1) % 64 == 63) {
+            const int q = EW * lane + EW - 1;
+            sm[q / 64] = q < N ? r[EW - 1] : kPadBits;
         }
-        for (int m = M / 64 + t; m < kSumPitch; m += 256) sm[m] = kPadBits;
-        // the kPrefix-th smallest value T
-        if (base <= nP - 1 && nP - 1 < base + E) {
+        for (int m = M / 64 + lane; m < kSumPitch; m += kWave) sm[m] = kPadBits;
+        // the kPrefix-th smallest value T (sorted position nP - 1)
+        const int pT = nP - 1;
+        uint32_t tv = 0u;
 #pragma unroll
-            for (int e = 0; e < E; ++e)
-                if (base + e == nP - 1) s_thr = r[e];
-        }
-        __syncthreads();
-        const uint32_t T = s_thr;
+        for (int e = 0; e < EW; ++e) tv = e == pT % EW ? r[e] : tv;
+        const uint32_t T = (uint32_t)readlane((int)tv, pT / EW);
         // frames below T, and the first frames at T in frame order
-        int cl = 0, ce = 0;
+        int LT = 0;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const bool real = base + e < N;
-            cl += real && raw[e] < T;
-            ce += real && raw[e] == T;
-        }
-        int LT, EQ;
-        int pl = block_excl_scan(cl, wtot, t, LT);
-        int pe = block_excl_scan(ce, wtot, t, EQ);
+        for (int e = 0; e < EW; ++e) LT += __popcll(__ballot(lane + kWave * e < N && raw[e] < T));
         const int need_eq = nP - LT;
+        int base = 0, eqseen = 0;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int j = base + e;
-            if (j >= N) continue;
-            const uint64_t key = ((uint64_t)raw[e] << 32) | (uint32_t)j;
-            if (raw[e] < T) {
-                sp[pl++] = key;
-            } else if (raw[e] == T) {
-                if (pe < need_eq) sp[LT + pe] = key;
-                ++pe;
-            }
+        for (int e = 0; e < EW; ++e) {
+            const int j = lane + kWave * e;
+            const bool real = j < N;
+            const uint64_t eqm = __ballot(real && raw[e] == T);
+            const bool eq = (eqm >> lane) & 1ull;
+            const bool take = (real && raw[e] < T) || (eq && eqseen + mbcnt(eqm) < need_eq);
+            eqseen += __popcll(eqm);
+            const uint64_t tm = __ballot(take);
+            if (take) sp[base + mbcnt(tm)] = ((uint64_t)raw[e] << 32) | (uint32_t)j;
+            base += __popcll(tm);
         }
-        for (int q = nP + t; q < kPrefix; q += 256) sp[q] = ~0ull;
-        __syncthreads();
-        // sort the kPrefix keys, two per thread, in registers
-        static_assert(kPrefix == 512, "two prefix keys per thread");
-        uint64_t pk[2] = {sp[2 * t], sp[2 * t + 1]};
-        sortnet::sort_blocked<uint64_t, 2, 256>(pk, sp, t);
-        uint16_t* si = SI + row * kPrefix;
-        const uint32_t f0 = 2 * t < N ? (uint32_t)(pk[0] & 0xffffu) : 0u;
-        const uint32_t f1 = 2 * t + 1 < N ? (uint32_t)(pk[1] & 0xffffu) : 0u;
-        reinterpret_cast<uint32_t*>(si)[t] = f0 | (f1 << 16);
-        __syncthreads();   // sp, lds and s_thr are rewritten by the next row
+        for (int q = base + lane; q < kPrefix; q += kWave) sp[q] = ~0ull;
+        lds_order();
+        uint64_t pk[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pk[u] = sp[8 * lane + u];
+        lds_order();
+        sortnet::sort_blocked<uint64_t, 8, kWave>(pk, sp, lane);
+        uint32_t f[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) f[u] = 8 * lane + u < N ? (uint32_t)(pk[u] & 0xffffu) : 0u;
+        *reinterpret_cast<uint4*>(SI + row * kPrefix + 8 * lane) =
+            make_uint4(f[0] | (f[1] << 16), f[2] | (f[3] << 16), f[4] | (f[5] << 16),
+                       f[6] | (f[7] << 16));
     }
 }
 
@@ -2479,20 +2465,21 @@ hipError_t launch_build_rows(const double* X, int natoms, int N, long long P, fl
                            P, rows, D);
         return hipGetLastError();
     }
-    // sorted rows: E frames per thread, 256 * E >= N
-    const unsigned g = capped_grid(rows);
+    // sorted rows: one wave per row, EW frames per lane, 64 * EW >= N, four
+    // rows per block
+    const unsigned g = capped_grid((rows + 3) / 4);
     if (N <= 256)
-        hipLaunchKernelGGL(ksg_build_sorted_kernel<1>, dim3(g), dim3(256), 0, s, X, N, P, rows, D,
-                           SD, SM, SI);
+        hipLaunchKernelGGL(ksg_build_sorted_wave_kernel<4>, dim3(g), dim3(256), 0, s, X, N, P, rows,
+                           D, SD, SM, SI);
     else if (N <= 512)
-        hipLaunchKernelGGL(ksg_build_sorted_kernel<2>, dim3(g), dim3(256), 0, s, X, N, P, rows, D,
-                           SD, SM, SI);
+        hipLaunchKernelGGL(ksg_build_sorted_wave_kernel<8>, dim3(g), dim3(256), 0, s, X, N, P, rows,
+                           D, SD, SM, SI);
     else if (N <= 1024)
-        hipLaunchKernelGGL(ksg_build_sorted_kernel<4>, dim3(g), dim3(256), 0, s, X, N, P, rows, D,
-                           SD, SM, SI);
+        hipLaunchKernelGGL(ksg_build_sorted_wave_kernel<16>, dim3(g), dim3(256), 0, s, X, N, P,
+                           rows, D, SD, SM, SI);
     else
-        hipLaunchKernelGGL(ksg_build_sorted_kernel<8>, dim3(g), dim3(256), 0, s, X, N, P, rows, D,
-                           SD, SM, SI);
+        hipLaunchKernelGGL(ksg_build_sorted_wave_kernel<32>, dim3(g), dim3(256), 0, s, X, N, P,
+                           rows, D, SD, SM, SI);
     return hipGetLastError();
 }
 
