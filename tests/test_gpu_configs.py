@@ -129,6 +129,17 @@ def test_dense_1000x10000_te(gpu, dense_traj, oracle_lib, lag):
     d = float(np.max(np.abs(te - want)))
     print(f"TE lag {lag}: max|dTE| {d:.2e}, kernel {ms:.1f} ms", flush=True)
     assert d <= 1e-10
+    # the same pairs through the per-target prefix stores (the automatic
+    # path of the full matrix): bit-identical
+    dense.te_set_path(3)
+    try:
+        te3, ms3 = dense.te_pairs(t, a1, a2, k=100, lag=lag)
+        st = dense.te_stats()
+    finally:
+        dense.te_set_path(0)
+    print(f"TE lag {lag} prefix stores: kernel {ms3:.1f} ms, {st}", flush=True)
+    assert st["prefix_path"] and st["prefix_resolved"] > 0
+    assert np.array_equal(te3, te)
 
 
 def test_surrogate_train_500x4000_hybrid_2000x5000(gpu, oracle_lib):
