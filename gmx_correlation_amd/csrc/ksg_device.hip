@@ -952,30 +952,40 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
     auto side_of = [&](int t) -> bool {
         return ((uint32_t)readlane((int)sidebits, t & (kWave - 1)) >> (t >> 6)) & 1u;
     };
-    // the next row: P values and frame indices, SP[T-1], Q summary, Q row
-    uint4 nv0 = make_uint4(0u, 0u, 0u, 0u), nv1 = nv0, ni = nv0;
+    // A prefetched row: P values and frame indices, SP[T-1], Q summary.  Two
+    // sets, A and B, taken in turn: row t is read from one while row t + 1
+    // loads into the other, so no row's registers are copied.  The Q row
+    // (nq) is staged to LDS before the next one loads: one set.
+    struct NextRow {
+        uint4 v0, v1, i;
+        uint32_t sm, L;
+    };
+    NextRow rA, rB;
+    rA.v0 = make_uint4(0u, 0u, 0u, 0u);
+    rA.v1 = rA.i = rA.v0;
+    rA.sm = rA.L = 0u;
+    rB = rA;
     uint4 nq[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) nq[m] = nv0;
-    uint32_t nsm = 0u, nL = 0u;
+    for (int m = 0; m < 8; ++m) nq[m] = rA.v0;
     // Each load takes the atom's base pointer (selected by the side, held in
     // scalar registers) and a 32-bit byte offset holding the row: one vector
     // add per load instead of 64-bit scalar pointer arithmetic per row (the
     // offsets stay below 2^32: rows < 2048, pitches < 2300 words).
-    auto issue = [&](int t) {
+    auto issue = [&](int t, NextRow& n) {
         const uint32_t row = (uint32_t)(i0 + t * step);   // < N
         const bool sa = side_of(t);
         const uint32_t* sd = sa ? pr.SDa : pr.SDb;
         const uint32_t so = (row * (uint32_t)pr.PS + 64u) * 4u;
-        nv0 = ld_lane<uint4>(sd, so + lane4 * 4u);
-        nv1 = ld_lane<uint4>(sd, so + lane4 * 4u + 1024u);
+        n.v0 = ld_lane<uint4>(sd, so + lane4 * 4u);
+        n.v1 = ld_lane<uint4>(sd, so + lane4 * 4u + 1024u);
         const uint16_t* si = sa ? pr.SIa : pr.SIb;
         const uint32_t io = row * (uint32_t)kPrefix * 2u;
         const uint2 i01 = ld_lane<uint2>(si, io + lane4 * 2u);
         const uint2 i23 = ld_lane<uint2>(si, io + lane4 * 2u + 512u);
-        ni = make_uint4(i01.x, i01.y, i23.x, i23.y);
-        if (lane == 0) nL = ld_lane<uint32_t>(sd, so + (uint32_t)(T - 1) * 4u);
-        nsm = ld_lane<uint32_t>(sa ? pr.SMb : pr.SMa, row * (uint32_t)kSumPitch * 4u + lane4);
+        n.i = make_uint4(i01.x, i01.y, i23.x, i23.y);
+        if (lane == 0) n.L = ld_lane<uint32_t>(sd, so + (uint32_t)(T - 1) * 4u);
+        n.sm = ld_lane<uint32_t>(sa ? pr.SMb : pr.SMa, row * (uint32_t)kSumPitch * 4u + lane4);
         const uint32_t* rq = sa ? pr.Db : pr.Da;
         const uint32_t qo = row * (uint32_t)pr.P * 4u + lane4 * 4u;
         // all 8 chunks: past the row's last chunk the last one again (a
@@ -992,31 +1002,9 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
     int pnp = 0, pc1 = 0;
     uint32_t pexq = 0u, peps = 0u;
     uint2 pblk = make_uint2(0u, 0u);
-    if (R > 0) issue(0);
-    for (int t = 0;; ++t) {
-        const bool have = t < R;
-        asm volatile("" : "+v"(lane4));
-        uint32_t v[8], q[8];
-        uint32_t sm = 0u, L = 0u;
-        bool sideA = false;
-        int row = 0;
-        if (have) {
-            row = i0 + t * step;
-            sideA = side_of(t);
-            v[0] = nv0.x; v[1] = nv0.y; v[2] = nv0.z; v[3] = nv0.w;
-            v[4] = nv1.x; v[5] = nv1.y; v[6] = nv1.z; v[7] = nv1.w;
-            const uint32_t idx[4] = {ni.x, ni.y, ni.z, ni.w};
-            sm = nsm;
-            L = (uint32_t)readlane((int)nL, 0);
-            // stage the Q row (frames 256m + 4l + c at lane l, chunk m)
-#pragma unroll
-            for (int m = 0; m < 8; ++m) reinterpret_cast<uint4*>(lds.qrow)[m * kWave + lane] = nq[m];
-            lds_order();
-#pragma unroll
-            for (int e = 0; e < 8; ++e) q[e] = lds.qrow[(idx[e >> 1] >> (16 * (e & 1))) & 0xffffu];
-            if (t + 1 < R) issue(t + 1);
-        }
-        // ---- previous row: Q count from its block, psi loads ---------------
+    // one row: false to go on, true past the last row
+    // the previous row: Q count from its block, psi loads
+    auto finish_prev = [&]() {
         acc += px + py;
         px = py = 0.0;
         if (prow >= 0) {
@@ -1041,7 +1029,30 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
             }
             prow = -1;
         }
-        if (!have) break;
+    };
+    // one row: false to go on, true past the last row
+    auto one_row = [&](int t, NextRow& cur, NextRow& nxt) -> bool {
+        if (t >= R) {
+            finish_prev();
+            return true;
+        }
+        asm volatile("" : "+v"(lane4));
+        const int row = i0 + t * step;
+        const bool sideA = side_of(t);
+        const uint32_t v[8] = {cur.v0.x, cur.v0.y, cur.v0.z, cur.v0.w,
+                               cur.v1.x, cur.v1.y, cur.v1.z, cur.v1.w};
+        const uint32_t idx[4] = {cur.i.x, cur.i.y, cur.i.z, cur.i.w};
+        const uint32_t sm = cur.sm;
+        const uint32_t L = (uint32_t)readlane((int)cur.L, 0);
+        // stage the Q row (frames 256m + 4l + c at lane l, chunk m)
+#pragma unroll
+        for (int m = 0; m < 8; ++m) reinterpret_cast<uint4*>(lds.qrow)[m * kWave + lane] = nq[m];
+        lds_order();
+        uint32_t q[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) q[e] = lds.qrow[(idx[e >> 1] >> (16 * (e & 1))) & 0xffffu];
+        if (t + 1 < R) issue(t + 1, nxt);
+        finish_prev();
         // ---- histogram of the prefix's joint keys over [L, H) --------------
         const uint32_t H = min((uint32_t)readlane((int)v[7], kWave - 1), kPadBits);
         // smallest sh with ((H - L - 1) >> sh) <= kPBins - 3
@@ -1148,7 +1159,7 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
                     o.rc_flag[off] = 0;
                 }
             }
-            continue;
+            return false;
         }
         // ---- nP from the prefix in registers (epsP < H); Q block load -------
         // one compare per value into a lane mask, counted on the scalar unit
@@ -1167,6 +1178,12 @@ __device__ __forceinline__ double rows_prefix(const PrefixRows& pr, int N, int T
         pside = sideA;
         pexq = ey;
         peps = eps_key;
+        return false;
+    };
+    if (R > 0) issue(0, rA);
+    for (int t = 0;; t += 2) {
+        if (one_row(t, rA, rB)) break;
+        if (one_row(t + 1, rB, rA)) break;
     }
     return acc + (px + py);
 }
